@@ -1,0 +1,162 @@
+/*
+ * inti_amd: C-ABI of the MI355X (gfx950) backend for Inti.jl's corrected
+ * Nystrom operators. Plain pointers and sizes only; every call returns an
+ * int status (INTI_OK = 0) and records a message for inti_last_error().
+ *
+ * Conventions
+ *   - Point sets are SoA: coords[d*n + i], d < dim; normals alike.
+ *   - Helmholtz values and densities are complex, interleaved (re, im)
+ *     doubles: the memory layout of Julia ComplexF64 and C double _Complex.
+ *     Laplace values and densities are real doubles.
+ *   - Matrices are column-major; indices are 0-based (the Julia shim adds 1).
+ *   - Host pointers are never retained past a call; device memory handed out
+ *     by inti_device_alloc is owned by the caller.
+ *   - One context owns one HIP stream; calls on one context are serialised
+ *     by a mutex, so any host thread may call in.
+ *
+ * Each entry point names the Inti.jl interface it replaces (file:line in the
+ * Inti.jl sources).
+ */
+#ifndef INTI_AMD_H
+#define INTI_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct inti_ctx inti_ctx;
+typedef struct inti_quad inti_quad;
+typedef struct inti_op inti_op;
+typedef struct inti_csr inti_csr;
+
+/* status codes */
+#define INTI_OK 0
+#define INTI_ERR_ARG 1         /* invalid argument (null pointer, bad id)   */
+#define INTI_ERR_HIP 2         /* HIP runtime error                          */
+#define INTI_ERR_UNSUPPORTED 3 /* kernel/option not available               */
+#define INTI_ERR_NOMEM 4       /* device allocation failed                   */
+#define INTI_ERR_SIZE 5        /* size mismatch between operands            */
+
+/* PDEs (src/kernels.jl:124 Laplace, :322 Helmholtz) */
+#define INTI_LAPLACE 0
+#define INTI_HELMHOLTZ 1
+
+/* kernels (src/kernels.jl:37-112): single layer, double layer, adjoint
+ * double layer, hypersingular, and the combined fields
+ * c0*SL + c1*DL and c0*ADL + c1*HS evaluated in one pass. */
+#define INTI_SL 0
+#define INTI_DL 1
+#define INTI_ADL 2
+#define INTI_HS 3
+#define INTI_COMB_SD 4
+#define INTI_COMB_AH 5
+
+/* arithmetic of the direct sum */
+#define INTI_FP64 0  /* FP64 throughout                                    */
+#define INTI_MIXED 1 /* FP64 geometry/accumulation, FP32 transcendentals   */
+
+/* ---------------------------------------------------------------- context */
+/* Library version (major*10000 + minor*100 + patch). */
+int inti_version(void);
+
+/* Create a context on a device (module __init__ of the Julia shim). */
+int inti_ctx_create(int device, inti_ctx **out);
+int inti_ctx_destroy(inti_ctx *ctx);
+/* Message of the last failed call on ctx ("" if none). ctx may be NULL for
+ * failures of inti_ctx_create. */
+const char *inti_last_error(const inti_ctx *ctx);
+int inti_ctx_synchronize(inti_ctx *ctx);
+/* The context's HIP stream (hipStream_t), for callers that interleave their
+ * own work. */
+int inti_ctx_stream(inti_ctx *ctx, void **stream);
+/* When on, the main kernel of every apply is bracketed by HIP events on the
+ * context stream and inti_ctx_last_kernel_ms returns its duration. */
+int inti_ctx_set_profiling(inti_ctx *ctx, int on);
+int inti_ctx_last_kernel_ms(inti_ctx *ctx, double *ms);
+
+/* Device buffers for callers keeping operands resident in HBM. */
+int inti_device_alloc(inti_ctx *ctx, size_t bytes, void **dptr);
+int inti_device_free(inti_ctx *ctx, void *dptr);
+int inti_memcpy_h2d(inti_ctx *ctx, void *dst, const void *src, size_t bytes);
+int inti_memcpy_d2h(inti_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* ------------------------------------------------------------- quadrature */
+/* Upload a quadrature (replaces handing Inti.Quadrature's QuadratureNode
+ * vector, src/quadrature.jl:7-11,65-70, to native code as the FMM ext does,
+ * ext/IntiFMM3DExt.jl:14-23). normals may be NULL (point targets). */
+int inti_quad_upload(inti_ctx *ctx, int dim, int64_t n, const double *coords,
+                     const double *normals, const double *weights,
+                     inti_quad **out);
+int inti_quad_destroy(inti_quad *q);
+
+/* --------------------------------------------------------------- operator */
+/* Matrix-free IntegralOperator (src/nystrom.jl:67-89): entries
+ * K(x_i, y_j) * w_j with K chosen by (pde, kind, dim, k). coefs (4 doubles,
+ * c0 re/im, c1 re/im) are read only for the combined kinds. */
+int inti_op_create(inti_ctx *ctx, int pde, int kind, int dim, double k,
+                   const double *coefs, const inti_quad *trg,
+                   const inti_quad *src, int precision, inti_op **out);
+int inti_op_destroy(inti_op *op);
+/* Adds cid * x to every apply (square operators only), e.g. the I/2 of
+ * I/2 + D - i k S (docs/src/examples/helmholtz_scattering.jl:230). */
+int inti_op_set_identity(inti_op *op, double re, double im);
+/* Launch shape: threads per block, targets per thread, source splits
+ * (0 = automatic). Used by the tuner; defaults are sensible. */
+int inti_op_set_launch(inti_op *op, int block, int tpt, int split);
+
+/* y = alpha*(A + dA)*x + beta*y for nrhs columns; host pointers
+ * (mul!(y, A, x, alpha, beta) of LinearMaps, src/api.jl:182-185). */
+int inti_op_apply(inti_op *op, int nrhs, const double *x, double *y,
+                  const double *alpha, const double *beta);
+/* The same with device pointers and leading dimensions (in elements);
+ * asynchronous on the context stream. */
+int inti_op_apply_device(inti_op *op, int nrhs, const double *x, int64_t ldx,
+                         double *y, int64_t ldy, const double *alpha,
+                         const double *beta);
+/* Dense column-major m x n matrix of the operator, written to host memory
+ * (assemble_matrix, src/nystrom.jl:103-124). */
+int inti_op_assemble_dense(inti_op *op, double *out);
+/* Fuse coef * csr into every following apply of op (the sparse correction
+ * of src/api.jl:171-185). Several attachments with the same pattern are
+ * merged into one CSR. */
+int inti_op_attach_correction(inti_op *op, const inti_csr *csr, double coef_re,
+                              double coef_im);
+int inti_op_detach_correction(inti_op *op);
+
+/* ---------------------------------------------------------- DIM correction */
+/* bdim_correction (src/bdim.jl:50-196). opS/opD are the SL/DL operators
+ * (derivative = 0) or ADL/HS operators (derivative = 1) over the same
+ * target and source quadratures; Theta = mu*G + S*B1 - D*B0 is computed
+ * matrix-free on the GPU.
+ *   qtags: nq x ne column-major source node indices per element.
+ *   near_ptr/near_idx: element -> corrected targets (CSR over elements),
+ *     or NULL/NULL for the self pattern (target == source,
+ *     src/quadrature.jl:250-256).
+ *   mu: green multiplier per target (src/nystrom.jl:196-207).
+ *   params: {sources_oversample_factor, sources_radius_multiplier}
+ *     (src/bdim.jl:7-10), NULL for the defaults {3, 1.5}.
+ *   diag (optional, 4 doubles): max R-diagonal ratio, max LS residual,
+ *     max |W|, max |Theta row| (the @debug report of src/bdim.jl:186-192).
+ * Returns the two corrections as CSR (rows = targets, columns = sources). */
+int inti_bdim_build(inti_op *opS, inti_op *opD, int derivative, int nq,
+                    int64_t ne, const int64_t *qtags, const int64_t *near_ptr,
+                    const int64_t *near_idx, const double *mu,
+                    const double *params, inti_csr **dS, inti_csr **dD,
+                    double *diag);
+
+/* --------------------------------------------------------------------- CSR */
+int inti_csr_info(const inti_csr *csr, int64_t *m, int64_t *n, int64_t *nnz,
+                  int *is_complex);
+/* Copy to host: rowptr[m+1], colind[nnz], val[nnz] (x2 if complex). The
+ * Julia shim turns this into SparseMatrixCSC (src/bdim.jl:193-194). */
+int inti_csr_export(const inti_csr *csr, int64_t *rowptr, int64_t *colind,
+                    double *val);
+int inti_csr_destroy(inti_csr *csr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* INTI_AMD_H */

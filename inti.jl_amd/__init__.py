@@ -1,0 +1,34 @@
+"""MI355X backend for Inti.jl's corrected Nystrom operators.
+
+Host-side mirror of the Inti.jl interface for this path (Quadrature,
+Laplace/Helmholtz kernels, IntegralOperator, assemble_matrix,
+bdim_correction, single_double_layer) over the C-ABI library
+libinti_amd.so (hand-written gfx950 HIP kernels). The directory name is not a
+Python identifier; load it with load_package() from __graft_entry__.
+"""
+from . import _lib
+from .api import (CorrectedOperator, adj_double_layer_hypersingular, combined_field,
+                  single_double_layer)
+from .bdim import DimParameters, bdim_correction, etype_to_nearest_points
+from .kernels import (AdjointDoubleLayerKernel, CombinedFieldKernel, DoubleLayerKernel,
+                      Helmholtz, HyperSingularKernel, Laplace, SingleLayerKernel,
+                      default_density_eltype, default_kernel_eltype)
+from .nystrom import IntegralOperator, SparseCorrection, _green_multiplier, assemble_matrix
+from .quadrature import (Quadrature, circle_quadrature, ellipse_quadrature, gauss_legendre,
+                         gauss_legendre_order, points, sphere_quadrature,
+                         vioreanu_rokhlin_triangle)
+
+IntiError = _lib.IntiError
+default_context = _lib.default_context
+Context = _lib.Context
+
+__all__ = [
+    "AdjointDoubleLayerKernel", "CombinedFieldKernel", "Context", "CorrectedOperator",
+    "DimParameters", "DoubleLayerKernel", "Helmholtz", "HyperSingularKernel",
+    "IntegralOperator", "IntiError", "Laplace", "Quadrature", "SingleLayerKernel",
+    "SparseCorrection", "_green_multiplier", "adj_double_layer_hypersingular",
+    "assemble_matrix", "bdim_correction", "circle_quadrature", "combined_field",
+    "default_context", "default_density_eltype", "default_kernel_eltype",
+    "ellipse_quadrature", "etype_to_nearest_points", "gauss_legendre", "gauss_legendre_order",
+    "points", "single_double_layer", "sphere_quadrature", "vioreanu_rokhlin_triangle",
+]

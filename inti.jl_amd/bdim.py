@@ -1,0 +1,80 @@
+"""bdim_correction on the GPU (Inti.jl src/bdim.jl:50-196) and the
+nearest-element target lists of src/quadrature.jl:249-286."""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .kernels import (AdjointDoubleLayerKernel, DoubleLayerKernel, HyperSingularKernel,
+                      SingleLayerKernel)
+from .nystrom import IntegralOperator, SparseCorrection
+
+
+@dataclass
+class DimParameters:
+    """DimParameters (bdim.jl:7-10)."""
+    sources_oversample_factor: float = 3.0
+    sources_radius_multiplier: float = 1.5
+
+
+def etype_to_nearest_points(target, source, maxdist=np.inf):
+    """Element -> target lists (quadrature.jl:249-286), as CSR over elements.
+
+    target is source: every element lists its own nodes (returns None, the
+    self pattern). Otherwise every target goes to the element owning its
+    nearest source node (dropped beyond maxdist); within an element, targets
+    are grouped by node, then in target order."""
+    if target is source:
+        return None
+    from scipy.spatial import cKDTree
+    tree = cKDTree(source.coords.T)
+    d, nearest = tree.query(target.coords.T, k=1)
+    nearest = np.where(d > maxdist, -1, nearest)
+    n = len(source)
+    by_node = [[] for _ in range(n)]
+    for i, j in enumerate(nearest):
+        if j >= 0:
+            by_node[j].append(i)
+    near_ptr = np.zeros(source.ne + 1, dtype=np.int64)
+    near_idx = []
+    for e in range(source.ne):
+        for j in source.qtags[:, e]:
+            near_idx.extend(by_node[j])
+        near_ptr[e + 1] = len(near_idx)
+    return near_ptr, np.asarray(near_idx, dtype=np.int64)
+
+
+def bdim_correction(op, target, source, Sop, Dop, *, green_multiplier, parameters=None,
+                    derivative=False, maxdist=np.inf, near=None, diagnostics=None):
+    """Corrections (dS, dD) such that S + dS and D + dD are accurate
+    (bdim.jl:50-61). Sop/Dop are the GPU IntegralOperators of SL/DL (or
+    ADL/HS if derivative) over (target, source); Theta = mu*G + S*B1 - D*B0 is
+    computed matrix-free on the GPU. Returns two SparseCorrection objects
+    (CSR on the device; .to_scipy() for host copies)."""
+    if not isinstance(Sop, IntegralOperator) or not isinstance(Dop, IntegralOperator):
+        raise TypeError("Sop and Dop must be GPU IntegralOperators over (target, source)")
+    want = (AdjointDoubleLayerKernel, HyperSingularKernel) if derivative else (SingleLayerKernel, DoubleLayerKernel)
+    if not (isinstance(Sop.kernel, want[0]) and isinstance(Dop.kernel, want[1])):
+        raise TypeError("Sop/Dop kernels do not match `derivative`")
+    if Sop.target is not target or Sop.source is not source:
+        raise ValueError("Sop must map source to target")
+    mu = np.ascontiguousarray(green_multiplier, dtype=np.float64)
+    if mu.shape != (len(target),):
+        raise ValueError("green_multiplier must have one entry per target")
+    params = parameters or DimParameters()
+    pv = np.array([params.sources_oversample_factor, params.sources_radius_multiplier])
+    if near is None:
+        near = etype_to_nearest_points(target, source, maxdist)
+    near_ptr, near_idx = (None, None) if near is None else near
+    ctx = Sop.ctx
+    hS, hD = C.c_void_p(), C.c_void_p()
+    diag = np.zeros(4)
+    ctx.check(ctx.lib.inti_bdim_build(
+        Sop.handle, Dop.handle, 1 if derivative else 0, source.nq, source.ne,
+        _lib.i64ptr(np.ascontiguousarray(source.qtags.T.reshape(-1))),  # nq x ne column-major
+        _lib.i64ptr(near_ptr), _lib.i64ptr(near_idx), _lib.dptr(mu), _lib.dptr(pv),
+        C.byref(hS), C.byref(hD), _lib.dptr(diag)))
+    if diagnostics is not None:
+        diagnostics.update(rdiag_ratio=diag[0], max_weight_norm=diag[2], max_theta_norm=diag[3])
+    return SparseCorrection(ctx, hS), SparseCorrection(ctx, hD)

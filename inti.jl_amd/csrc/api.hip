@@ -1,0 +1,496 @@
+// C-ABI entry points of libinti_amd (declared in include/inti_amd.h):
+// context / stream / profiling, quadrature upload, operators, corrections.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "common.hpp"
+
+namespace inti {
+
+static thread_local std::string g_create_err;
+
+int set_error(Ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    else g_create_err = msg;
+    return code;
+}
+
+int ctx_scratch(Ctx *ctx, int slot, size_t bytes, void **ptr) {
+    if (bytes == 0) bytes = 256;
+    if (ctx->scratch_bytes[slot] < bytes) {
+        if (ctx->scratch[slot]) {
+            INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            INTI_HIP(ctx, hipFree(ctx->scratch[slot]));
+            ctx->scratch[slot] = nullptr;
+            ctx->scratch_bytes[slot] = 0;
+        }
+        size_t want = bytes + bytes / 4;
+        if (hipMalloc(&ctx->scratch[slot], want) != hipSuccess) {
+            ctx->scratch[slot] = nullptr;
+            return set_error(ctx, INTI_ERR_NOMEM, "scratch allocation failed");
+        }
+        ctx->scratch_bytes[slot] = want;
+    }
+    *ptr = ctx->scratch[slot];
+    return INTI_OK;
+}
+
+int csr_alloc(Ctx *ctx, int64_t m, int64_t n, int64_t nnz, int cplx, Csr **out) {
+    Csr *c = new inti_csr();
+    c->ctx = ctx; c->m = m; c->n = n; c->nnz = nnz; c->is_complex = cplx;
+    if (hipMalloc(&c->rowptr, sizeof(int32_t) * (m + 1)) != hipSuccess ||
+        hipMalloc(&c->colind, sizeof(int32_t) * (nnz > 0 ? nnz : 1)) != hipSuccess ||
+        hipMalloc(&c->val, sizeof(double) * (cplx ? 2 : 1) * (nnz > 0 ? nnz : 1)) != hipSuccess) {
+        if (c->rowptr) (void)hipFree(c->rowptr);
+        if (c->colind) (void)hipFree(c->colind);
+        if (c->val) (void)hipFree(c->val);
+        delete static_cast<inti_csr *>(c);
+        return set_error(ctx, INTI_ERR_NOMEM, "CSR allocation failed");
+    }
+    *out = c;
+    return INTI_OK;
+}
+
+void csr_free(Csr *c) {
+    if (!c) return;
+    if (c->rowptr) (void)hipFree(c->rowptr);
+    if (c->colind) (void)hipFree(c->colind);
+    if (c->val) (void)hipFree(c->val);
+    delete static_cast<inti_csr *>(c);
+}
+
+int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *qtags,
+               const int64_t *near_ptr, const int64_t *near_idx, const double *mu,
+               const double *params, Csr **outS, Csr **outD, double *diag_out);
+
+}  // namespace inti
+
+using namespace inti;
+
+#define LOCK(ctx) std::lock_guard<std::mutex> lock_((ctx)->mu)
+#define REQUIRE(ctx, cond, msg) \
+    do { if (!(cond)) return set_error((ctx), INTI_ERR_ARG, (msg)); } while (0)
+
+extern "C" {
+
+int inti_version(void) { return 100; }
+
+int inti_ctx_create(int device, inti_ctx **out) {
+    if (!out) return set_error(nullptr, INTI_ERR_ARG, "out is null");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return set_error(nullptr, INTI_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= count) return set_error(nullptr, INTI_ERR_ARG, "device out of range");
+    if (hipSetDevice(device) != hipSuccess) return set_error(nullptr, INTI_ERR_HIP, "hipSetDevice failed");
+    inti_ctx *c = new inti_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return set_error(nullptr, INTI_ERR_HIP, "stream/event creation failed");
+    }
+    *out = c;
+    return INTI_OK;
+}
+
+int inti_ctx_destroy(inti_ctx *ctx) {
+    if (!ctx) return INTI_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (int s = 0; s < 4; ++s)
+        if (ctx->scratch[s]) (void)hipFree(ctx->scratch[s]);
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return INTI_OK;
+}
+
+const char *inti_last_error(const inti_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+int inti_ctx_synchronize(inti_ctx *ctx) {
+    if (!ctx) return INTI_ERR_ARG;
+    INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return INTI_OK;
+}
+
+int inti_ctx_stream(inti_ctx *ctx, void **stream) {
+    if (!ctx || !stream) return INTI_ERR_ARG;
+    *stream = (void *)ctx->stream;
+    return INTI_OK;
+}
+
+int inti_ctx_set_profiling(inti_ctx *ctx, int on) {
+    if (!ctx) return INTI_ERR_ARG;
+    ctx->profiling = on ? 1 : 0;
+    return INTI_OK;
+}
+
+int inti_ctx_last_kernel_ms(inti_ctx *ctx, double *ms) {
+    if (!ctx || !ms) return INTI_ERR_ARG;
+    LOCK(ctx);
+    INTI_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    float t = 0.f;
+    INTI_HIP(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    *ms = t;
+    return INTI_OK;
+}
+
+int inti_device_alloc(inti_ctx *ctx, size_t bytes, void **dptr) {
+    if (!ctx || !dptr) return INTI_ERR_ARG;
+    if (hipMalloc(dptr, bytes ? bytes : 1) != hipSuccess)
+        return set_error(ctx, INTI_ERR_NOMEM, "device allocation failed");
+    return INTI_OK;
+}
+
+int inti_device_free(inti_ctx *ctx, void *dptr) {
+    if (!ctx) return INTI_ERR_ARG;
+    LOCK(ctx);
+    INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    INTI_HIP(ctx, hipFree(dptr));
+    return INTI_OK;
+}
+
+int inti_memcpy_h2d(inti_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return INTI_ERR_ARG;
+    LOCK(ctx);
+    INTI_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return INTI_OK;
+}
+
+int inti_memcpy_d2h(inti_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return INTI_ERR_ARG;
+    LOCK(ctx);
+    INTI_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return INTI_OK;
+}
+
+// ------------------------------------------------------------- quadrature
+int inti_quad_upload(inti_ctx *ctx, int dim, int64_t n, const double *coords,
+                     const double *normals, const double *weights, inti_quad **out) {
+    if (!ctx) return INTI_ERR_ARG;
+    REQUIRE(ctx, out != nullptr, "quad_upload: out is null");
+    REQUIRE(ctx, dim == 2 || dim == 3, "quad_upload: dim must be 2 or 3");
+    REQUIRE(ctx, n >= 0, "quad_upload: negative size");
+    REQUIRE(ctx, n == 0 || (coords && weights), "quad_upload: coords and weights are required");
+    LOCK(ctx);
+    inti_quad *q = new inti_quad();
+    q->ctx = ctx; q->dim = dim; q->n = n;
+    const size_t cb = sizeof(double) * dim * (n > 0 ? n : 1);
+    if (hipMalloc(&q->coords, cb) != hipSuccess ||
+        hipMalloc(&q->weights, sizeof(double) * (n > 0 ? n : 1)) != hipSuccess ||
+        (normals && hipMalloc(&q->normals, cb) != hipSuccess)) {
+        if (q->coords) (void)hipFree(q->coords);
+        if (q->weights) (void)hipFree(q->weights);
+        if (q->normals) (void)hipFree(q->normals);
+        delete q;
+        return set_error(ctx, INTI_ERR_NOMEM, "quad_upload: allocation failed");
+    }
+    if (n > 0) {
+        INTI_HIP(ctx, hipMemcpyAsync(q->coords, coords, sizeof(double) * dim * n, hipMemcpyHostToDevice, ctx->stream));
+        INTI_HIP(ctx, hipMemcpyAsync(q->weights, weights, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+        if (normals)
+            INTI_HIP(ctx, hipMemcpyAsync(q->normals, normals, sizeof(double) * dim * n, hipMemcpyHostToDevice, ctx->stream));
+        for (int d = 0; d < dim; ++d) {
+            double lo = coords[(int64_t)d * n], hi = lo;
+            for (int64_t i = 1; i < n; ++i) {
+                lo = std::min(lo, coords[(int64_t)d * n + i]);
+                hi = std::max(hi, coords[(int64_t)d * n + i]);
+            }
+            q->lo[d] = lo; q->hi[d] = hi;
+        }
+        INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    *out = q;
+    return INTI_OK;
+}
+
+int inti_quad_destroy(inti_quad *q) {
+    if (!q) return INTI_ERR_ARG;
+    Ctx *ctx = q->ctx;
+    {
+        LOCK(ctx);
+        (void)hipStreamSynchronize(ctx->stream);
+        if (q->coords) (void)hipFree(q->coords);
+        if (q->weights) (void)hipFree(q->weights);
+        if (q->normals) (void)hipFree(q->normals);
+    }
+    delete q;
+    return INTI_OK;
+}
+
+// --------------------------------------------------------------- operator
+int inti_op_create(inti_ctx *ctx, int pde, int kind, int dim, double k, const double *coefs,
+                   const inti_quad *trg, const inti_quad *src, int precision, inti_op **out) {
+    if (!ctx) return INTI_ERR_ARG;
+    REQUIRE(ctx, out && trg && src, "op_create: null argument");
+    REQUIRE(ctx, pde == INTI_LAPLACE || pde == INTI_HELMHOLTZ, "op_create: unknown pde");
+    REQUIRE(ctx, kind >= INTI_SL && kind <= INTI_COMB_AH, "op_create: unknown kernel kind");
+    REQUIRE(ctx, dim == 2 || dim == 3, "op_create: dim must be 2 or 3");
+    REQUIRE(ctx, precision == INTI_FP64 || precision == INTI_MIXED, "op_create: unknown precision");
+    // IntegralOperator(k, X, Y) requires equal dimensions (src/nystrom.jl:79-85)
+    if (trg->dim != dim || src->dim != dim)
+        return set_error(ctx, INTI_ERR_SIZE, "op_create: quadrature dimension differs from the operator's");
+    const bool needs_src_n = kind != INTI_SL && kind != INTI_ADL;
+    const bool needs_trg_n = kind == INTI_ADL || kind == INTI_HS || kind == INTI_COMB_AH;
+    if (needs_src_n && !src->normals)
+        return set_error(ctx, INTI_ERR_ARG, "op_create: this kernel needs source normals");
+    if (needs_trg_n && !trg->normals)
+        return set_error(ctx, INTI_ERR_ARG, "op_create: this kernel needs target normals");
+    if (pde == INTI_HELMHOLTZ) REQUIRE(ctx, k > 0.0 && std::isfinite(k), "op_create: Helmholtz needs a real k > 0");
+    const bool comb = kind == INTI_COMB_SD || kind == INTI_COMB_AH;
+    if (comb) {
+        REQUIRE(ctx, coefs != nullptr, "op_create: combined kinds need coefs");
+        if (pde == INTI_LAPLACE && (coefs[1] != 0.0 || coefs[3] != 0.0))
+            return set_error(ctx, INTI_ERR_UNSUPPORTED, "op_create: Laplace combinations take real coefficients");
+    }
+    inti_op *op = new inti_op();
+    op->ctx = ctx; op->pde = pde; op->kind = kind; op->dim = dim; op->k = k;
+    op->precision = precision; op->trg = trg; op->src = src;
+    if (comb) std::memcpy(op->coefs, coefs, sizeof(double) * 4);
+    *out = op;
+    return INTI_OK;
+}
+
+int inti_op_destroy(inti_op *op) {
+    if (!op) return INTI_ERR_ARG;
+    {
+        LOCK(op->ctx);
+        (void)hipStreamSynchronize(op->ctx->stream);
+        csr_free(op->corr);
+    }
+    delete op;
+    return INTI_OK;
+}
+
+int inti_op_set_identity(inti_op *op, double re, double im) {
+    if (!op) return INTI_ERR_ARG;
+    if (op->trg->n != op->src->n)
+        return set_error(op->ctx, INTI_ERR_SIZE, "op_set_identity: operator is not square");
+    if (op->pde == INTI_LAPLACE && im != 0.0)
+        return set_error(op->ctx, INTI_ERR_UNSUPPORTED, "op_set_identity: Laplace takes a real coefficient");
+    op->cid[0] = re; op->cid[1] = im;
+    return INTI_OK;
+}
+
+int inti_op_set_launch(inti_op *op, int block, int tpt, int split) {
+    if (!op) return INTI_ERR_ARG;
+    if (block != 256 || tpt != 1 || split < 0)
+        return set_error(op->ctx, INTI_ERR_UNSUPPORTED, "op_set_launch: block 256, tpt 1, split >= 0");
+    op->block = block; op->tpt = tpt; op->split = split;
+    return INTI_OK;
+}
+
+static int apply_device_locked(inti_op *op, int nrhs, const double *x, int64_t ldx, double *y,
+                               int64_t ldy, const double *alpha, const double *beta) {
+    DirectLaunch L;
+    L.op = op; L.nrhs = nrhs; L.x = x; L.ldx = ldx; L.y = y; L.ldy = ldy;
+    L.alpha[0] = alpha ? alpha[0] : 1.0; L.alpha[1] = alpha ? alpha[1] : 0.0;
+    L.beta[0] = beta ? beta[0] : 0.0; L.beta[1] = beta ? beta[1] : 0.0;
+    if (op->pde == INTI_LAPLACE && (L.alpha[1] != 0.0 || L.beta[1] != 0.0))
+        return set_error(op->ctx, INTI_ERR_UNSUPPORTED, "apply: Laplace operators take real alpha/beta");
+    return launch_apply(L);
+}
+
+int inti_op_apply_device(inti_op *op, int nrhs, const double *x, int64_t ldx, double *y,
+                         int64_t ldy, const double *alpha, const double *beta) {
+    if (!op) return INTI_ERR_ARG;
+    Ctx *ctx = op->ctx;
+    REQUIRE(ctx, nrhs >= 0, "apply: nrhs < 0");
+    REQUIRE(ctx, nrhs == 0 || (x && y), "apply: null operand");
+    REQUIRE(ctx, ldx >= op->src->n && ldy >= op->trg->n, "apply: leading dimension too small");
+    LOCK(ctx);
+    return apply_device_locked(op, nrhs, x, ldx, y, ldy, alpha, beta);
+}
+
+int inti_op_apply(inti_op *op, int nrhs, const double *x, double *y, const double *alpha,
+                  const double *beta) {
+    if (!op) return INTI_ERR_ARG;
+    Ctx *ctx = op->ctx;
+    REQUIRE(ctx, nrhs >= 0, "apply: nrhs < 0");
+    REQUIRE(ctx, nrhs == 0 || (x && y), "apply: null operand");
+    if (nrhs == 0) return INTI_OK;
+    LOCK(ctx);
+    const int cw = op_is_complex(op) ? 2 : 1;
+    const int64_t m = op->trg->n, n = op->src->n;
+    void *dx = nullptr, *dy = nullptr;
+    int st = ctx_scratch(ctx, kScrX, sizeof(double) * cw * n * nrhs, &dx);
+    if (st) return st;
+    st = ctx_scratch(ctx, kScrY, sizeof(double) * cw * m * nrhs, &dy);
+    if (st) return st;
+    INTI_HIP(ctx, hipMemcpyAsync(dx, x, sizeof(double) * cw * n * nrhs, hipMemcpyHostToDevice, ctx->stream));
+    const bool beta_nz = beta && (beta[0] != 0.0 || beta[1] != 0.0);
+    if (beta_nz)
+        INTI_HIP(ctx, hipMemcpyAsync(dy, y, sizeof(double) * cw * m * nrhs, hipMemcpyHostToDevice, ctx->stream));
+    st = apply_device_locked(op, nrhs, (const double *)dx, n, (double *)dy, m, alpha, beta);
+    if (st) return st;
+    INTI_HIP(ctx, hipMemcpyAsync(y, dy, sizeof(double) * cw * m * nrhs, hipMemcpyDeviceToHost, ctx->stream));
+    INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return INTI_OK;
+}
+
+int inti_op_assemble_dense(inti_op *op, double *out) {
+    if (!op) return INTI_ERR_ARG;
+    Ctx *ctx = op->ctx;
+    REQUIRE(ctx, out != nullptr, "assemble: out is null");
+    LOCK(ctx);
+    const int cw = op_is_complex(op) ? 2 : 1;
+    const size_t bytes = sizeof(double) * cw * (size_t)op->trg->n * op->src->n;
+    if (bytes == 0) return INTI_OK;
+    void *d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) return set_error(ctx, INTI_ERR_NOMEM, "assemble: allocation failed");
+    int st = launch_assemble(op, (double *)d);
+    if (st == INTI_OK) {
+        if (hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            st = set_error(ctx, INTI_ERR_HIP, "assemble: copy back failed");
+    }
+    (void)hipFree(d);
+    return st;
+}
+
+// merged correction: c_new = c_old + coef * csr (same pattern), or a copy
+__global__ void k_axpy_vals(int64_t nnz, int cplx, const double *src, double cr, double ci,
+                            double *dst, int accumulate) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= nnz) return;
+    if (cplx) {
+        const double vr = src[2 * q], vi = src[2 * q + 1];
+        double yr = cr * vr - ci * vi, yi = cr * vi + ci * vr;
+        if (accumulate) { yr += dst[2 * q]; yi += dst[2 * q + 1]; }
+        dst[2 * q] = yr; dst[2 * q + 1] = yi;
+    } else {
+        double yr = cr * src[q];
+        if (accumulate) yr += dst[q];
+        dst[q] = yr;
+    }
+}
+
+int inti_op_attach_correction(inti_op *op, const inti_csr *csr, double coef_re, double coef_im) {
+    if (!op) return INTI_ERR_ARG;
+    Ctx *ctx = op->ctx;
+    REQUIRE(ctx, csr != nullptr, "attach: csr is null");
+    if (csr->m != op->trg->n || csr->n != op->src->n)
+        return set_error(ctx, INTI_ERR_SIZE, "attach: correction shape differs from the operator's");
+    if (csr->is_complex != (op_is_complex(op) ? 1 : 0))
+        return set_error(ctx, INTI_ERR_ARG, "attach: correction value type differs from the operator's");
+    if (!csr->is_complex && coef_im != 0.0)
+        return set_error(ctx, INTI_ERR_UNSUPPORTED, "attach: real correction takes a real coefficient");
+    LOCK(ctx);
+    const int cplx = csr->is_complex;
+    int accumulate = 0;
+    if (op->corr) {
+        // same pattern required for an in-place merge
+        bool same = op->corr->nnz == csr->nnz;
+        if (same && csr->nnz > 0) {
+            std::vector<int32_t> a(csr->m + 1 + csr->nnz), b(a.size());
+            INTI_HIP(ctx, hipMemcpy(a.data(), op->corr->rowptr, sizeof(int32_t) * (csr->m + 1), hipMemcpyDeviceToHost));
+            INTI_HIP(ctx, hipMemcpy(a.data() + csr->m + 1, op->corr->colind, sizeof(int32_t) * csr->nnz, hipMemcpyDeviceToHost));
+            INTI_HIP(ctx, hipMemcpy(b.data(), csr->rowptr, sizeof(int32_t) * (csr->m + 1), hipMemcpyDeviceToHost));
+            INTI_HIP(ctx, hipMemcpy(b.data() + csr->m + 1, csr->colind, sizeof(int32_t) * csr->nnz, hipMemcpyDeviceToHost));
+            same = a == b;
+        }
+        if (!same)
+            return set_error(ctx, INTI_ERR_UNSUPPORTED, "attach: corrections with different patterns cannot be merged");
+        accumulate = 1;
+    } else {
+        Csr *c = nullptr;
+        int st = csr_alloc(ctx, csr->m, csr->n, csr->nnz, cplx, &c);
+        if (st) return st;
+        INTI_HIP(ctx, hipMemcpyAsync(c->rowptr, csr->rowptr, sizeof(int32_t) * (csr->m + 1), hipMemcpyDeviceToDevice, ctx->stream));
+        if (csr->nnz > 0)
+            INTI_HIP(ctx, hipMemcpyAsync(c->colind, csr->colind, sizeof(int32_t) * csr->nnz, hipMemcpyDeviceToDevice, ctx->stream));
+        op->corr = c;
+    }
+    if (csr->nnz > 0) {
+        hipLaunchKernelGGL(k_axpy_vals, dim3((unsigned)((csr->nnz + 255) / 256)), dim3(256), 0, ctx->stream,
+                           csr->nnz, cplx, csr->val, coef_re, coef_im, op->corr->val, accumulate);
+        INTI_HIP(ctx, hipGetLastError());
+    }
+    INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return INTI_OK;
+}
+
+int inti_op_detach_correction(inti_op *op) {
+    if (!op) return INTI_ERR_ARG;
+    LOCK(op->ctx);
+    (void)hipStreamSynchronize(op->ctx->stream);
+    csr_free(op->corr);
+    op->corr = nullptr;
+    return INTI_OK;
+}
+
+// ---------------------------------------------------------- DIM correction
+int inti_bdim_build(inti_op *opS, inti_op *opD, int derivative, int nq, int64_t ne,
+                    const int64_t *qtags, const int64_t *near_ptr, const int64_t *near_idx,
+                    const double *mu, const double *params, inti_csr **dS, inti_csr **dD,
+                    double *diag) {
+    if (!opS) return INTI_ERR_ARG;
+    Ctx *ctx = opS->ctx;
+    REQUIRE(ctx, opD && qtags && mu && dS && dD, "bdim: null argument");
+    REQUIRE(ctx, opD->ctx == ctx, "bdim: operators on different contexts");
+    REQUIRE(ctx, opS->trg == opD->trg && opS->src == opD->src, "bdim: S and D must share target and source");
+    REQUIRE(ctx, opS->pde == opD->pde && opS->dim == opD->dim && opS->k == opD->k,
+            "bdim: S and D must be the same PDE");
+    const int kS = derivative ? INTI_ADL : INTI_SL, kD = derivative ? INTI_HS : INTI_DL;
+    REQUIRE(ctx, opS->kind == kS && opD->kind == kD,
+            "bdim: expects (SL, DL) operators, or (ADL, HS) with derivative = 1");
+    REQUIRE(ctx, nq > 0 && ne > 0, "bdim: empty element set");
+    REQUIRE(ctx, (near_ptr == nullptr) == (near_idx == nullptr), "bdim: near_ptr and near_idx go together");
+    REQUIRE(ctx, params == nullptr || (params[0] > 0 && params[1] > 0), "bdim: bad parameters");
+    LOCK(ctx);
+    Csr *s = nullptr, *d = nullptr;
+    int st = bdim_build(opS, opD, derivative, nq, ne, qtags, near_ptr, near_idx, mu, params, &s, &d, diag);
+    if (st) return st;
+    *dS = static_cast<inti_csr *>(s);
+    *dD = static_cast<inti_csr *>(d);
+    return INTI_OK;
+}
+
+// --------------------------------------------------------------------- CSR
+int inti_csr_info(const inti_csr *csr, int64_t *m, int64_t *n, int64_t *nnz, int *is_complex) {
+    if (!csr) return INTI_ERR_ARG;
+    if (m) *m = csr->m;
+    if (n) *n = csr->n;
+    if (nnz) *nnz = csr->nnz;
+    if (is_complex) *is_complex = csr->is_complex;
+    return INTI_OK;
+}
+
+int inti_csr_export(const inti_csr *csr, int64_t *rowptr, int64_t *colind, double *val) {
+    if (!csr) return INTI_ERR_ARG;
+    Ctx *ctx = csr->ctx;
+    LOCK(ctx);
+    std::vector<int32_t> rp(csr->m + 1), ci(csr->nnz);
+    INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    INTI_HIP(ctx, hipMemcpy(rp.data(), csr->rowptr, sizeof(int32_t) * (csr->m + 1), hipMemcpyDeviceToHost));
+    if (csr->nnz > 0) {
+        INTI_HIP(ctx, hipMemcpy(ci.data(), csr->colind, sizeof(int32_t) * csr->nnz, hipMemcpyDeviceToHost));
+        if (val)
+            INTI_HIP(ctx, hipMemcpy(val, csr->val, sizeof(double) * (csr->is_complex ? 2 : 1) * csr->nnz,
+                                    hipMemcpyDeviceToHost));
+    }
+    if (rowptr) for (int64_t i = 0; i <= csr->m; ++i) rowptr[i] = rp[i];
+    if (colind) for (int64_t p = 0; p < csr->nnz; ++p) colind[p] = ci[p];
+    return INTI_OK;
+}
+
+int inti_csr_destroy(inti_csr *csr) {
+    if (!csr) return INTI_ERR_ARG;
+    Ctx *ctx = csr->ctx;
+    {
+        LOCK(ctx);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    csr_free(csr);
+    return INTI_OK;
+}
+
+}  // extern "C"

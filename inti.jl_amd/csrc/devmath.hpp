@@ -1,0 +1,165 @@
+// Device math for the pair kernels: FP64 reciprocal / inverse square root
+// refined from the hardware estimates, the MIXED variants that use the FP32
+// transcendental unit, phase reduction in FP64, and the Hankel functions
+// H0^(1), H1^(1) of real argument (series + rational modulus/phase fits,
+// coefficients from gen_hankel.py).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hankel_coeffs.h"
+
+namespace inti {
+
+constexpr double kTwoOverPi = 0.63661977236758134308;
+constexpr double kSqrt2OverPi = 0.79788456080286535588;  // sqrt(2/pi)
+constexpr double kEulerGamma = 0.57721566490153286061;
+constexpr double kQuarterPi = 0.78539816339744830962;
+constexpr double kLn2 = 0.69314718055994530942;
+
+// 1/x in FP64: v_rcp_f64 estimate + two Newton steps (error ~1 ulp).
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// 1/sqrt(x) in FP64: v_rsq_f64 estimate + two Newton steps.
+__device__ __forceinline__ double rsq64(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double h = __builtin_fma(-(x * y), y, 1.0);
+    y = __builtin_fma(0.5 * y, h, y);
+    h = __builtin_fma(-(x * y), y, 1.0);
+    return __builtin_fma(0.5 * y, h, y);
+}
+
+// FP32 hardware estimate refined once in FP64: ~1e-14 relative, enough for
+// distances that enter a phase k*d.
+__device__ __forceinline__ double rsq_mixed_phase(double x) {
+    double y = (double)__builtin_amdgcn_rsqf((float)x);
+    double h = __builtin_fma(-(x * y), y, 1.0);
+    return __builtin_fma(0.5 * y, h, y);
+}
+
+template <bool MIXED>
+__device__ __forceinline__ double rsqv(double x) {
+    if constexpr (MIXED) return (double)__builtin_amdgcn_rsqf((float)x);
+    else return rsq64(x);
+}
+// inverse square root feeding a phase
+template <bool MIXED>
+__device__ __forceinline__ double rsqp(double x) {
+    if constexpr (MIXED) return rsq_mixed_phase(x);
+    else return rsq64(x);
+}
+template <bool MIXED>
+__device__ __forceinline__ double rcpv(double x) {
+    if constexpr (MIXED) return (double)__builtin_amdgcn_rcpf((float)x);
+    else return rcp64(x);
+}
+// natural logarithm
+template <bool MIXED>
+__device__ __forceinline__ double logv(double x) {
+    if constexpr (MIXED) return (double)(__builtin_amdgcn_logf((float)x) * 0.69314718f);
+    else return log(x);
+}
+
+// (sin t, cos t). MIXED: t is reduced to revolutions in FP64 (exact to
+// ~1e-16 absolute) and fed to the FP32 sin/cos unit, whose argument is in
+// revolutions.
+template <bool MIXED>
+__device__ __forceinline__ void sincosv(double t, double &s, double &c) {
+    if constexpr (MIXED) {
+        double u = t * 0.15915494309189533577;
+        double f = u - __builtin_rint(u);
+        float ff = (float)f;
+        s = (double)__builtin_amdgcn_sinf(ff);
+        c = (double)__builtin_amdgcn_cosf(ff);
+    } else {
+        sincos(t, &s, &c);
+    }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ T horner(const T (&c)[N], T u) {
+    T s = c[N - 1];
+#pragma unroll
+    for (int i = N - 2; i >= 0; --i) s = __builtin_fma(s, u, c[i]);
+    return s;
+}
+
+// H0^(1)(x) and H1^(1)(x), x > 0, FP64.
+__device__ __forceinline__ void hankel01_f64(double x, double &h0r, double &h0i,
+                                             double &h1r, double &h1i) {
+    if (x < HK_F64_XS) {
+        double u = 0.25 * x * x;
+        double a0 = horner(HK_F64_A0, u), a1 = horner(HK_F64_A1, u);
+        double b0 = horner(HK_F64_B0, u), b1 = horner(HK_F64_B1, u);
+        double L = log(0.5 * x);
+        double ix = rcp64(x);
+        double J0 = a0, J1 = 0.5 * x * a1;
+        h0r = J0;
+        h0i = kTwoOverPi * ((L + kEulerGamma) * J0 + b0);
+        h1r = J1;
+        h1i = -kTwoOverPi * ix + kTwoOverPi * L * J1 - (0.5 / kPi) * x * b1;
+    } else {
+        double ix = rcp64(x);
+        double t = (HK_F64_XS * HK_F64_XS) * ix * ix;
+        double P0 = horner(HK_F64_P0_NUM, t) * rcp64(horner(HK_F64_P0_DEN, t));
+        double Q0 = horner(HK_F64_Q0_NUM, t) * rcp64(horner(HK_F64_Q0_DEN, t)) * ix;
+        double P1 = horner(HK_F64_P1_NUM, t) * rcp64(horner(HK_F64_P1_DEN, t));
+        double Q1 = horner(HK_F64_Q1_NUM, t) * rcp64(horner(HK_F64_Q1_DEN, t)) * ix;
+        double amp = kSqrt2OverPi * rsq64(x);
+        double s, c;
+        sincos(x - kQuarterPi, &s, &c);
+        // H0 = amp (P0 + i Q0)(c + i s); H1 = amp (P1 + i Q1)(s - i c)
+        h0r = amp * (P0 * c - Q0 * s);
+        h0i = amp * (P0 * s + Q0 * c);
+        h1r = amp * (P1 * s + Q1 * c);
+        h1i = amp * (Q1 * s - P1 * c);
+    }
+}
+
+// MIXED: FP32 series/rationals and the FP32 sin/cos unit; the phase
+// x - pi/4 is reduced in FP64 first.
+__device__ __forceinline__ void hankel01_mixed(double xd, double &h0r, double &h0i,
+                                               double &h1r, double &h1i) {
+    float x = (float)xd;
+    if (x < HK_F32_XS) {
+        float u = 0.25f * x * x;
+        float a0 = horner(HK_F32_A0, u), a1 = horner(HK_F32_A1, u);
+        float b0 = horner(HK_F32_B0, u), b1 = horner(HK_F32_B1, u);
+        float L = __builtin_amdgcn_logf(0.5f * x) * 0.69314718f;
+        float ix = __builtin_amdgcn_rcpf(x);
+        float J0 = a0, J1 = 0.5f * x * a1;
+        h0r = J0;
+        h0i = 0.63661977f * ((L + 0.57721566f) * J0 + b0);
+        h1r = J1;
+        h1i = -0.63661977f * ix + 0.63661977f * L * J1 - 0.15915494f * x * b1;
+    } else {
+        float ix = __builtin_amdgcn_rcpf(x);
+        float t = (HK_F32_XS * HK_F32_XS) * ix * ix;
+        float P0 = horner(HK_F32_P0_NUM, t) * __builtin_amdgcn_rcpf(horner(HK_F32_P0_DEN, t));
+        float Q0 = horner(HK_F32_Q0_NUM, t) * __builtin_amdgcn_rcpf(horner(HK_F32_Q0_DEN, t)) * ix;
+        float P1 = horner(HK_F32_P1_NUM, t) * __builtin_amdgcn_rcpf(horner(HK_F32_P1_DEN, t));
+        float Q1 = horner(HK_F32_Q1_NUM, t) * __builtin_amdgcn_rcpf(horner(HK_F32_Q1_DEN, t)) * ix;
+        float amp = 0.79788456f * __builtin_amdgcn_rsqf(x);
+        double u = xd * 0.15915494309189533577 - 0.125;
+        float f = (float)(u - __builtin_rint(u));
+        float s = __builtin_amdgcn_sinf(f), c = __builtin_amdgcn_cosf(f);
+        h0r = amp * (P0 * c - Q0 * s);
+        h0i = amp * (P0 * s + Q0 * c);
+        h1r = amp * (P1 * s + Q1 * c);
+        h1i = amp * (Q1 * s - P1 * c);
+    }
+}
+
+template <bool MIXED>
+__device__ __forceinline__ void hankel01(double x, double &h0r, double &h0i, double &h1r,
+                                         double &h1i) {
+    if constexpr (MIXED) hankel01_mixed(x, h0r, h0i, h1r, h1i);
+    else hankel01_f64(x, h0r, h0i, h1r, h1i);
+}
+
+}  // namespace inti

@@ -1,0 +1,162 @@
+"""IntegralOperator / assemble_matrix on the GPU (Inti.jl src/nystrom.jl).
+
+An IntegralOperator is matrix-free: entries K(x_i, y_j) * w_j
+(nystrom.jl:93-96) are recomputed by the direct-sum kernel at every apply.
+A sparse correction (from bdim_correction) can be attached, which fuses
+y += coef * dA x into the same apply.
+"""
+import ctypes as C
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from .kernels import CombinedFieldKernel, Helmholtz
+
+
+def _green_multiplier(s):
+    """-1 inside, 0 outside, -1/2 on (nystrom.jl:196-207)."""
+    if s == "inside":
+        return -1.0
+    if s == "outside":
+        return 0.0
+    if s == "on":
+        return -0.5
+    raise ValueError(f"Unknown target location {s}. Expected :inside, :outside, or :on")
+
+
+class SparseCorrection:
+    """A CSR correction held on the device (rows = targets)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.handle = handle
+        m, n, nnz, cp = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int()
+        ctx.check(ctx.lib.inti_csr_info(handle, C.byref(m), C.byref(n), C.byref(nnz), C.byref(cp)))
+        self.shape = (m.value, n.value)
+        self.nnz = nnz.value
+        self.is_complex = bool(cp.value)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self.ctx.lib.inti_csr_destroy(self.handle)
+        except Exception:
+            pass
+
+    def export(self):
+        """(rowptr, colind, values) as numpy arrays, 0-based CSR."""
+        m, n = self.shape
+        rowptr = np.zeros(m + 1, dtype=np.int64)
+        colind = np.zeros(self.nnz, dtype=np.int64)
+        val = np.zeros(self.nnz, dtype=np.complex128 if self.is_complex else np.float64)
+        self.ctx.check(self.ctx.lib.inti_csr_export(
+            self.handle, _lib.i64ptr(rowptr), _lib.i64ptr(colind), _lib.dptr(val)))
+        return rowptr, colind, val
+
+    def to_scipy(self):
+        rowptr, colind, val = self.export()
+        return sp.csr_matrix((val, colind, rowptr), shape=self.shape)
+
+
+class IntegralOperator:
+    """IntegralOperator(kernel, target, source=target) (nystrom.jl:67-89).
+
+    precision: "fp64" (default) or "mixed" (FP32 transcendental unit, FP64
+    geometry and accumulation)."""
+
+    def __init__(self, kernel, target, source=None, precision="fp64", ctx=None):
+        source = target if source is None else source
+        if target.dim != source.dim:
+            raise ValueError("target and source must have the same ambient dimension")
+        if kernel.op.dim != source.dim:
+            raise ValueError("kernel dimension differs from the quadrature's")
+        self.kernel = kernel
+        self.target = target
+        self.source = source
+        self.precision = precision
+        self.ctx = ctx or _lib.default_context()
+        self._h = None
+        self._corr = []  # keep attached corrections alive
+        self.identity = 0.0
+
+    @property
+    def shape(self):
+        return (len(self.target), len(self.source))
+
+    @property
+    def dtype(self):
+        return np.complex128 if self.kernel.is_complex else np.float64
+
+    @property
+    def handle(self):
+        if self._h is None:
+            k = self.kernel
+            coefs = None
+            if isinstance(k, CombinedFieldKernel):
+                coefs = np.array([k.c0.real, k.c0.imag, k.c1.real, k.c1.imag])
+            h = C.c_void_p()
+            prec = _lib.MIXED if self.precision == "mixed" else _lib.FP64
+            self.ctx.check(self.ctx.lib.inti_op_create(
+                self.ctx.handle, k.op.pde, k.kind, k.op.dim, k.op.k,
+                _lib.dptr(coefs), self.target.device(self.ctx), self.source.device(self.ctx),
+                prec, C.byref(h)))
+            self._h = h
+        return self._h
+
+    def __del__(self):
+        try:
+            if self._h is not None:
+                self.ctx.lib.inti_op_destroy(self._h)
+        except Exception:
+            pass
+
+    def set_identity(self, c):
+        c = complex(c)
+        self.ctx.check(self.ctx.lib.inti_op_set_identity(self.handle, c.real, c.imag))
+        self.identity = c
+
+    def attach(self, corr, coef=1.0):
+        """Fuse coef * corr (a SparseCorrection) into every apply."""
+        coef = complex(coef)
+        self.ctx.check(self.ctx.lib.inti_op_attach_correction(self.handle, corr.handle,
+                                                              coef.real, coef.imag))
+        self._corr.append(corr)
+
+    def detach(self):
+        self.ctx.check(self.ctx.lib.inti_op_detach_correction(self.handle))
+        self._corr = []
+
+    def apply(self, x, y=None, alpha=1.0, beta=0.0):
+        """y = alpha * A x + beta * y; x is (n,) or (n, nrhs)."""
+        x = np.asarray(x)
+        vec = x.ndim == 1
+        X = x.reshape(len(self.source), -1)
+        nrhs = X.shape[1]
+        Xc = np.ascontiguousarray(X.T, dtype=self.dtype)  # column-major n x nrhs
+        if y is None:
+            Yc = np.zeros((nrhs, len(self.target)), dtype=self.dtype)
+        else:
+            Yc = np.ascontiguousarray(np.asarray(y).reshape(len(self.target), -1).T, dtype=self.dtype)
+        a = np.array([complex(alpha).real, complex(alpha).imag])
+        b = np.array([complex(beta).real, complex(beta).imag])
+        self.ctx.check(self.ctx.lib.inti_op_apply(self.handle, nrhs, _lib.dptr(Xc), _lib.dptr(Yc),
+                                                  _lib.dptr(a), _lib.dptr(b)))
+        out = Yc.T
+        return out.reshape(-1) if vec else np.ascontiguousarray(out)
+
+    def __matmul__(self, x):
+        return self.apply(x)
+
+
+def assemble_matrix(iop):
+    """Dense matrix of an IntegralOperator (nystrom.jl:103-124); the entries
+    are computed on the GPU (FP64)."""
+    m, n = iop.shape
+    out = np.zeros((n, m), dtype=iop.dtype)  # column-major m x n
+    iop.ctx.check(iop.ctx.lib.inti_op_assemble_dense(iop.handle, _lib.dptr(out)))
+    return out.T
+
+
+def is_helmholtz(op):
+    return isinstance(op, Helmholtz)

@@ -1,0 +1,187 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the CPU
+baseline leg of bench.py as the checker. The product never imports it.
+
+All values are returned as complex128 (Laplace values have zero imaginary
+part). Point sets are passed as objects with .coords (dim, n), .normals
+(dim, n) or None, .weights (n,), .qtags (nq, ne).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            subprocess.run(["make", "-s", "-C", HERE], check=True)
+        lib = C.CDLL(LIB)
+        lib.ork_kernel.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp]
+        lib.ork_hankel1.argtypes = [C.c_int, C.c_double, _dp]
+        lib.ork_assemble.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int64, _dp, _dp,
+                                     C.c_int64, _dp, _dp, _dp, _dp]
+        lib.ork_matvec.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int64, _dp, _dp,
+                                   C.c_int64, _dp, _dp, _dp, C.c_int, _dp, _dp, C.c_int64,
+                                   C.c_int64, C.c_int]
+        lib.ork_dim_monopoles.restype = C.c_int
+        lib.ork_dim_monopoles.argtypes = [C.c_int, C.c_int64, _dp, C.c_int, C.c_double, C.c_double, _dp]
+        lib.ork_bdim.restype = C.c_int64
+        lib.ork_bdim.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int64, _dp, _dp,
+                                 C.c_int64, _dp, _dp, _dp, C.c_int, C.c_int64, _i64p, _i64p, _i64p,
+                                 _dp, C.c_double, C.c_double, _i64p, _i64p, _dp, _dp, _dp]
+        lib.ork_nearest_lists.restype = C.c_int64
+        lib.ork_nearest_lists.argtypes = [C.c_int, C.c_int64, _dp, C.c_int64, _dp, C.c_int, C.c_int64,
+                                          _i64p, C.c_double, _i64p, _i64p]
+        lib.ork_csr_matvec.argtypes = [C.c_int64, C.c_int64, _i64p, _i64p, _dp, C.c_int, _dp, _dp,
+                                       _dp, _dp]
+        _lib = lib
+    return _lib
+
+
+def _d(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_i64p)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _coefs(coefs):
+    c = np.zeros(4) if coefs is None else np.asarray(
+        [complex(coefs[0]).real, complex(coefs[0]).imag, complex(coefs[1]).real,
+         complex(coefs[1]).imag], dtype=np.float64)
+    return c
+
+
+def hankel1(n, x):
+    out = np.zeros(2)
+    _load().ork_hankel1(n, float(x), _d(out))
+    return complex(out[0], out[1])
+
+
+def kernel(pde, dim, kind, k, x, nx, y, ny, coefs=None):
+    out = np.zeros(2)
+    c = _coefs(coefs)
+    _load().ork_kernel(pde, dim, kind, k, _d(c), _d(_f64(x)), _d(_f64(nx)), _d(_f64(y)),
+                       _d(_f64(ny)), _d(out))
+    return complex(out[0], out[1])
+
+
+def assemble(pde, dim, kind, k, trg, src, coefs=None):
+    m, n = trg.coords.shape[1], src.coords.shape[1]
+    out = np.zeros((n, m), dtype=np.complex128)
+    c = _coefs(coefs)
+    _load().ork_assemble(pde, dim, kind, k, _d(c), m, _d(_f64(trg.coords)), _d(_f64(trg.normals)),
+                         n, _d(_f64(src.coords)), _d(_f64(src.normals)), _d(_f64(src.weights)),
+                         _d(out.view(np.float64)))
+    return out.T
+
+
+def matvec(pde, dim, kind, k, trg, src, sigma, coefs=None, rows=None, nthreads=0):
+    """Direct sum y = K sigma for rows [lo, hi) (others are zero)."""
+    m, n = trg.coords.shape[1], src.coords.shape[1]
+    s = np.asarray(sigma, dtype=np.complex128)
+    vec = s.ndim == 1
+    S = s.reshape(n, -1)
+    nrhs = S.shape[1]
+    X = np.ascontiguousarray(S.T)
+    Y = np.zeros((nrhs, m), dtype=np.complex128)
+    lo, hi = (0, m) if rows is None else rows
+    c = _coefs(coefs)
+    _load().ork_matvec(pde, dim, kind, k, _d(c), m, _d(_f64(trg.coords)), _d(_f64(trg.normals)),
+                       n, _d(_f64(src.coords)), _d(_f64(src.normals)), _d(_f64(src.weights)),
+                       nrhs, _d(X.view(np.float64)), _d(Y.view(np.float64)), lo, hi, nthreads)
+    out = Y.T
+    return out.reshape(-1) if vec else out
+
+
+def monopoles(dim, src_coords, nq, oversample=3.0, radius_mult=1.5):
+    ns = int(np.ceil(oversample * nq))
+    zs = np.zeros((dim, ns))
+    n = src_coords.shape[1]
+    _load().ork_dim_monopoles(dim, n, _d(_f64(src_coords)), nq, oversample, radius_mult, _d(zs))
+    return zs
+
+
+def bdim(pde, dim, k, trg, src, mu, derivative=False, near=None, oversample=3.0,
+         radius_mult=1.5):
+    """CSR (rowptr, colind, valS, valD) and diagnostics of bdim_correction."""
+    m, n = trg.coords.shape[1], src.coords.shape[1]
+    nq, ne = src.qtags.shape
+    qt = np.ascontiguousarray(src.qtags.T.reshape(-1), dtype=np.int64)
+    if near is None:
+        cnt = nq * ne
+        near_ptr = near_idx = None
+    else:
+        near_ptr = np.ascontiguousarray(near[0], dtype=np.int64)
+        near_idx = np.ascontiguousarray(near[1], dtype=np.int64)
+        cnt = int(near_ptr[-1])
+    nnz_max = max(cnt * nq, 1)
+    rowptr = np.zeros(m + 1, dtype=np.int64)
+    colind = np.zeros(nnz_max, dtype=np.int64)
+    valS = np.zeros(nnz_max, dtype=np.complex128)
+    valD = np.zeros(nnz_max, dtype=np.complex128)
+    diag = np.zeros(4)
+    nnz = _load().ork_bdim(pde, dim, k, 1 if derivative else 0, m, _d(_f64(trg.coords)),
+                           _d(_f64(trg.normals)), n, _d(_f64(src.coords)), _d(_f64(src.normals)),
+                           _d(_f64(src.weights)), nq, ne, _i(qt), _i(near_ptr), _i(near_idx),
+                           _d(_f64(mu)), oversample, radius_mult, _i(rowptr), _i(colind),
+                           _d(valS.view(np.float64)), _d(valD.view(np.float64)), _d(diag))
+    if nnz < 0:
+        raise ValueError("oracle bdim rejected its arguments")
+    return rowptr, colind[:nnz], valS[:nnz], valD[:nnz], diag
+
+
+def nearest_lists(trg, src, maxdist=np.inf):
+    m, n = trg.coords.shape[1], src.coords.shape[1]
+    nq, ne = src.qtags.shape
+    qt = np.ascontiguousarray(src.qtags.T.reshape(-1), dtype=np.int64)
+    near_ptr = np.zeros(ne + 1, dtype=np.int64)
+    near_idx = np.zeros(max(m, 1), dtype=np.int64)
+    cnt = _load().ork_nearest_lists(trg.coords.shape[0], m, _d(_f64(trg.coords)), n,
+                                    _d(_f64(src.coords)), nq, ne, _i(qt), float(maxdist),
+                                    _i(near_ptr), _i(near_idx))
+    return near_ptr, near_idx[:cnt]
+
+
+def csr_matvec(m, n, rowptr, colind, val, x):
+    x = np.asarray(x, dtype=np.complex128)
+    vec = x.ndim == 1
+    X = np.ascontiguousarray(x.reshape(n, -1).T)
+    Y = np.zeros((X.shape[0], m), dtype=np.complex128)
+    a = np.array([1.0, 0.0])
+    b = np.array([0.0, 0.0])
+    vv = np.ascontiguousarray(val, dtype=np.complex128)
+    _load().ork_csr_matvec(m, n, _i(np.ascontiguousarray(rowptr, dtype=np.int64)),
+                           _i(np.ascontiguousarray(colind, dtype=np.int64)), _d(vv.view(np.float64)),
+                           X.shape[0], _d(X.view(np.float64)), _d(a), _d(b), _d(Y.view(np.float64)))
+    out = Y.T
+    return out.reshape(-1) if vec else out
+
+
+def corrected_matvec(op_pde, dim, kind, k, quad, sigma, mu, which="D", coefs=None, nthreads=0):
+    """(A + dA) sigma on one surface, A the SL (which='S') or DL ('D')
+    operator, dA from the oracle's bdim_correction (self pattern)."""
+    rowptr, colind, vS, vD, _ = bdim(op_pde, dim, k, quad, quad, mu)
+    y = matvec(op_pde, dim, kind, k, quad, quad, sigma, coefs=coefs, nthreads=nthreads)
+    val = vD if which == "D" else vS
+    return y + csr_matvec(len(quad.weights), len(quad.weights), rowptr, colind, val, sigma)

@@ -1,0 +1,511 @@
+/*
+ * CPU FP64 restatement of Inti.jl's corrected Nystrom hot path.
+ * TEST INFRASTRUCTURE ONLY: the checker for the GPU library, never shipped
+ * or called by it. Each function cites the reference lines it follows.
+ */
+#include "inti_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double _Complex cplx;
+
+/* src/utils.jl:235 */
+static const double SAME_POINT_TOLERANCE = 1.0e-14;
+static const double PI = 3.14159265358979323846;
+
+/* ------------------------------------------------------------------------ */
+/* Hankel functions of the first kind (the reference calls Bessels.hankelh1,
+ * src/kernels.jl:344). Here: C library j0/j1/jn/y0/y1/yn.                 */
+void ork_hankel1(int n, double x, double out[2]) {
+    double j, y;
+    if (n == 0) { j = j0(x); y = y0(x); }
+    else if (n == 1) { j = j1(x); y = y1(x); }
+    else { j = jn(n, x); y = yn(n, x); }
+    out[0] = j; out[1] = y;
+}
+
+static inline cplx hankel(int n, double x) {
+    double h[2];
+    ork_hankel1(n, x, h);
+    return h[0] + I * h[1];
+}
+
+static inline double dotn(int dim, const double *a, const double *b) {
+    double s = 0.0;
+    for (int d = 0; d < dim; ++d) s += a[d] * b[d];
+    return s;
+}
+
+/* One kernel, point coordinates packed (x[dim]). kernels.jl:137-204,347-413 */
+static cplx kernel_pt(int pde, int dim, int kind, double k, const double *x,
+                      const double *nx, const double *y, const double *ny) {
+    double r[3] = {0, 0, 0};
+    for (int d = 0; d < dim; ++d) r[d] = x[d] - y[d];
+    double d2 = dotn(dim, r, r);
+    double d = sqrt(d2);
+    if (d <= SAME_POINT_TOLERANCE) return 0.0;
+    if (pde == ORK_LAPLACE) {
+        if (kind == ORK_SL) {
+            return dim == 2 ? -1.0 / (2 * PI) * log(d) : 1.0 / (4 * PI) / d;
+        } else if (kind == ORK_DL) {
+            double rdn = dotn(dim, r, ny);
+            return dim == 2 ? 1.0 / (2 * PI) / (d * d) * rdn
+                            : 1.0 / (4 * PI) / (d * d * d) * rdn;
+        } else if (kind == ORK_ADL) {
+            double rdn = dotn(dim, r, nx);
+            return dim == 2 ? -1.0 / (2 * PI) / (d * d) * rdn
+                            : -1.0 / (4 * PI) / (d * d * d) * rdn;
+        } else { /* HS */
+            double id2 = 1.0 / (d * d);
+            double nxdny = dotn(dim, nx, ny);
+            double rdnx = dotn(dim, r, nx), rdny = dotn(dim, r, ny);
+            if (dim == 2) return id2 / (2 * PI) * (nxdny - 2 * rdnx * rdny * id2);
+            return id2 / (4 * PI * d) * (nxdny - 3 * rdnx * rdny * id2);
+        }
+    }
+    /* Helmholtz */
+    if (dim == 2) {
+        double kd = k * d;
+        if (kind == ORK_SL) return I / 4.0 * hankel(0, kd);
+        if (kind == ORK_DL) return I * k / (4 * d) * hankel(1, kd) * dotn(dim, r, ny);
+        if (kind == ORK_ADL) return -I * k / (4 * d) * hankel(1, kd) * dotn(dim, r, nx);
+        {
+            double rdnx = dotn(dim, r, nx), rdny = dotn(dim, r, ny);
+            double nxdny = dotn(dim, nx, ny);
+            cplx h1 = hankel(1, kd), h2 = hankel(2, kd);
+            cplx a = -I * k * k / (4 * d * d) * h2;
+            cplx b = I * k / (4 * d) * h1;
+            return a * rdnx * rdny + b * nxdny;
+        }
+    }
+    {
+        cplx eikd = cexp(I * k * d);
+        if (kind == ORK_SL) return 1.0 / (4 * PI) / d * eikd;
+        if (kind == ORK_DL)
+            return 1.0 / (4 * PI * d * d) * eikd * (-I * k + 1.0 / d) * dotn(dim, r, ny);
+        if (kind == ORK_ADL)
+            return -1.0 / (4 * PI * d * d) * eikd * (-I * k + 1.0 / d) * dotn(dim, r, nx);
+        {
+            double rdnx = dotn(dim, r, nx), rdny = dotn(dim, r, ny);
+            double nxdny = dotn(dim, nx, ny);
+            double id2 = 1.0 / (d * d);
+            cplx b = eikd * id2 / (4 * PI) * (-I * k + 1.0 / d);
+            cplx a = eikd * id2 * id2 / (4 * PI * d) * (3.0 * (d * I * k - 1.0) + d * d * k * k);
+            return a * rdnx * rdny + b * nxdny;
+        }
+    }
+}
+
+static cplx kernel_any(int pde, int dim, int kind, double k, const double coefs[4],
+                       const double *x, const double *nx, const double *y,
+                       const double *ny) {
+    if (kind == ORK_COMB_SD || kind == ORK_COMB_AH) {
+        int ka = kind == ORK_COMB_SD ? ORK_SL : ORK_ADL;
+        int kb = kind == ORK_COMB_SD ? ORK_DL : ORK_HS;
+        cplx c0 = coefs[0] + I * coefs[1], c1 = coefs[2] + I * coefs[3];
+        return c0 * kernel_pt(pde, dim, ka, k, x, nx, y, ny) +
+               c1 * kernel_pt(pde, dim, kb, k, x, nx, y, ny);
+    }
+    return kernel_pt(pde, dim, kind, k, x, nx, y, ny);
+}
+
+static inline void gather(int dim, const double *soa, int64_t n, int64_t i, double *p) {
+    for (int d = 0; d < dim; ++d) p[d] = soa ? soa[d * n + i] : 0.0;
+}
+
+void ork_kernel(int pde, int dim, int kind, double k, const double coefs[4],
+                const double *x, const double *nx, const double *y,
+                const double *ny, double out[2]) {
+    static const double zero3[3] = {0, 0, 0};
+    cplx v = kernel_any(pde, dim, kind, k, coefs, x, nx ? nx : zero3, y, ny ? ny : zero3);
+    out[0] = creal(v); out[1] = cimag(v);
+}
+
+/* nystrom.jl:117-124: column-threaded dense fill, out[i,j] = K(X[i],Y[j]) w_j */
+void ork_assemble(int pde, int dim, int kind, double k, const double coefs[4],
+                  int64_t m, const double *tx, const double *tn,
+                  int64_t n, const double *sx, const double *sn, const double *sw,
+                  double *out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < n; ++j) {
+        double y[3], ny[3], x[3], nx[3];
+        gather(dim, sx, n, j, y);
+        gather(dim, sn, n, j, ny);
+        for (int64_t i = 0; i < m; ++i) {
+            gather(dim, tx, m, i, x);
+            gather(dim, tn, m, i, nx);
+            cplx v = kernel_any(pde, dim, kind, k, coefs, x, nx, y, ny) * sw[j];
+            out[2 * (i + m * j)] = creal(v);
+            out[2 * (i + m * j) + 1] = cimag(v);
+        }
+    }
+}
+
+/* Matrix-free counterpart of S*x (nystrom.jl:93-96 entries), rows threaded. */
+void ork_matvec(int pde, int dim, int kind, double k, const double coefs[4],
+                int64_t m, const double *tx, const double *tn,
+                int64_t n, const double *sx, const double *sn, const double *sw,
+                int nrhs, const double *xin, double *yout,
+                int64_t row_lo, int64_t row_hi, int nthreads) {
+#ifdef _OPENMP
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#else
+    int nt = 1; (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
+    for (int64_t i = row_lo; i < row_hi; ++i) {
+        double x[3], nx[3], y[3], ny[3];
+        gather(dim, tx, m, i, x);
+        gather(dim, tn, m, i, nx);
+        cplx acc[64];
+        int r0 = 0;
+        while (r0 < nrhs) {
+            int nb = nrhs - r0 < 64 ? nrhs - r0 : 64;
+            for (int r = 0; r < nb; ++r) acc[r] = 0.0;
+            for (int64_t j = 0; j < n; ++j) {
+                gather(dim, sx, n, j, y);
+                gather(dim, sn, n, j, ny);
+                cplx kv = kernel_any(pde, dim, kind, k, coefs, x, nx, y, ny) * sw[j];
+                for (int r = 0; r < nb; ++r) {
+                    const double *xv = xin + 2 * (j + n * (int64_t)(r0 + r));
+                    acc[r] += kv * (xv[0] + I * xv[1]);
+                }
+            }
+            for (int r = 0; r < nb; ++r) {
+                double *yv = yout + 2 * (i + m * (int64_t)(r0 + r));
+                yv[0] = creal(acc[r]); yv[1] = cimag(acc[r]);
+            }
+            r0 += nb;
+        }
+    }
+}
+
+/* utils.jl:142-149 and utils.jl:158-171; centre and radius per bdim.jl:79-92 */
+int ork_dim_monopoles(int dim, int64_t n, const double *sx, int nq,
+                      double oversample, double radius_mult, double *zs) {
+    int ns = (int)ceil(oversample * nq);
+    double lo[3], hi[3];
+    for (int d = 0; d < dim; ++d) {
+        lo[d] = INFINITY; hi[d] = -INFINITY;
+        for (int64_t j = 0; j < n; ++j) {
+            double v = sx[d * n + j];
+            lo[d] = v < lo[d] ? v : lo[d];
+            hi[d] = v > hi[d] ? v : hi[d];
+        }
+    }
+    double xc[3], diag2 = 0.0;
+    for (int d = 0; d < dim; ++d) {
+        xc[d] = (lo[d] + hi[d]) / 2;
+        diag2 += (hi[d] - lo[d]) * (hi[d] - lo[d]);
+    }
+    double R = radius_mult * sqrt(diag2) / 2;
+    if (dim == 2) {
+        for (int i = 0; i < ns; ++i) {
+            double th = 2 * PI * i / ns;
+            zs[0 * ns + i] = R * cos(th) + xc[0];
+            zs[1 * ns + i] = R * sin(th) + xc[1];
+        }
+    } else {
+        double phi = PI * (3 - sqrt(5.0));
+        for (int i = 1; i <= ns; ++i) {
+            double ytmp = 1 - ((double)(i - 1) / (ns - 1)) * 2;
+            double radius = sqrt(1 - ytmp * ytmp);
+            double theta = phi * i;
+            zs[0 * ns + i - 1] = cos(theta) * radius * R + xc[0];
+            zs[1 * ns + i - 1] = ytmp * R + xc[1];
+            zs[2 * ns + i - 1] = sin(theta) * radius * R + xc[2];
+        }
+    }
+    return ns;
+}
+
+/* Householder reflector, the generic unblocked QR path taken by
+ * qr!(transpose(M)) in bdim.jl:163: nu = copysign(|x|, re x1),
+ * x1 += nu, x[1] = -nu, x[2:] /= x1, tau = x1/nu.                       */
+static cplx reflector(cplx *x, int len) {
+    double s = 0.0;
+    for (int i = 0; i < len; ++i) s += creal(x[i]) * creal(x[i]) + cimag(x[i]) * cimag(x[i]);
+    double normu = sqrt(s);
+    if (normu == 0.0) return 0.0;
+    cplx xi1 = x[0];
+    double nu = copysign(normu, creal(xi1));
+    xi1 += nu;
+    x[0] = -nu;
+    for (int i = 1; i < len; ++i) x[i] /= xi1;
+    return xi1 / nu;
+}
+
+/* A is rows x cols, column-major (lda = rows). In-place QR, tau[cols]. */
+static void qr_unblocked(cplx *A, int rows, int cols, cplx *tau) {
+    for (int kk = 0; kk < cols && kk < rows; ++kk) {
+        cplx *x = A + kk + (int64_t)rows * kk;
+        int len = rows - kk;
+        cplx t = reflector(x, len);
+        tau[kk] = t;
+        for (int j = kk + 1; j < cols; ++j) {
+            cplx *Aj = A + kk + (int64_t)rows * j;
+            cplx v = Aj[0];
+            for (int i = 1; i < len; ++i) v += conj(x[i]) * Aj[i];
+            v = conj(t) * v;
+            Aj[0] -= v;
+            for (int i = 1; i < len; ++i) Aj[i] -= x[i] * v;
+        }
+    }
+}
+
+/* Least-squares solve with the packed QR: b <- Q^H b, then R x = b[0:cols]
+ * by column-oriented back substitution (the generic QR ldiv! path). */
+static void qr_solve(const cplx *A, int rows, int cols, const cplx *tau, cplx *b) {
+    for (int kk = 0; kk < cols && kk < rows; ++kk) {
+        const cplx *x = A + kk + (int64_t)rows * kk;
+        cplx v = b[kk];
+        for (int i = kk + 1; i < rows; ++i) v += conj(x[i - kk]) * b[i];
+        v = conj(tau[kk]) * v;
+        b[kk] -= v;
+        for (int i = kk + 1; i < rows; ++i) b[i] -= x[i - kk] * v;
+    }
+    for (int j = cols - 1; j >= 0; --j) {
+        cplx xj = b[j] / A[j + (int64_t)rows * j];
+        b[j] = xj;
+        for (int i = j - 1; i >= 0; --i) b[i] -= A[i + (int64_t)rows * j] * xj;
+    }
+}
+
+typedef struct { int64_t row, col; cplx s, d; } triplet;
+
+static int cmp_triplet(const void *a, const void *b) {
+    const triplet *p = (const triplet *)a, *q = (const triplet *)b;
+    if (p->row != q->row) return p->row < q->row ? -1 : 1;
+    if (p->col != q->col) return p->col < q->col ? -1 : 1;
+    return 0;
+}
+
+/* bdim.jl:50-196 */
+int64_t ork_bdim(int pde, int dim, double k, int derivative,
+                 int64_t m, const double *tx, const double *tn,
+                 int64_t n, const double *sx, const double *sn, const double *sw,
+                 int nq, int64_t ne, const int64_t *qtags,
+                 const int64_t *near_ptr, const int64_t *near_idx,
+                 const double *mu, double oversample, double radius_mult,
+                 int64_t *rowptr, int64_t *colind, double *valS, double *valD,
+                 double *diag) {
+    const int self = near_ptr == NULL;
+    if (self && m != n) return -1;
+    int ns = (int)ceil(oversample * nq);
+    if (ns > 64 || ns < 2 * nq) return -1;
+    double *zs = (double *)malloc(sizeof(double) * dim * ns);
+    ork_dim_monopoles(dim, n, sx, nq, oversample, radius_mult, zs);
+    const int kG = derivative ? ORK_ADL : ORK_SL; /* bdim.jl:112 */
+    const int kS = derivative ? ORK_ADL : ORK_SL;
+    const int kD = derivative ? ORK_HS : ORK_DL;
+
+    /* traces (bdim.jl:94-103): B0[j,k] = G(y_j, z_k), B1[j,k] = ADL(y_j, z_k) */
+    cplx *B0 = (cplx *)malloc(sizeof(cplx) * n * ns);
+    cplx *B1 = (cplx *)malloc(sizeof(cplx) * n * ns);
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < n; ++j) {
+        double y[3], ny[3], z[3];
+        gather(dim, sx, n, j, y);
+        gather(dim, sn, n, j, ny);
+        for (int kk = 0; kk < ns; ++kk) {
+            gather(dim, zs, ns, kk, z);
+            B0[j + n * kk] = kernel_pt(pde, dim, ORK_SL, k, y, ny, z, NULL);
+            B1[j + n * kk] = kernel_pt(pde, dim, ORK_ADL, k, y, ny, z, NULL);
+        }
+    }
+
+    /* targets needing a correction, in element order */
+    int64_t nlisted = 0;
+    if (self) nlisted = n;
+    else nlisted = near_ptr[ne];
+    /* local row of each global target (-1 = not corrected) */
+    int64_t *loc = (int64_t *)malloc(sizeof(int64_t) * (m > 0 ? m : 1));
+    for (int64_t i = 0; i < m; ++i) loc[i] = -1;
+    int64_t *glob = (int64_t *)malloc(sizeof(int64_t) * (nlisted > 0 ? nlisted : 1));
+    int64_t nloc = 0;
+    for (int64_t e = 0; e < ne; ++e) {
+        int64_t cnt = self ? nq : near_ptr[e + 1] - near_ptr[e];
+        for (int64_t t = 0; t < cnt; ++t) {
+            int64_t i = self ? qtags[t + (int64_t)nq * e] : near_idx[near_ptr[e] + t];
+            if (loc[i] < 0) { loc[i] = nloc; glob[nloc++] = i; }
+        }
+    }
+
+    /* Theta (bdim.jl:106-118) for the listed targets:
+     * Theta = mu*G(x,z) + S*B1 - D*B0 */
+    cplx *Th = (cplx *)malloc(sizeof(cplx) * (nloc > 0 ? nloc : 1) * ns);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t l = 0; l < nloc; ++l) {
+        int64_t i = glob[l];
+        double x[3], nx[3], y[3], ny[3], z[3];
+        gather(dim, tx, m, i, x);
+        gather(dim, tn, m, i, nx);
+        cplx acc[64];
+        for (int kk = 0; kk < ns; ++kk) {
+            gather(dim, zs, ns, kk, z);
+            acc[kk] = mu[i] * kernel_pt(pde, dim, kG, k, x, nx, z, NULL);
+        }
+        cplx sacc[64], dacc[64];
+        for (int kk = 0; kk < ns; ++kk) { sacc[kk] = 0; dacc[kk] = 0; }
+        for (int64_t j = 0; j < n; ++j) {
+            gather(dim, sx, n, j, y);
+            gather(dim, sn, n, j, ny);
+            cplx Sij = kernel_pt(pde, dim, kS, k, x, nx, y, ny) * sw[j];
+            cplx Dij = kernel_pt(pde, dim, kD, k, x, nx, y, ny) * sw[j];
+            for (int kk = 0; kk < ns; ++kk) {
+                sacc[kk] += Sij * B1[j + n * kk];
+                dacc[kk] += Dij * B0[j + n * kk];
+            }
+        }
+        for (int kk = 0; kk < ns; ++kk) Th[l + nloc * kk] = acc[kk] + sacc[kk] - dacc[kk];
+    }
+
+    /* per-element least squares (bdim.jl:137-185) */
+    const int nc = 2 * nq;
+    int64_t nnz = 0;
+    for (int64_t e = 0; e < ne; ++e) nnz += (int64_t)nq * (self ? nq : near_ptr[e + 1] - near_ptr[e]);
+    triplet *trip = (triplet *)malloc(sizeof(triplet) * (nnz > 0 ? nnz : 1));
+    double dg[4] = {0, 0, 0, 0};
+#pragma omp parallel
+    {
+        cplx *A = (cplx *)malloc(sizeof(cplx) * ns * nc);
+        cplx *tau = (cplx *)malloc(sizeof(cplx) * nc);
+        cplx *b = (cplx *)malloc(sizeof(cplx) * ns);
+        double ldg[4] = {0, 0, 0, 0};
+#pragma omp for schedule(dynamic, 8)
+        for (int64_t e = 0; e < ne; ++e) {
+            int64_t cnt = self ? nq : near_ptr[e + 1] - near_ptr[e];
+            if (cnt == 0) continue;
+            const int64_t *jg = qtags + (int64_t)nq * e;
+            /* A = transpose(M), M = [B0[jg,:]; B1[jg,:]] (2nq x ns) */
+            for (int kk = 0; kk < ns; ++kk)
+                for (int r = 0; r < nq; ++r) {
+                    A[kk + ns * r] = B0[jg[r] + n * kk];
+                    A[kk + ns * (nq + r)] = B1[jg[r] + n * kk];
+                }
+            qr_unblocked(A, ns, nc, tau);
+            double rmax = 0, rmin = INFINITY;
+            for (int r = 0; r < nc; ++r) {
+                double a = cabs(A[r + ns * r]);
+                rmax = a > rmax ? a : rmax;
+                rmin = a < rmin ? a : rmin;
+            }
+            if (rmax / rmin > ldg[0]) ldg[0] = rmax / rmin;
+            for (int64_t t = 0; t < cnt; ++t) {
+                int64_t i = self ? jg[t] : near_idx[near_ptr[e] + t];
+                int64_t l = loc[i];
+                double bn = 0;
+                for (int kk = 0; kk < ns; ++kk) {
+                    b[kk] = Th[l + nloc * kk];
+                    bn += creal(b[kk]) * creal(b[kk]) + cimag(b[kk]) * cimag(b[kk]);
+                }
+                if (sqrt(bn) > ldg[3]) ldg[3] = sqrt(bn);
+                qr_solve(A, ns, nc, tau, b);
+                double wn = 0;
+                for (int r = 0; r < nc; ++r) wn += creal(b[r]) * creal(b[r]) + cimag(b[r]) * cimag(b[r]);
+                if (sqrt(wn) > ldg[2]) ldg[2] = sqrt(wn);
+                int64_t slot0 = 0;
+                if (self) slot0 = ((int64_t)nq * e + t) * nq;
+                else slot0 = (near_ptr[e] + t) * nq;
+                for (int r = 0; r < nq; ++r) {
+                    triplet *tp = trip + slot0 + r;
+                    tp->row = i;
+                    tp->col = jg[r];
+                    tp->s = -b[nq + r]; /* bdim.jl:179 */
+                    tp->d = b[r];       /* bdim.jl:180 */
+                }
+            }
+        }
+#pragma omp critical
+        {
+            for (int q = 0; q < 4; ++q) dg[q] = ldg[q] > dg[q] ? ldg[q] : dg[q];
+        }
+        free(A); free(tau); free(b);
+    }
+    /* sparse(Is, Js, V, m, n) semantics, emitted as CSR (sorted, summed) */
+    qsort(trip, nnz, sizeof(triplet), cmp_triplet);
+    int64_t out = 0;
+    for (int64_t i = 0; i <= m; ++i) rowptr[i] = 0;
+    for (int64_t p = 0; p < nnz; ++p) {
+        if (out > 0 && trip[p].row == trip[out - 1].row && trip[p].col == trip[out - 1].col) {
+            trip[out - 1].s += trip[p].s;
+            trip[out - 1].d += trip[p].d;
+        } else {
+            trip[out++] = trip[p];
+        }
+    }
+    for (int64_t p = 0; p < out; ++p) {
+        rowptr[trip[p].row + 1]++;
+        colind[p] = trip[p].col;
+        valS[2 * p] = creal(trip[p].s); valS[2 * p + 1] = cimag(trip[p].s);
+        valD[2 * p] = creal(trip[p].d); valD[2 * p + 1] = cimag(trip[p].d);
+    }
+    for (int64_t i = 0; i < m; ++i) rowptr[i + 1] += rowptr[i];
+    if (diag) for (int q = 0; q < 4; ++q) diag[q] = dg[q];
+    free(trip); free(Th); free(glob); free(loc); free(B0); free(B1); free(zs);
+    return out;
+}
+
+/* quadrature.jl:264-286 with a brute-force nearest neighbour */
+int64_t ork_nearest_lists(int dim, int64_t m, const double *tx, int64_t n,
+                          const double *sx, int nq, int64_t ne,
+                          const int64_t *qtags, double maxdist,
+                          int64_t *near_ptr, int64_t *near_idx) {
+    int64_t *owner = (int64_t *)malloc(sizeof(int64_t) * n);
+    for (int64_t e = 0; e < ne; ++e)
+        for (int q = 0; q < nq; ++q) owner[qtags[q + (int64_t)nq * e]] = e;
+    int64_t *nearest = (int64_t *)malloc(sizeof(int64_t) * (m > 0 ? m : 1));
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < m; ++i) {
+        double best = INFINITY;
+        int64_t bj = -1;
+        for (int64_t j = 0; j < n; ++j) {
+            double s = 0;
+            for (int d = 0; d < dim; ++d) {
+                double r = tx[d * m + i] - sx[d * n + j];
+                s += r * r;
+            }
+            if (s < best) { best = s; bj = j; }
+        }
+        nearest[i] = sqrt(best) > maxdist ? -1 : bj;
+    }
+    /* per element: node order within the element, then target order */
+    int64_t cnt = 0;
+    near_ptr[0] = 0;
+    for (int64_t e = 0; e < ne; ++e) {
+        for (int q = 0; q < nq; ++q) {
+            int64_t node = qtags[q + (int64_t)nq * e];
+            for (int64_t i = 0; i < m; ++i)
+                if (nearest[i] == node) near_idx[cnt++] = i;
+        }
+        near_ptr[e + 1] = cnt;
+    }
+    free(owner); free(nearest);
+    return cnt;
+}
+
+void ork_csr_matvec(int64_t m, int64_t n, const int64_t *rowptr,
+                    const int64_t *colind, const double *val, int nrhs,
+                    const double *x, const double alpha[2], const double beta[2],
+                    double *y) {
+    cplx a = alpha[0] + I * alpha[1], bt = beta[0] + I * beta[1];
+    for (int r = 0; r < nrhs; ++r) {
+        for (int64_t i = 0; i < m; ++i) {
+            cplx s = 0;
+            for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+                const double *xv = x + 2 * (colind[p] + n * (int64_t)r);
+                s += (val[2 * p] + I * val[2 * p + 1]) * (xv[0] + I * xv[1]);
+            }
+            double *yv = y + 2 * (i + m * (int64_t)r);
+            cplx yo = yv[0] + I * yv[1];
+            cplx res = a * s + (bt == 0 ? 0 : bt * yo);
+            yv[0] = creal(res); yv[1] = cimag(res);
+        }
+    }
+}

@@ -1,0 +1,246 @@
+"""GPU parity: libinti_amd (through the C-ABI) against the CPU oracle on the
+same inputs. FP64 bar: relative error <= 1e-12 for entries and matvecs;
+correction sparsity pattern bit-exact. MIXED bar: <= 1e-6."""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LAP, HEL = 0, 1
+KINDS = range(6)
+KNAMES = ["SL", "DL", "ADL", "HS", "COMB_SD", "COMB_AH"]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def geoms(inti):
+    return {2: inti.circle_quadrature(24, qorder=8), 3: inti.sphere_quadrature(2, base="ico")}
+
+
+def _kernel(inti, pde, dim, kind, k):
+    op = inti.Laplace(dim=dim) if pde == LAP else inti.Helmholtz(k=k, dim=dim)
+    if kind in (4, 5):
+        c0, c1 = (0.7, -1.3) if pde == LAP else (0.4 - 1.1j, 0.9 + 0.3j)
+        return inti.CombinedFieldKernel(op, c0, c1, "SD" if kind == 4 else "AH"), (c0, c1)
+    cls = [inti.SingleLayerKernel, inti.DoubleLayerKernel, inti.AdjointDoubleLayerKernel,
+           inti.HyperSingularKernel][kind]
+    return cls(op), None
+
+
+def _k(pde, dim):
+    return 0.0 if pde == LAP else (4.0 if dim == 2 else 3.0)
+
+
+CASES = list(itertools.product([LAP, HEL], [2, 3], KINDS))
+
+
+@pytest.mark.parametrize("pde,dim,kind", CASES)
+def test_assemble_matches_oracle(inti, oracle, geoms, pde, dim, kind):
+    say(f"[gpu] assemble pde={pde} dim={dim} kind={KNAMES[kind]}")
+    Q = geoms[dim]
+    k = _k(pde, dim)
+    K, coefs = _kernel(inti, pde, dim, kind, k)
+    A = inti.assemble_matrix(inti.IntegralOperator(K, Q, Q))
+    R = oracle.assemble(pde, dim, kind, k, Q, Q, coefs=coefs)
+    if pde == LAP:
+        R = R.real
+    err = np.abs(A - R).max() / np.abs(R).max()
+    fro = np.linalg.norm(A - R) / np.linalg.norm(R)
+    assert err <= 1e-12 and fro <= 1e-13, (err, fro)
+
+
+@pytest.mark.parametrize("pde,dim,kind", CASES)
+def test_apply_matches_oracle(inti, oracle, geoms, pde, dim, kind):
+    say(f"[gpu] apply pde={pde} dim={dim} kind={KNAMES[kind]}")
+    Q = geoms[dim]
+    k = _k(pde, dim)
+    K, coefs = _kernel(inti, pde, dim, kind, k)
+    rng = np.random.default_rng(kind + 10 * dim + 100 * pde)
+    n = len(Q)
+    x = rng.normal(size=n) if pde == LAP else rng.normal(size=n) + 1j * rng.normal(size=n)
+    ref = oracle.matvec(pde, dim, kind, k, Q, Q, x, coefs=coefs)
+    y = inti.IntegralOperator(K, Q, Q) @ x
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) <= 1e-12
+    ym = inti.IntegralOperator(K, Q, Q, precision="mixed") @ x
+    assert np.linalg.norm(ym - ref) / np.linalg.norm(ref) <= 1e-6
+
+
+def test_apply_alpha_beta_identity_multirhs(inti, oracle, geoms):
+    say("[gpu] apply alpha/beta/identity nrhs=3")
+    Q = geoms[3]
+    op = inti.Helmholtz(k=2.0, dim=3)
+    A = inti.IntegralOperator(inti.DoubleLayerKernel(op), Q, Q)
+    A.set_identity(0.5)
+    rng = np.random.default_rng(2)
+    n = len(Q)
+    X = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    Y0 = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    alpha, beta = 0.3 - 0.2j, -1.5 + 0.25j
+    Y = A.apply(X, Y0, alpha, beta)
+    ref = alpha * (oracle.matvec(HEL, 3, 1, 2.0, Q, Q, X) + 0.5 * X) + beta * Y0
+    assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) <= 1e-12
+
+
+def test_apply_split_and_padding(inti, oracle):
+    """Several source chunks and a ragged last tile (N not a multiple of 256)."""
+    say("[gpu] apply split/padding N=4320")
+    Q = inti.sphere_quadrature(6, base="ico")
+    op = inti.Helmholtz(k=6.0, dim=3)
+    K = inti.CombinedFieldKernel(op, -6.0j, 1.0, "SD")
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=len(Q)) + 1j * rng.normal(size=len(Q))
+    rows = (1000, 1400)
+    ref = oracle.matvec(HEL, 3, 4, 6.0, Q, Q, x, coefs=(-6.0j, 1.0), rows=rows)[rows[0]:rows[1]]
+    iop = inti.IntegralOperator(K, Q, Q)
+    for split in (0, 1, 3, 8):
+        if split:
+            iop.ctx.check(iop.ctx.lib.inti_op_set_launch(iop.handle, 256, 1, split))
+        y = (iop @ x)[rows[0]:rows[1]]
+        assert np.linalg.norm(y - ref) / np.linalg.norm(ref) <= 1e-12, split
+
+
+@pytest.mark.parametrize("pde,dim", [(LAP, 2), (HEL, 2), (LAP, 3), (HEL, 3)])
+@pytest.mark.parametrize("derivative", [False, True])
+def test_bdim_matches_oracle(inti, oracle, pde, dim, derivative):
+    say(f"[gpu] bdim pde={pde} dim={dim} derivative={derivative}")
+    Q = inti.circle_quadrature(40, qorder=8) if dim == 2 else inti.sphere_quadrature(3, base="ico")
+    k = 0.0 if pde == LAP else 2.5
+    op = inti.Laplace(dim=dim) if pde == LAP else inti.Helmholtz(k=k, dim=dim)
+    if derivative:
+        Sop = inti.IntegralOperator(inti.AdjointDoubleLayerKernel(op), Q, Q)
+        Dop = inti.IntegralOperator(inti.HyperSingularKernel(op), Q, Q)
+    else:
+        Sop = inti.IntegralOperator(inti.SingleLayerKernel(op), Q, Q)
+        Dop = inti.IntegralOperator(inti.DoubleLayerKernel(op), Q, Q)
+    mu = np.full(len(Q), -0.5)
+    diag = {}
+    dS, dD = inti.bdim_correction(op, Q, Q, Sop, Dop, green_multiplier=mu,
+                                  derivative=derivative, diagnostics=diag)
+    rp, ci, vS, vD = *dS.export()[:2], dS.export()[2], dD.export()[2]
+    orp, oci, ovS, ovD, odiag = oracle.bdim(pde, dim, k, Q, Q, mu, derivative=derivative)
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci)  # pattern bit-exact
+    if pde == LAP:
+        ovS, ovD = ovS.real, ovD.real
+    # weights of an ill-conditioned least-squares problem: ~cond * eps
+    for v, o in ((vS, ovS), (vD, ovD)):
+        assert np.abs(v - o).max() <= 1e-9 * np.abs(o).max()
+    # the corrected matvec is what users see: FP64 bar
+    rng = np.random.default_rng(9)
+    n = len(Q)
+    x = rng.normal(size=n) if pde == LAP else rng.normal(size=n) + 1j * rng.normal(size=n)
+    Dop.attach(dD)
+    y = Dop @ x
+    kD = 3 if derivative else 1
+    ref = oracle.matvec(pde, dim, kD, k, Q, Q, x) + oracle.csr_matvec(n, n, orp, oci, ovD, x)
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) <= 1e-12
+    assert diag["rdiag_ratio"] > 1
+
+
+def test_bdim_offsurface_targets(inti, oracle):
+    """Targets off the curve (dim_test.jl:42-90): GPU vs oracle, and the
+    reference's thresholds D+ 1 ~ 0, D- 1 ~ -1."""
+    say("[gpu] bdim off-surface targets")
+    Q = inti.circle_quadrature(32, qorder=3)
+    g = np.linspace(-1.5, 1.5, 31)
+    P = np.array([[a, b] for a in g for b in g]).T
+    r = np.linalg.norm(P, axis=0)
+    op = inti.Laplace(dim=2)
+    for inside in (True, False):
+        idx = np.where((r < 1.0) if inside else (r >= 1.0))[0]
+        T = inti.points(P[:, idx])
+        S, D = inti.single_double_layer(op=op, target=T, source=Q, compression={"method": "gpu"},
+                                        correction={"method": "dim", "maxdist": np.inf,
+                                                    "target_location": "inside" if inside else "outside"})
+        v = D @ np.ones(len(Q))
+        near = oracle.nearest_lists(T, Q)
+        mu = np.full(len(idx), -1.0 if inside else 0.0)
+        orp, oci, ovS, ovD, _ = oracle.bdim(LAP, 2, 0.0, T, Q, mu, near=near)
+        ref = oracle.matvec(LAP, 2, 1, 0.0, T, Q, np.ones(len(Q))) + \
+            oracle.csr_matvec(len(idx), len(Q), orp, oci, ovD, np.ones(len(Q)))
+        assert np.linalg.norm(v - ref.real) / np.linalg.norm(ref) <= 1e-12
+        target = -1.0 if inside else 0.0
+        far = np.abs(r[idx] - 1) > 0.05
+        assert np.abs(v[far] - target).max() < 1e-3
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_green_identity_gpu_path(inti, dim):
+    """green_identities_test.jl:93-104 through single_double_layer on the GPU."""
+    say(f"[gpu] Green identity dim={dim}")
+    Q = inti.circle_quadrature(13, qorder=5) if dim == 2 else inti.sphere_quadrature(2, base="ico")
+    for op in (inti.Laplace(dim=dim), inti.Helmholtz(k=1.2, dim=dim)):
+        xs = np.full(dim, 3.0)
+        G = inti.SingleLayerKernel(op)
+        # exterior source point -> interior identity with sigma = 1/2
+        import oracle as O
+        pde = LAP if isinstance(op, inti.Laplace) else HEL
+        u = np.array([O.kernel(pde, dim, 0, op.k, Q.coords[:, j], Q.normals[:, j], xs, None) for j in range(len(Q))])
+        dudn = np.array([O.kernel(pde, dim, 2, op.k, Q.coords[:, j], Q.normals[:, j], xs, None) for j in range(len(Q))])
+        if pde == LAP:
+            u, dudn = u.real, dudn.real
+        Sm = inti.assemble_matrix(inti.IntegralOperator(G, Q, Q))
+        Dm = inti.assemble_matrix(inti.IntegralOperator(inti.DoubleLayerKernel(op), Q, Q))
+        e0 = np.abs(Sm @ dudn - Dm @ u - 0.5 * u).max() / np.abs(u).max()
+        S, D = inti.single_double_layer(op=op, target=Q, source=Q, compression={"method": "none"},
+                                        correction={"method": "dim"})
+        e1 = np.abs(S @ dudn - D @ u - 0.5 * u).max() / np.abs(u).max()
+        Sg, Dg = inti.single_double_layer(op=op, target=Q, source=Q, compression={"method": "gpu"},
+                                          correction={"method": "dim"})
+        e2 = np.abs(Sg @ dudn - Dg @ u - 0.5 * u).max() / np.abs(u).max()
+        assert e1 < 1e-2 and e1 < e0
+        assert abs(e2 - e1) <= 1e-10 * max(e1, 1e-16) + 1e-14
+
+
+def test_combined_field_operator(inti, oracle):
+    """Corrected I/2 + D - i k S in one fused apply vs the oracle composite."""
+    say("[gpu] combined field operator")
+    Q = inti.sphere_quadrature(3, base="ico")
+    k = 2 * np.pi
+    op = inti.Helmholtz(k=k, dim=3)
+    L, (dS, dD) = inti.combined_field(op=op, quad=Q, c_single=-1j * k, c_double=1.0, c_identity=0.5)
+    rng = np.random.default_rng(11)
+    n = len(Q)
+    x = rng.normal(size=n) + 1j * rng.normal(size=n)
+    y = L @ x
+    mu = np.full(n, -0.5)
+    orp, oci, ovS, ovD, _ = oracle.bdim(HEL, 3, k, Q, Q, mu)
+    ref = 0.5 * x + oracle.matvec(HEL, 3, 4, k, Q, Q, x, coefs=(-1j * k, 1.0)) + \
+        oracle.csr_matvec(n, n, orp, oci, ovD - 1j * k * ovS, x)
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) <= 1e-12
+    Lm, _ = inti.combined_field(op=op, quad=Q, c_single=-1j * k, c_double=1.0, c_identity=0.5,
+                                precision="mixed", corrections=(dS, dD))
+    ym = Lm @ x
+    assert np.linalg.norm(ym - ref) / np.linalg.norm(ref) <= 1e-6
+
+
+def test_error_paths(inti):
+    say("[gpu] error paths")
+    Q2 = inti.circle_quadrature(8)
+    Q3 = inti.sphere_quadrature(1)
+    P = inti.points(np.zeros((2, 4)))
+    with pytest.raises(ValueError):
+        inti.IntegralOperator(inti.DoubleLayerKernel(inti.Laplace(dim=2)), Q3, Q2)
+    # DL needs source normals; a point set has none
+    bad = inti.IntegralOperator(inti.DoubleLayerKernel(inti.Laplace(dim=2)), Q2, P)
+    with pytest.raises(inti.IntiError) as e:
+        bad.handle
+    assert e.value.code == 1
+    A = inti.IntegralOperator(inti.SingleLayerKernel(inti.Laplace(dim=2)), P, Q2)
+    with pytest.raises(inti.IntiError):
+        A.set_identity(1.0)  # not square
+    S = inti.IntegralOperator(inti.SingleLayerKernel(inti.Laplace(dim=2)), Q2, Q2)
+    D = inti.IntegralOperator(inti.DoubleLayerKernel(inti.Laplace(dim=2)), Q2, Q2)
+    dS, dD = inti.bdim_correction(inti.Laplace(dim=2), Q2, Q2, S, D,
+                                  green_multiplier=np.full(len(Q2), -0.5))
+    H = inti.IntegralOperator(inti.DoubleLayerKernel(inti.Helmholtz(k=1.0, dim=2)), Q2, Q2)
+    with pytest.raises(inti.IntiError):
+        H.attach(dD)  # real correction on a complex operator
+    with pytest.raises(inti.IntiError):
+        A.attach(dD)  # shape mismatch
