@@ -3,6 +3,7 @@
 The library is built in-tree next to this file. There is no fallback: if it
 is missing or has no GPU to run on, calls raise.
 """
+import atexit
 import ctypes as C
 import os
 import threading
@@ -61,7 +62,21 @@ SIGNATURES = {
 }
 
 _lib = None
-_lock = threading.Lock()
+_lock = threading.RLock()
+_exiting = False
+
+
+@atexit.register
+def _mark_exit():
+    """At interpreter exit no handle is released any more: the process
+    teardown frees the device memory, and no library call races the
+    finalisation order of Python objects."""
+    global _exiting
+    _exiting = True
+
+
+def exiting():
+    return _exiting
 
 
 class IntiError(RuntimeError):
@@ -125,9 +140,9 @@ class Context:
         self.lib = lib
 
     def close(self):
-        if self.handle:
+        if self.handle and not _exiting:
             self.lib.inti_ctx_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -161,7 +176,6 @@ _default_ctx = None
 def default_context():
     global _default_ctx
     with _lock:
-        pass
-    if _default_ctx is None:
-        _default_ctx = Context(0)
+        if _default_ctx is None:
+            _default_ctx = Context(0)
     return _default_ctx

@@ -39,7 +39,7 @@ class SparseCorrection:
 
     def __del__(self):
         try:
-            if self.handle:
+            if self.handle and not _lib.exiting():
                 self.ctx.lib.inti_csr_destroy(self.handle)
         except Exception:
             pass
@@ -106,7 +106,7 @@ class IntegralOperator:
 
     def __del__(self):
         try:
-            if self._h is not None:
+            if self._h is not None and not _lib.exiting():
                 self.ctx.lib.inti_op_destroy(self._h)
         except Exception:
             pass

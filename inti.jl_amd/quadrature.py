@@ -18,6 +18,8 @@ import math
 
 import numpy as np
 
+from . import _lib
+
 
 # ----------------------------------------------------------- reference rules
 def _legendre_and_derivative(n, t):
@@ -141,7 +143,6 @@ class Quadrature:
 
     def device(self, ctx):
         """Device copy on ctx (uploaded once, cached)."""
-        from . import _lib
         key = id(ctx)
         if key not in self._dev:
             import ctypes as C
@@ -153,6 +154,8 @@ class Quadrature:
         return self._dev[key][1]
 
     def __del__(self):
+        if _lib.exiting():
+            return
         for ctx, h in getattr(self, "_dev", {}).values():
             try:
                 ctx.lib.inti_quad_destroy(h)

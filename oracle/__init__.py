@@ -15,38 +15,49 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+LIB_FMA = os.path.join(HERE, "liboracle_fma.so")
 
 _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
 _lib = None
+_lib_fma = None
 
 
-def _load():
-    global _lib
+def _load(variant=None):
+    """The oracle library; variant="fma" gives the FMA-contracted build."""
+    global _lib, _lib_fma
+    if variant == "fma":
+        if _lib_fma is None:
+            _lib_fma = _bind(LIB_FMA)
+        return _lib_fma
     if _lib is None:
-        if not os.path.exists(LIB):
-            subprocess.run(["make", "-s", "-C", HERE], check=True)
-        lib = C.CDLL(LIB)
-        lib.ork_kernel.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp]
-        lib.ork_hankel1.argtypes = [C.c_int, C.c_double, _dp]
-        lib.ork_assemble.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int64, _dp, _dp,
-                                     C.c_int64, _dp, _dp, _dp, _dp]
-        lib.ork_matvec.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int64, _dp, _dp,
-                                   C.c_int64, _dp, _dp, _dp, C.c_int, _dp, _dp, C.c_int64,
-                                   C.c_int64, C.c_int]
-        lib.ork_dim_monopoles.restype = C.c_int
-        lib.ork_dim_monopoles.argtypes = [C.c_int, C.c_int64, _dp, C.c_int, C.c_double, C.c_double, _dp]
-        lib.ork_bdim.restype = C.c_int64
-        lib.ork_bdim.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int64, _dp, _dp,
-                                 C.c_int64, _dp, _dp, _dp, C.c_int, C.c_int64, _i64p, _i64p, _i64p,
-                                 _dp, C.c_double, C.c_double, _i64p, _i64p, _dp, _dp, _dp]
-        lib.ork_nearest_lists.restype = C.c_int64
-        lib.ork_nearest_lists.argtypes = [C.c_int, C.c_int64, _dp, C.c_int64, _dp, C.c_int, C.c_int64,
-                                          _i64p, C.c_double, _i64p, _i64p]
-        lib.ork_csr_matvec.argtypes = [C.c_int64, C.c_int64, _i64p, _i64p, _dp, C.c_int, _dp, _dp,
-                                       _dp, _dp]
-        _lib = lib
+        _lib = _bind(LIB)
     return _lib
+
+
+def _bind(path):
+    if not os.path.exists(path):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    lib = C.CDLL(path)
+    lib.ork_kernel.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.ork_hankel1.argtypes = [C.c_int, C.c_double, _dp]
+    lib.ork_assemble.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int64, _dp, _dp,
+                                 C.c_int64, _dp, _dp, _dp, _dp]
+    lib.ork_matvec.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int64, _dp, _dp,
+                               C.c_int64, _dp, _dp, _dp, C.c_int, _dp, _dp, C.c_int64,
+                               C.c_int64, C.c_int]
+    lib.ork_dim_monopoles.restype = C.c_int
+    lib.ork_dim_monopoles.argtypes = [C.c_int, C.c_int64, _dp, C.c_int, C.c_double, C.c_double, _dp]
+    lib.ork_bdim.restype = C.c_int64
+    lib.ork_bdim.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int64, _dp, _dp,
+                             C.c_int64, _dp, _dp, _dp, C.c_int, C.c_int64, _i64p, _i64p, _i64p,
+                             _dp, C.c_double, C.c_double, _i64p, _i64p, _dp, _dp, _dp]
+    lib.ork_nearest_lists.restype = C.c_int64
+    lib.ork_nearest_lists.argtypes = [C.c_int, C.c_int64, _dp, C.c_int64, _dp, C.c_int, C.c_int64,
+                                      _i64p, C.c_double, _i64p, _i64p]
+    lib.ork_csr_matvec.argtypes = [C.c_int64, C.c_int64, _i64p, _i64p, _dp, C.c_int, _dp, _dp,
+                                   _dp, _dp]
+    return lib
 
 
 def _d(a):
@@ -123,7 +134,7 @@ def monopoles(dim, src_coords, nq, oversample=3.0, radius_mult=1.5):
 
 
 def bdim(pde, dim, k, trg, src, mu, derivative=False, near=None, oversample=3.0,
-         radius_mult=1.5):
+         radius_mult=1.5, variant=None):
     """CSR (rowptr, colind, valS, valD) and diagnostics of bdim_correction."""
     m, n = trg.coords.shape[1], src.coords.shape[1]
     nq, ne = src.qtags.shape
@@ -141,7 +152,7 @@ def bdim(pde, dim, k, trg, src, mu, derivative=False, near=None, oversample=3.0,
     valS = np.zeros(nnz_max, dtype=np.complex128)
     valD = np.zeros(nnz_max, dtype=np.complex128)
     diag = np.zeros(4)
-    nnz = _load().ork_bdim(pde, dim, k, 1 if derivative else 0, m, _d(_f64(trg.coords)),
+    nnz = _load(variant).ork_bdim(pde, dim, k, 1 if derivative else 0, m, _d(_f64(trg.coords)),
                            _d(_f64(trg.normals)), n, _d(_f64(src.coords)), _d(_f64(src.normals)),
                            _d(_f64(src.weights)), nq, ne, _i(qt), _i(near_ptr), _i(near_idx),
                            _d(_f64(mu)), oversample, radius_mult, _i(rowptr), _i(colind),

@@ -133,8 +133,9 @@ def test_monopoles(oracle):
     assert np.isclose(z3[1, 0], 1 + 1.5 * np.sqrt(12) / 2) and np.isclose(z3[1, -1], 1 - 1.5 * np.sqrt(12) / 2)
 
 
-def _bdim_pieces(oracle, pde, dim, k, Q, mu):
-    """A = M^T per element and Theta rows, rebuilt from kernel values."""
+def _bdim_pieces(oracle, pde, dim, k, Q, mu, derivative=False):
+    """A = M^T per element and Theta rows, rebuilt from kernel values
+    (bdim.jl:94-118)."""
     nq = Q.qtags.shape[0]
     zs = oracle.monopoles(dim, Q.coords, nq)
     ns = zs.shape[1]
@@ -143,9 +144,10 @@ def _bdim_pieces(oracle, pde, dim, k, Q, mu):
                     for q in range(ns)] for j in range(n)])
     B1 = np.array([[oracle.kernel(pde, dim, ADL, k, Q.coords[:, j], Q.normals[:, j], zs[:, q], None)
                     for q in range(ns)] for j in range(n)])
-    G = np.array([[oracle.kernel(pde, dim, SL, k, Q.coords[:, i], Q.normals[:, i], zs[:, q], None)
+    kG, kS, kD = (ADL, ADL, HS) if derivative else (SL, SL, DL)
+    G = np.array([[oracle.kernel(pde, dim, kG, k, Q.coords[:, i], Q.normals[:, i], zs[:, q], None)
                    for q in range(ns)] for i in range(n)])
-    Th = mu[:, None] * G + oracle.matvec(pde, dim, SL, k, Q, Q, B1) - oracle.matvec(pde, dim, DL, k, Q, Q, B0)
+    Th = mu[:, None] * G + oracle.matvec(pde, dim, kS, k, Q, Q, B1) - oracle.matvec(pde, dim, kD, k, Q, Q, B0)
     return B0, B1, Th
 
 
