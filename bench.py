@@ -56,7 +56,7 @@ def time_apply(ctx, lib, op, dx, dy, n, steps, warmup):
     return (t1 - t0) / steps, float(np.mean(kms))
 
 
-def cpu_baseline(oracle, Q, x, k, target_seconds=12.0):
+def cpu_baseline(oracle, Q, x, k, target_seconds=15.0):
     """The oracle (FP64, OpenMP over all host threads) on a bounded row
     sample of the same direct sum; pair-evals/s."""
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
