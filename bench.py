@@ -77,13 +77,30 @@ def cpu_baseline(oracle, Q, x, k, target_seconds=15.0):
                       f"of {n} against all {n} sources ({rows * n:.3g} pairs, {t:.1f} s)"}
 
 
+def calibrate_copy(ctx, lib, nbytes):
+    """A plain copy of known size (reads nbytes, writes nbytes) to scale the
+    profiler's FETCH_SIZE / WRITE_SIZE counters."""
+    a, b = C.c_void_p(), C.c_void_p()
+    ctx.check(lib.inti_device_alloc(ctx.handle, nbytes, C.byref(a)))
+    ctx.check(lib.inti_device_alloc(ctx.handle, nbytes, C.byref(b)))
+    for _ in range(5):
+        ctx.check(lib.inti_device_copy(ctx.handle, b, a, nbytes))
+    log(f"bench: copied {nbytes} bytes 5 times")
+    ctx.check(lib.inti_device_free(ctx.handle, a))
+    ctx.check(lib.inti_device_free(ctx.handle, b))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--freq", type=int, default=23)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
+                    help="profiling runs: time one leg only, no checks, no JSON")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="profiler calibration: run a plain device copy of this many bytes 5 times")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -94,6 +111,9 @@ def main():
     inti = load_package()
     lib = inti._lib.load()
     ctx = inti.default_context()
+    if args.copy_bytes:
+        calibrate_copy(ctx, lib, args.copy_bytes)
+        return
     ctx.set_profiling(True)
 
     k = 2 * np.pi
@@ -117,6 +137,12 @@ def main():
     ctx.check(lib.inti_memcpy_h2d(ctx.handle, dx, xc.ctypes.data_as(C.c_void_p), nbytes))
 
     pairs = float(n) * n
+    if args.only:
+        L = Lmx if args.only == "mixed" else L64
+        t, kms = time_apply(ctx, lib, L.iop.handle, dx, dy, n, args.steps, args.warmup)
+        log(f"bench: {args.only} only: {t * 1e3:.3f} ms/apply, kernel {kms:.3f} ms, "
+            f"{pairs / t / 1e9:.1f} Gpair/s")
+        return
     # FP64 leg (reference for the ratio and the error)
     t64, k64 = time_apply(ctx, lib, L64.iop.handle, dx, dy, n, max(3, args.steps // 2), 1)
     y64 = np.zeros(n, dtype=np.complex128)

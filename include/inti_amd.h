@@ -82,6 +82,10 @@ int inti_device_alloc(inti_ctx *ctx, size_t bytes, void **dptr);
 int inti_device_free(inti_ctx *ctx, void *dptr);
 int inti_memcpy_h2d(inti_ctx *ctx, void *dst, const void *src, size_t bytes);
 int inti_memcpy_d2h(inti_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel on the context stream
+ * (bytes a multiple of 16). No Inti.jl counterpart: a kernel of exactly
+ * known HBM traffic, used to calibrate the profiler's byte counters. */
+int inti_device_copy(inti_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* ------------------------------------------------------------- quadrature */
 /* Upload a quadrature (replaces handing Inti.Quadrature's QuadratureNode

@@ -42,6 +42,7 @@ SIGNATURES = {
     "inti_device_free": (C.c_int, [_vp, _vp]),
     "inti_memcpy_h2d": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     "inti_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "inti_device_copy": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     "inti_quad_upload": (C.c_int, [_vp, C.c_int, C.c_int64, _dp, _dp, _dp, C.POINTER(_vp)]),
     "inti_quad_destroy": (C.c_int, [_vp]),
     "inti_op_create": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, _dp, _vp, _vp,

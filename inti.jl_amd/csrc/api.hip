@@ -172,6 +172,28 @@ int inti_memcpy_d2h(inti_ctx *ctx, void *dst, const void *src, size_t bytes) {
     return INTI_OK;
 }
 
+__global__ __launch_bounds__(256) void k_copy16(const double2 *__restrict__ src,
+                                                 double2 *__restrict__ dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+int inti_device_copy(inti_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return INTI_ERR_ARG;
+    REQUIRE(ctx, bytes % 16 == 0, "device_copy: bytes must be a multiple of 16");
+    REQUIRE(ctx, bytes == 0 || (dst && src), "device_copy: null pointer");
+    LOCK(ctx);
+    const int64_t n = (int64_t)(bytes / 16);
+    if (n == 0) return INTI_OK;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8 * 1024) blocks = 8 * 1024;
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                       (const double2 *)src, (double2 *)dst, n);
+    INTI_HIP(ctx, hipGetLastError());
+    INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return INTI_OK;
+}
+
 // ------------------------------------------------------------- quadrature
 int inti_quad_upload(inti_ctx *ctx, int dim, int64_t n, const double *coords,
                      const double *normals, const double *weights, inti_quad **out) {

@@ -28,7 +28,12 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs a) {
             ny[d] = a.sn ? a.sn[d * a.n + j] : 0.0;
         }
         double kr, ki;
-        pair_value<PDE, DIM, KIND, false>(r, nx, ny, a.p, kr, ki);
+        pair_raw<PDE, DIM, KIND, false>(r, nx, ny, a.p, kr, ki);
+        {
+            const double tr = a.p.postr * kr - a.p.posti * ki;
+            ki = a.p.postr * ki + a.p.posti * kr;
+            kr = tr;
+        }
         const double w = a.sw[j];
         if constexpr (CPLX) {
             a.out[2 * e] = kr * w;
@@ -59,19 +64,19 @@ static AsmFn asm_kind(int kind) {
 
 int launch_assemble(const Op *op, double *out_dev) {
     AsmFn fn = nullptr;
-    if (op->pde == INTI_LAPLACE && op->dim == 2) fn = asm_kind<INTI_LAPLACE, 2>(op->kind);
-    if (op->pde == INTI_LAPLACE && op->dim == 3) fn = asm_kind<INTI_LAPLACE, 3>(op->kind);
-    if (op->pde == INTI_HELMHOLTZ && op->dim == 2) fn = asm_kind<INTI_HELMHOLTZ, 2>(op->kind);
-    if (op->pde == INTI_HELMHOLTZ && op->dim == 3) fn = asm_kind<INTI_HELMHOLTZ, 3>(op->kind);
+    int kind = op->kind;
+    const KParams kp = op_params(op, &kind);
+    if (op->pde == INTI_LAPLACE && op->dim == 2) fn = asm_kind<INTI_LAPLACE, 2>(kind);
+    if (op->pde == INTI_LAPLACE && op->dim == 3) fn = asm_kind<INTI_LAPLACE, 3>(kind);
+    if (op->pde == INTI_HELMHOLTZ && op->dim == 2) fn = asm_kind<INTI_HELMHOLTZ, 2>(kind);
+    if (op->pde == INTI_HELMHOLTZ && op->dim == 3) fn = asm_kind<INTI_HELMHOLTZ, 3>(kind);
     if (!fn) return set_error(op->ctx, INTI_ERR_UNSUPPORTED, "no assembly kernel for this operator");
     AsmArgs a;
     a.m = op->trg->n; a.n = op->src->n;
     a.tx = op->trg->coords; a.tn = op->trg->normals;
     a.sx = op->src->coords; a.sn = op->src->normals; a.sw = op->src->weights;
     a.out = out_dev;
-    a.p.k = op->k;
-    a.p.c0r = op->coefs[0]; a.p.c0i = op->coefs[1];
-    a.p.c1r = op->coefs[2]; a.p.c1i = op->coefs[3];
+    a.p = kp;
     const int64_t total = a.m * a.n;
     if (total == 0) return INTI_OK;
     int64_t blocks = (total + 255) / 256;

@@ -49,8 +49,8 @@ __global__ __launch_bounds__(256) void k_traces(TraceArgs a) {
             ny[d] = a.sn[d * a.n + j];
             dummy[d] = 0.0;
         }
-        pair_value<PDE, DIM, INTI_SL, false>(r, ny, dummy, a.p, g0r, g0i);
-        pair_value<PDE, DIM, INTI_ADL, false>(r, ny, dummy, a.p, g1r, g1i);
+        pair_value<PDE, DIM, INTI_SL>(r, ny, dummy, a.p, g0r, g0i);
+        pair_value<PDE, DIM, INTI_ADL>(r, ny, dummy, a.p, g1r, g1i);
     }
     a.B0[e] = make_double2(g0r, g0i);
     a.B1[e] = make_double2(g1r, g1i);
@@ -127,8 +127,8 @@ __global__ __launch_bounds__(kThetaBS) void k_theta(ThetaArgs a) {
             }
             const double w = s_geo[jj][2 * DIM];
             double sr, si, dr, di;
-            pair_value<PDE, DIM, KS, false>(r, nt, ny, a.p, sr, si);
-            pair_value<PDE, DIM, KD, false>(r, nt, ny, a.p, dr, di);
+            pair_value<PDE, DIM, KS>(r, nt, ny, a.p, sr, si);
+            pair_value<PDE, DIM, KD>(r, nt, ny, a.p, dr, di);
             sr *= w; si *= w; dr *= w; di *= w;
 #pragma unroll
             for (int k = 0; k < NS; ++k) {
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void k_theta_reduce(ThetaRedArgs a) {
         dummy[d] = 0.0;
     }
     double gr, gi;
-    pair_value<PDE, DIM, DERIV ? INTI_ADL : INTI_SL, false>(r, nx, dummy, a.p, gr, gi);
+    pair_value<PDE, DIM, DERIV ? INTI_ADL : INTI_SL>(r, nx, dummy, a.p, gr, gi);
     const double mu = a.mu[i];
     double tr = mu * gr, ti = mu * gi;
     double sr = 0.0, si = 0.0;
@@ -585,9 +585,7 @@ int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *q
     INTI_HIP(ctx, up(dperm.p, perm.data(), perm.size() * sizeof(int32_t)));
     INTI_HIP(ctx, up(dmu.p, mu, sizeof(double) * m));
 
-    KParams kp;
-    kp.k = opS->k;
-    kp.c0r = 1; kp.c0i = 0; kp.c1r = 1; kp.c1i = 0;
+    KParams kp = basic_params(opS->k);
 
     // traces
     {

@@ -65,19 +65,18 @@ __device__ __forceinline__ double logv(double x) {
     else return log(x);
 }
 
-// (sin t, cos t). MIXED: t is reduced to revolutions in FP64 (exact to
-// ~1e-16 absolute) and fed to the FP32 sin/cos unit, whose argument is in
+// (sin 2 pi u, cos 2 pi u) for a phase u given in revolutions. FP64: the
+// sinpi/cospi pair, whose argument reduction is exact. MIXED: u is reduced to
+// [-1/2, 1/2] in FP64 and fed to the FP32 sin/cos unit, whose argument is in
 // revolutions.
 template <bool MIXED>
-__device__ __forceinline__ void sincosv(double t, double &s, double &c) {
+__device__ __forceinline__ void sincos_rev(double u, double &s, double &c) {
     if constexpr (MIXED) {
-        double u = t * 0.15915494309189533577;
-        double f = u - __builtin_rint(u);
-        float ff = (float)f;
+        float ff = (float)(u - __builtin_rint(u));
         s = (double)__builtin_amdgcn_sinf(ff);
         c = (double)__builtin_amdgcn_cosf(ff);
     } else {
-        sincos(t, &s, &c);
+        sincospi(2.0 * u, &s, &c);
     }
 }
 

@@ -9,6 +9,8 @@
 // that every lane then reads by broadcast. Block b works on chunk b % split,
 // so with split a multiple of 8 all blocks of one XCD (blocks are dealt to
 // the 8 XCDs round robin) share one source chunk in their L2.
+#include <complex>
+
 #include "common.hpp"
 #include "pair.hpp"
 
@@ -99,7 +101,7 @@ __global__ __launch_bounds__(BS) void k_apply(ApplyArgs a) {
 #pragma unroll
             for (int d = 0; d < DIM; ++d) r[d] = xt[d] - ys[d];
             double kr, ki;
-            pair_value<PDE, DIM, KIND, MIXED>(r, nt, ns, a.p, kr, ki);
+            pair_raw<PDE, DIM, KIND, MIXED>(r, nt, ns, a.p, kr, ki);
             if constexpr (CPLX) {
                 accr = __builtin_fma(kr, q_r, accr);
                 accr = __builtin_fma(-ki, q_i, accr);
@@ -131,6 +133,7 @@ struct ReduceArgs {
     const double *x;  // density (for the correction and identity terms)
     double *y;
     double ar, ai, br, bi, cr, ci;
+    double pr, pi;  // post factor of the direct sum
     int beta_zero;
     int has_id;
 };
@@ -149,6 +152,11 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
         }
     } else {
         for (int c = 0; c < a.split; ++c) sr += a.part[(int64_t)c * a.m + i];
+    }
+    {
+        const double tr = a.pr * sr - a.pi * si, ti = a.pr * si + a.pi * sr;
+        sr = tr;
+        si = ti;
     }
     if (a.rowptr) {
         const int32_t p0 = a.rowptr[i], p1 = a.rowptr[i + 1];
@@ -225,11 +233,40 @@ static ApplyFn pick_apply(int pde, int dim, int kind, int mixed) {
 
 bool op_is_complex(const Op *op) { return op->pde == INTI_HELMHOLTZ; }
 
-static KParams op_params(const Op *op) {
-    KParams p;
-    p.k = op->k;
-    p.c0r = op->coefs[0]; p.c0i = op->coefs[1];
-    p.c1r = op->coefs[2]; p.c1i = op->coefs[3];
+KParams op_params(const Op *op, int *eff_kind) {
+    using cd = std::complex<double>;
+    KParams p = basic_params(op->k);
+    int kind = op->kind;
+    const cd c0(op->coefs[0], op->coefs[1]), c1(op->coefs[2], op->coefs[3]);
+    cd post, ch(0.0, 0.0);
+    double pr, pi;
+    if (kind == INTI_COMB_SD || kind == INTI_COMB_AH) {
+        const int ka = kind == INTI_COMB_SD ? INTI_SL : INTI_ADL;
+        const int kb = kind == INTI_COMB_SD ? INTI_DL : INTI_HS;
+        double ar, ai, br, bi;
+        basic_post(op->pde, op->dim, ka, op->k, ar, ai);
+        basic_post(op->pde, op->dim, kb, op->k, br, bi);
+        if (c1 == cd(0.0, 0.0)) {  // only the first component remains
+            kind = ka;
+            post = c0 * cd(ar, ai);
+        } else {
+            // K = c1 postB (rawB + ch rawA), ch = c0 postA / (c1 postB); the
+            // 3D/2D Helmholtz raw values of the combined kinds fold the
+            // remaining constants as documented with helm3_raw / helm2_raw
+            post = c1 * cd(br, bi);
+            ch = c0 * cd(ar, ai) / post;
+            if (op->pde == INTI_HELMHOLTZ && op->dim == 3 && kind == INTI_COMB_AH)
+                ch = -ch;  // the ADL raw part inside COMB_AH carries -rdnx
+            if (op->pde == INTI_HELMHOLTZ && op->dim == 2 && kind == INTI_COMB_AH)
+                ch = -ch;
+        }
+    } else {
+        basic_post(op->pde, op->dim, kind, op->k, pr, pi);
+        post = cd(pr, pi);
+    }
+    p.postr = post.real(); p.posti = post.imag();
+    p.chr = ch.real(); p.chi = ch.imag();
+    if (eff_kind) *eff_kind = kind;
     return p;
 }
 
@@ -258,7 +295,9 @@ int launch_apply(const DirectLaunch &L) {
     const int cw = cplx ? 2 : 1;
     const int64_t m = op->trg->n, n = op->src->n;
     if (m == 0 || L.nrhs == 0) return INTI_OK;
-    ApplyFn fn = pick_apply(op->pde, op->dim, op->kind, op->precision == INTI_MIXED);
+    int kind = op->kind;
+    const KParams kp = op_params(op, &kind);
+    ApplyFn fn = pick_apply(op->pde, op->dim, kind, op->precision == INTI_MIXED);
     if (!fn) return set_error(ctx, INTI_ERR_UNSUPPORTED, "no direct-sum kernel for this operator");
     int split;
     int64_t chunk;
@@ -277,7 +316,7 @@ int launch_apply(const DirectLaunch &L) {
             a.tx = op->trg->coords; a.tn = op->trg->normals;
             a.sx = op->src->coords; a.sn = op->src->normals; a.sw = op->src->weights;
             a.x = xr; a.part = part; a.split = split; a.chunk = chunk;
-            a.p = op_params(op);
+            a.p = kp;
             dim3 grid((unsigned)(ntiles * split));
             if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
             fn(a, grid, ctx->stream);
@@ -294,6 +333,7 @@ int launch_apply(const DirectLaunch &L) {
         ra.br = L.beta[0]; ra.bi = L.beta[1];
         ra.beta_zero = (L.beta[0] == 0.0 && L.beta[1] == 0.0);
         ra.cr = op->cid[0]; ra.ci = op->cid[1];
+        ra.pr = kp.postr; ra.pi = kp.posti;
         ra.has_id = (op->cid[0] != 0.0 || op->cid[1] != 0.0);
         dim3 rg((unsigned)((m + 255) / 256));
         if (m > 0) {

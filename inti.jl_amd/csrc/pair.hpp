@@ -3,16 +3,28 @@
 // adjoint double layer (ADL), hypersingular (HS) and the combined fields
 // c0*SL + c1*DL, c0*ADL + c1*HS, written for one target against one source.
 // r = x - y; nx is the target normal, ny the source normal.
+//
+// Every kernel is split as K = post * raw: `raw` is what the direct sum
+// accumulates per pair, `post` (a complex constant of the operator: 1/(4 pi),
+// i k/4, the combined-field coefficient c1, ...) multiplies the finished sum
+// once per target. Coincident points (d <= 1e-14) give raw = 0: the distance
+// is clamped away from zero so that every intermediate stays finite, and one
+// scalar factor of raw is zeroed.
 #pragma once
 #include "common.hpp"
 #include "devmath.hpp"
 
 namespace inti {
 
+// Per-operator constants seen by the pair kernels.
 struct KParams {
-    double k;
-    double c0r, c0i, c1r, c1i;
+    double k;          // wavenumber
+    double kappa;      // k / (2 pi): phase in revolutions per unit length
+    double chr, chi;   // normalised combined-field coefficient (see post_factor)
+    double postr, posti;
 };
+
+constexpr double kClampD2 = 1.0e-30;  // stand-in for d^2 = 0 (masked anyway)
 
 template <int DIM>
 __device__ __forceinline__ double dotv(const double *a, const double *b) {
@@ -22,151 +34,187 @@ __device__ __forceinline__ double dotv(const double *a, const double *b) {
     return s;
 }
 
-// Laplace: real kernel value (constants included).
+__device__ __forceinline__ double mask(bool ok, double v) { return ok ? v : 0.0; }
+
+// ------------------------------------------------------------------ Laplace
+// raw values; posts: SL2 -1/(4pi), SL3 1/(4pi), DL2 1/(2pi), DL3 1/(4pi),
+// ADL2 -1/(2pi), ADL3 -1/(4pi), HS2 1/(2pi), HS3 1/(4pi).
 template <int DIM, int KIND, bool MIXED>
-__device__ __forceinline__ double laplace_pair(const double *r, double d2s, const double *nx,
-                                               const double *ny, const KParams &p) {
+__device__ __forceinline__ double laplace_raw(const double *r, double d2s, bool ok,
+                                              const double *nx, const double *ny,
+                                              const KParams &p) {
     if constexpr (KIND == INTI_SL) {
-        if constexpr (DIM == 2) return -0.5 * kInv2Pi * logv<MIXED>(d2s);  // -log(d)/(2pi)
-        else return kInv4Pi * rsqv<MIXED>(d2s);
+        if constexpr (DIM == 2) return mask(ok, logv<MIXED>(d2s));
+        else return mask(ok, rsqv<MIXED>(d2s));
     } else if constexpr (KIND == INTI_DL || KIND == INTI_ADL) {
-        const double sg = KIND == INTI_DL ? 1.0 : -1.0;
         double rdn = dotv<DIM>(r, KIND == INTI_DL ? ny : nx);
-        if constexpr (DIM == 2) return (sg * kInv2Pi) * rdn * rcpv<MIXED>(d2s);
+        if constexpr (DIM == 2) return mask(ok, rdn) * rcpv<MIXED>(d2s);
         else {
             double id = rsqv<MIXED>(d2s);
-            return (sg * kInv4Pi) * rdn * (id * id * id);
+            return mask(ok, rdn) * (id * id * id);
         }
     } else if constexpr (KIND == INTI_HS) {
         double nxny = dotv<DIM>(nx, ny);
         double rdnx = dotv<DIM>(r, nx), rdny = dotv<DIM>(r, ny);
         if constexpr (DIM == 2) {
-            double id2 = rcpv<MIXED>(d2s);
-            return kInv2Pi * id2 * (nxny - 2.0 * rdnx * rdny * id2);
+            double id2 = mask(ok, rcpv<MIXED>(d2s));
+            return id2 * (nxny - 2.0 * rdnx * rdny * id2);
         } else {
             double id = rsqv<MIXED>(d2s);
             double id2 = id * id;
-            return kInv4Pi * id2 * id * (nxny - 3.0 * rdnx * rdny * id2);
+            return mask(ok, id2 * id) * (nxny - 3.0 * rdnx * rdny * id2);
         }
-    } else if constexpr (KIND == INTI_COMB_SD) {
-        return p.c0r * laplace_pair<DIM, INTI_SL, MIXED>(r, d2s, nx, ny, p) +
-               p.c1r * laplace_pair<DIM, INTI_DL, MIXED>(r, d2s, nx, ny, p);
-    } else {
-        return p.c0r * laplace_pair<DIM, INTI_ADL, MIXED>(r, d2s, nx, ny, p) +
-               p.c1r * laplace_pair<DIM, INTI_HS, MIXED>(r, d2s, nx, ny, p);
+    } else if constexpr (KIND == INTI_COMB_SD) {  // DL + ch*SL
+        return laplace_raw<DIM, INTI_DL, MIXED>(r, d2s, ok, nx, ny, p) +
+               p.chr * laplace_raw<DIM, INTI_SL, MIXED>(r, d2s, ok, nx, ny, p);
+    } else {  // HS + ch*ADL
+        return laplace_raw<DIM, INTI_HS, MIXED>(r, d2s, ok, nx, ny, p) +
+               p.chr * laplace_raw<DIM, INTI_ADL, MIXED>(r, d2s, ok, nx, ny, p);
     }
 }
 
-// Helmholtz 3D: complex kernel value.
+// ---------------------------------------------------------------- Helmholtz 3D
+// raw = e^{ikd} * f * B; posts: SL, DL, HS 1/(4pi), ADL -1/(4pi),
+// COMB_SD c1/(4pi) with ch = c0/c1, COMB_AH c1/(4pi) with ch = c0/c1.
 template <int KIND, bool MIXED>
-__device__ __forceinline__ void helm3_pair(const double *r, double d2s, const double *nx,
-                                           const double *ny, const KParams &p, double &kr,
-                                           double &ki) {
+__device__ __forceinline__ void helm3_raw(const double *r, double d2s, bool ok, const double *nx,
+                                          const double *ny, const KParams &p, double &kr,
+                                          double &ki) {
     const double k = p.k;
     double id = rsqp<MIXED>(d2s);
     double d = d2s * id;
     double s, c;
-    sincosv<MIXED>(k * d, s, c);
-    double br, bi;  // bracket B; K = e^{ikd} * B * scale / (4 pi)
-    double scale;
+    sincos_rev<MIXED>(p.kappa * d, s, c);
+    double br, bi, f;
     if constexpr (KIND == INTI_SL) {
-        br = 1.0; bi = 0.0; scale = id;
+        br = 1.0; bi = 0.0; f = mask(ok, id);
     } else if constexpr (KIND == INTI_DL || KIND == INTI_ADL) {
-        double rdn = KIND == INTI_DL ? dotv<3>(r, ny) : -dotv<3>(r, nx);
-        double a = rdn * id * id;  // (1/d - ik) rdn / d^2
-        br = a * id; bi = -k * a; scale = 1.0;
+        double a = mask(ok, dotv<3>(r, KIND == INTI_DL ? ny : nx) * id * id);
+        br = a * id; bi = -k * a; f = 1.0;
     } else if constexpr (KIND == INTI_HS) {
         double nxny = dotv<3>(nx, ny);
         double id2 = id * id;
         double g = dotv<3>(r, nx) * dotv<3>(r, ny) * id2 * id;
         br = id * nxny + g * (k * k * d2s - 3.0);
         bi = 3.0 * k * d * g - k * nxny;
-        scale = id2;
-    } else if constexpr (KIND == INTI_COMB_SD) {
+        f = mask(ok, id2);
+    } else if constexpr (KIND == INTI_COMB_SD) {  // DL + ch*SL = e*id*[ch + (a id, -k a)]
         double a = dotv<3>(r, ny) * id;
-        double vr = a * id, vi = -k * a;  // DL bracket / SL bracket
-        br = p.c0r + p.c1r * vr - p.c1i * vi;
-        bi = p.c0i + p.c1r * vi + p.c1i * vr;
-        scale = id;
-    } else {  // COMB_AH = c0*ADL + c1*HS
+        br = p.chr + a * id;
+        bi = p.chi - k * a;
+        f = mask(ok, id);
+    } else {  // HS + ch*ADL = e*id2*[HS bracket - ch*rdnx*(id, -k)]
         double nxny = dotv<3>(nx, ny);
         double rdnx = dotv<3>(r, nx);
         double id2 = id * id;
         double g = rdnx * dotv<3>(r, ny) * id2 * id;
-        double hr = id * nxny + g * (k * k * d2s - 3.0);  // HS bracket (x id2)
-        double hi = 3.0 * k * d * g - k * nxny;
-        double a = -rdnx;                                 // ADL bracket (x id2)
-        double ar = a * id, ai = -k * a;
-        br = p.c0r * ar - p.c0i * ai + p.c1r * hr - p.c1i * hi;
-        bi = p.c0r * ai + p.c0i * ar + p.c1r * hi + p.c1i * hr;
-        scale = id2;
+        double ar = -rdnx * id, ai = k * rdnx;
+        br = id * nxny + g * (k * k * d2s - 3.0) + p.chr * ar - p.chi * ai;
+        bi = 3.0 * k * d * g - k * nxny + p.chr * ai + p.chi * ar;
+        f = mask(ok, id2);
     }
-    double cs = c * scale * kInv4Pi, ss = s * scale * kInv4Pi;
-    kr = cs * br - ss * bi;
-    ki = cs * bi + ss * br;
+    double cf = c * f, sf = s * f;
+    kr = cf * br - sf * bi;
+    ki = cf * bi + sf * br;
 }
 
-// Helmholtz 2D: complex kernel value via H0, H1 (H2 = 2 H1 / x - H0).
+// ---------------------------------------------------------------- Helmholtz 2D
+// K = (i/4) Z; raw chosen so that post = i/4 (SL), i k/4 (DL, HS),
+// -i k/4 (ADL), i c1 k/4 (COMB_SD with ch = c0/(c1 k), COMB_AH with
+// ch = c0/c1).
 template <int KIND, bool MIXED>
-__device__ __forceinline__ void helm2_pair(const double *r, double d2s, const double *nx,
-                                           const double *ny, const KParams &p, double &kr,
-                                           double &ki) {
+__device__ __forceinline__ void helm2_raw(const double *r, double d2s, bool ok, const double *nx,
+                                          const double *ny, const KParams &p, double &kr,
+                                          double &ki) {
     const double k = p.k;
     double id = rsqp<MIXED>(d2s);
     double x = k * (d2s * id);
     double h0r, h0i, h1r, h1i;
     hankel01<MIXED>(x, h0r, h0i, h1r, h1i);
-    // K = i * (Zr + i Zi) / 4  with Z the bracket below
-    double zr, zi;
     if constexpr (KIND == INTI_SL) {
-        zr = h0r; zi = h0i;
+        kr = mask(ok, h0r); ki = mask(ok, h0i);
     } else if constexpr (KIND == INTI_DL || KIND == INTI_ADL) {
-        double f = k * id * (KIND == INTI_DL ? dotv<2>(r, ny) : -dotv<2>(r, nx));
-        zr = f * h1r; zi = f * h1i;
+        double f = mask(ok, id * dotv<2>(r, KIND == INTI_DL ? ny : nx));
+        kr = f * h1r; ki = f * h1i;
     } else if constexpr (KIND == INTI_HS) {
         double ix = rcpv<MIXED>(x);
         double h2r = 2.0 * ix * h1r - h0r, h2i = 2.0 * ix * h1i - h0i;
-        double fa = -k * k * id * id * dotv<2>(r, nx) * dotv<2>(r, ny);
-        double fb = k * id * dotv<2>(nx, ny);
-        zr = fa * h2r + fb * h1r; zi = fa * h2i + fb * h1i;
-    } else if constexpr (KIND == INTI_COMB_SD) {
-        double f = k * id * dotv<2>(r, ny);
-        double dr = f * h1r, di = f * h1i;
-        zr = p.c0r * h0r - p.c0i * h0i + p.c1r * dr - p.c1i * di;
-        zi = p.c0r * h0i + p.c0i * h0r + p.c1r * di + p.c1i * dr;
-    } else {
+        double fa = mask(ok, -k * id * id * dotv<2>(r, nx) * dotv<2>(r, ny));
+        double fb = mask(ok, id * dotv<2>(nx, ny));
+        kr = fa * h2r + fb * h1r; ki = fa * h2i + fb * h1i;
+    } else if constexpr (KIND == INTI_COMB_SD) {  // rdny id H1 + ch H0
+        double f = id * dotv<2>(r, ny);
+        double zr = f * h1r + p.chr * h0r - p.chi * h0i;
+        double zi = f * h1i + p.chr * h0i + p.chi * h0r;
+        kr = mask(ok, zr); ki = mask(ok, zi);
+    } else {  // HS raw - ch * rdnx id H1
         double ix = rcpv<MIXED>(x);
         double rdnx = dotv<2>(r, nx);
         double h2r = 2.0 * ix * h1r - h0r, h2i = 2.0 * ix * h1i - h0i;
-        double fa = -k * k * id * id * rdnx * dotv<2>(r, ny);
-        double fb = k * id * dotv<2>(nx, ny);
-        double hr = fa * h2r + fb * h1r, hi = fa * h2i + fb * h1i;
-        double f = -k * id * rdnx;
-        double ar = f * h1r, ai = f * h1i;
-        zr = p.c0r * ar - p.c0i * ai + p.c1r * hr - p.c1i * hi;
-        zi = p.c0r * ai + p.c0i * ar + p.c1r * hi + p.c1i * hr;
+        double fa = -k * id * id * rdnx * dotv<2>(r, ny);
+        double fb = id * dotv<2>(nx, ny);
+        double fc = -rdnx * id;
+        double ar = fc * h1r, ai = fc * h1i;
+        double zr = fa * h2r + fb * h1r + p.chr * ar - p.chi * ai;
+        double zi = fa * h2i + fb * h1i + p.chr * ai + p.chi * ar;
+        kr = mask(ok, zr); ki = mask(ok, zi);
     }
-    kr = -0.25 * zi;
-    ki = 0.25 * zr;
 }
 
-// Unified entry: value of K(x, y) (zero for coincident points).
+// raw value of the pair (post applied by the caller)
 template <int PDE, int DIM, int KIND, bool MIXED>
-__device__ __forceinline__ void pair_value(const double *r, const double *nx, const double *ny,
-                                           const KParams &p, double &kr, double &ki) {
+__device__ __forceinline__ void pair_raw(const double *r, const double *nx, const double *ny,
+                                         const KParams &p, double &kr, double &ki) {
     double d2 = dotv<DIM>(r, r);
     bool ok = d2 > kSamePointTol2;
-    double d2s = ok ? d2 : 1.0;
+    double d2s = fmax(d2, kClampD2);
     if constexpr (PDE == INTI_LAPLACE) {
-        kr = laplace_pair<DIM, KIND, MIXED>(r, d2s, nx, ny, p);
+        kr = laplace_raw<DIM, KIND, MIXED>(r, d2s, ok, nx, ny, p);
         ki = 0.0;
     } else if constexpr (DIM == 3) {
-        helm3_pair<KIND, MIXED>(r, d2s, nx, ny, p, kr, ki);
+        helm3_raw<KIND, MIXED>(r, d2s, ok, nx, ny, p, kr, ki);
     } else {
-        helm2_pair<KIND, MIXED>(r, d2s, nx, ny, p, kr, ki);
+        helm2_raw<KIND, MIXED>(r, d2s, ok, nx, ny, p, kr, ki);
     }
-    kr = ok ? kr : 0.0;
-    ki = ok ? ki : 0.0;
+}
+
+inline KParams basic_params(double k) {
+    KParams p;
+    p.k = k;
+    p.kappa = k * kInv2Pi;
+    p.chr = p.chi = 0.0;
+    p.postr = 1.0; p.posti = 0.0;
+    return p;
+}
+
+// Kernel parameters of an operator and the kind actually evaluated (a
+// combined kind with c1 = 0 is its first component).
+KParams op_params(const Op *op, int *eff_kind);
+
+// post factor of the four basic kinds (host and device)
+__host__ __device__ inline void basic_post(int pde, int dim, int kind, double k, double &pr,
+                                           double &pi) {
+    pr = 0.0; pi = 0.0;
+    if (pde == INTI_LAPLACE) {
+        if (dim == 2) pr = kind == INTI_SL ? -0.5 * kInv2Pi : kind == INTI_ADL ? -kInv2Pi : kInv2Pi;
+        else pr = kind == INTI_ADL ? -kInv4Pi : kInv4Pi;
+    } else if (dim == 3) {
+        pr = kind == INTI_ADL ? -kInv4Pi : kInv4Pi;
+    } else {
+        pi = kind == INTI_SL ? 0.25 : kind == INTI_ADL ? -0.25 * k : 0.25 * k;
+    }
+}
+
+// Full value K = post * raw for the four basic kinds (setup kernels).
+template <int PDE, int DIM, int KIND>
+__device__ __forceinline__ void pair_value(const double *r, const double *nx, const double *ny,
+                                           const KParams &p, double &kr, double &ki) {
+    static_assert(KIND <= INTI_HS, "combined kinds carry their post factor in KParams");
+    double rr, ri, pr, pi;
+    pair_raw<PDE, DIM, KIND, false>(r, nx, ny, p, rr, ri);
+    basic_post(PDE, DIM, KIND, p.k, pr, pi);
+    kr = pr * rr - pi * ri;
+    ki = pr * ri + pi * rr;
 }
 
 }  // namespace inti
