@@ -1,0 +1,6 @@
+// Direct-sum kernels instantiated for one (PDE, dimension): INTI_HELMHOLTZ, 3D.
+#include "apply_kernel.hpp"
+
+namespace inti {
+ApplyFn apply_pick_helm3(int kind, int mixed, int nr) { return apply_pick_pd<INTI_HELMHOLTZ, 3>(kind, mixed, nr); }
+}  // namespace inti

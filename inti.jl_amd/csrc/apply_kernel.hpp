@@ -1,0 +1,191 @@
+// Tiled target x source direct sum (matrix-free IntegralOperator apply) for
+// NR right-hand sides at once.
+//
+// Layout: one thread per target, 256 targets per block; the source range is
+// cut into `split` chunks so that a launch has >> 256 workgroups. Each source
+// tile is read with coalesced SoA loads and staged in LDS: the geometry as a
+// packed record and q_j,r = w_j sigma_j,r for the NR right-hand sides; every
+// lane then reads them by broadcast. The kernel value of a pair is computed
+// once and applied to all NR densities. Block b works on chunk b % split, so
+// with split a multiple of 8 all blocks of one XCD (blocks are dealt to the 8
+// XCDs round robin) share one source chunk in their L2.
+#pragma once
+#include "common.hpp"
+#include "pair.hpp"
+
+namespace inti {
+
+struct ApplyArgs {
+    int64_t m, n;
+    const double *tx, *tn;
+    const double *sx, *sn, *sw;
+    const double *x;  // n x nr, leading dimension ldx (elements)
+    int64_t ldx;
+    double *part;     // nr x split x m partial sums
+    int split;
+    int64_t chunk;
+    KParams p;
+};
+
+// far-away padding source for the last tile (its density is zero)
+constexpr double kPadCoord = 1.0e3;
+constexpr int kApplyBS = 256;
+
+template <int NR>
+struct ApplyTile {
+    static constexpr int TT = NR == 1 ? 256 : NR <= 4 ? 128 : 64;
+};
+
+template <int PDE, int DIM, int KIND, bool MIXED, int NR>
+__global__ __launch_bounds__(kApplyBS) void k_apply(ApplyArgs a) {
+    constexpr bool CPLX = PDE == INTI_HELMHOLTZ;
+    constexpr int BS = kApplyBS;
+    constexpr int TT = ApplyTile<NR>::TT;
+    constexpr int NG = DIM == 3 ? 3 : 2;  // double2 slots of a geometry record
+    __shared__ double2 s_geo[TT * NG];
+    __shared__ double2 s_q[TT * NR];
+    const int tid = threadIdx.x;
+    const int64_t b = blockIdx.x;
+    const int64_t chunk = b % a.split;
+    const int64_t tile = b / a.split;
+    const int64_t m = a.m, n = a.n;
+
+    const int64_t i = tile * BS + tid;
+    const int64_t ic = i < m ? i : m - 1;
+    double xt[DIM], nt[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        xt[d] = a.tx[d * m + ic];
+        nt[d] = a.tn ? a.tn[d * m + ic] : 0.0;
+    }
+    double accr[NR], acci[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) { accr[r] = 0.0; acci[r] = 0.0; }
+    const int64_t j0 = chunk * a.chunk;
+    const int64_t j1 = j0 + a.chunk < n ? j0 + a.chunk : n;
+    for (int64_t s0 = j0; s0 < j1; s0 += TT) {
+        if (tid < TT) {
+            const int64_t j = s0 + tid;
+            double y[3] = {kPadCoord, kPadCoord, kPadCoord}, ny[3] = {0, 0, 0};
+            if (j < j1) {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    y[d] = a.sx[d * n + j];
+                    ny[d] = a.sn ? a.sn[d * n + j] : 0.0;
+                }
+            }
+            double2 *rec = s_geo + tid * NG;
+            if constexpr (DIM == 3) {
+                rec[0] = make_double2(y[0], y[1]);
+                rec[1] = make_double2(y[2], ny[0]);
+                rec[2] = make_double2(ny[1], ny[2]);
+            } else {
+                rec[0] = make_double2(y[0], y[1]);
+                rec[1] = make_double2(ny[0], ny[1]);
+            }
+        }
+        for (int q = tid; q < TT * NR; q += BS) {
+            const int jj = q % TT, r = q / TT;
+            const int64_t j = s0 + jj;
+            double qr = 0.0, qi = 0.0;
+            if (j < j1) {
+                const double w = a.sw[j];
+                if constexpr (CPLX) {
+                    const double *xp = a.x + 2 * (j + r * a.ldx);
+                    qr = w * xp[0];
+                    qi = w * xp[1];
+                } else {
+                    qr = w * a.x[j + r * a.ldx];
+                }
+            }
+            s_q[jj * NR + r] = make_double2(qr, qi);
+        }
+        __syncthreads();
+        // two pairs in flight for one right-hand side; one for many (the
+        // accumulators already fill the register file) and for the 2D
+        // Helmholtz kernels (the Hankel evaluation does)
+        constexpr int UNR = (NR == 1 && !(CPLX && DIM == 2)) ? 2 : 1;
+#pragma unroll UNR
+        for (int jj = 0; jj < TT; ++jj) {
+            const double2 *rc = s_geo + jj * NG;
+            double ys[DIM], ns[DIM];
+            if constexpr (DIM == 3) {
+                const double2 v0 = rc[0], v1 = rc[1], v2 = rc[2];
+                ys[0] = v0.x; ys[1] = v0.y; ys[2] = v1.x;
+                ns[0] = v1.y; ns[1] = v2.x; ns[2] = v2.y;
+            } else {
+                const double2 v0 = rc[0], v1 = rc[1];
+                ys[0] = v0.x; ys[1] = v0.y;
+                ns[0] = v1.x; ns[1] = v1.y;
+            }
+            double rv[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) rv[d] = xt[d] - ys[d];
+            double kr, ki;
+            pair_raw<PDE, DIM, KIND, MIXED>(rv, nt, ns, a.p, kr, ki);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const double2 q = s_q[jj * NR + r];
+                if constexpr (CPLX) {
+                    accr[r] = __builtin_fma(kr, q.x, accr[r]);
+                    accr[r] = __builtin_fma(-ki, q.y, accr[r]);
+                    acci[r] = __builtin_fma(kr, q.y, acci[r]);
+                    acci[r] = __builtin_fma(ki, q.x, acci[r]);
+                } else {
+                    accr[r] = __builtin_fma(kr, q.x, accr[r]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (i < m) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int64_t off = ((int64_t)r * a.split + chunk) * m + i;
+            if constexpr (CPLX) reinterpret_cast<double2 *>(a.part)[off] = make_double2(accr[r], acci[r]);
+            else a.part[off] = accr[r];
+        }
+    }
+}
+
+using ApplyFn = void (*)(const ApplyArgs &, dim3, hipStream_t);
+
+template <int PDE, int DIM, int KIND, bool MIXED, int NR>
+static void apply_launcher(const ApplyArgs &a, dim3 grid, hipStream_t s) {
+    hipLaunchKernelGGL((k_apply<PDE, DIM, KIND, MIXED, NR>), grid, dim3(kApplyBS), 0, s, a);
+}
+
+template <int PDE, int DIM, bool MIXED, int NR>
+static ApplyFn apply_kind(int kind) {
+    switch (kind) {
+        case INTI_SL: return apply_launcher<PDE, DIM, INTI_SL, MIXED, NR>;
+        case INTI_DL: return apply_launcher<PDE, DIM, INTI_DL, MIXED, NR>;
+        case INTI_ADL: return apply_launcher<PDE, DIM, INTI_ADL, MIXED, NR>;
+        case INTI_HS: return apply_launcher<PDE, DIM, INTI_HS, MIXED, NR>;
+        case INTI_COMB_SD: return apply_launcher<PDE, DIM, INTI_COMB_SD, MIXED, NR>;
+        case INTI_COMB_AH: return apply_launcher<PDE, DIM, INTI_COMB_AH, MIXED, NR>;
+    }
+    return nullptr;
+}
+
+// NR in {1, 4, 16}
+template <int PDE, int DIM>
+static ApplyFn apply_pick_pd(int kind, int mixed, int nr) {
+    if (mixed) {
+        if (nr == 1) return apply_kind<PDE, DIM, true, 1>(kind);
+        if (nr == 4) return apply_kind<PDE, DIM, true, 4>(kind);
+        if (nr == 16) return apply_kind<PDE, DIM, true, 16>(kind);
+    } else {
+        if (nr == 1) return apply_kind<PDE, DIM, false, 1>(kind);
+        if (nr == 4) return apply_kind<PDE, DIM, false, 4>(kind);
+        if (nr == 16) return apply_kind<PDE, DIM, false, 16>(kind);
+    }
+    return nullptr;
+}
+
+ApplyFn apply_pick_lap2(int kind, int mixed, int nr);
+ApplyFn apply_pick_lap3(int kind, int mixed, int nr);
+ApplyFn apply_pick_helm2(int kind, int mixed, int nr);
+ApplyFn apply_pick_helm3(int kind, int mixed, int nr);
+
+}  // namespace inti

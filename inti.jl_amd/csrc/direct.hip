@@ -1,132 +1,18 @@
-// Tiled target x source direct sum (matrix-free IntegralOperator apply)
-// and its epilogue pass (split reduction + fused CSR correction + alpha/beta),
-// plus the dense assembly kernel (assemble_matrix).
-//
-// Layout: one thread per target, BS targets per block; the source range is
-// cut into `split` chunks so that a launch has >> 256 workgroups. Each
-// source tile (BS sources) is read with coalesced SoA loads, multiplied by
-// the density (q_j = w_j * sigma_j) and staged in LDS as a packed record
-// that every lane then reads by broadcast. Block b works on chunk b % split,
-// so with split a multiple of 8 all blocks of one XCD (blocks are dealt to
-// the 8 XCDs round robin) share one source chunk in their L2.
+// Direct-sum driver: operator parameters (post factors), the choice of
+// source chunks, the multi-right-hand-side dispatch of k_apply
+// (apply_kernel.hpp), and the epilogue pass (split reduction + post factor +
+// fused CSR correction + identity + alpha/beta).
 #include <complex>
 
-#include "common.hpp"
-#include "pair.hpp"
+#include "apply_kernel.hpp"
 
 namespace inti {
-
-struct ApplyArgs {
-    int64_t m, n;
-    const double *tx, *tn;
-    const double *sx, *sn, *sw;
-    const double *x;
-    double *part;
-    int split;
-    int64_t chunk;
-    KParams p;
-};
-
-// far-away padding source for the last tile (its density is zero)
-constexpr double kPadCoord = 1.0e3;
-
-template <int PDE, int DIM, int KIND, bool MIXED, int BS>
-__global__ __launch_bounds__(BS) void k_apply(ApplyArgs a) {
-    constexpr bool CPLX = PDE == INTI_HELMHOLTZ;
-    constexpr int NR2 = DIM == 3 ? 4 : 3;  // double2 slots per source record
-    __shared__ double2 s_rec[BS * NR2];
-    const int tid = threadIdx.x;
-    const int64_t b = blockIdx.x;
-    const int64_t chunk = b % a.split;
-    const int64_t tile = b / a.split;
-    const int64_t m = a.m, n = a.n;
-
-    const int64_t i = tile * BS + tid;
-    const int64_t ic = i < m ? i : m - 1;
-    double xt[DIM], nt[DIM];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        xt[d] = a.tx[d * m + ic];
-        nt[d] = a.tn ? a.tn[d * m + ic] : 0.0;
-    }
-    double accr = 0.0, acci = 0.0;
-    const int64_t j0 = chunk * a.chunk;
-    const int64_t j1 = j0 + a.chunk < n ? j0 + a.chunk : n;
-    for (int64_t s0 = j0; s0 < j1; s0 += BS) {
-        const int64_t j = s0 + tid;
-        double y[3] = {kPadCoord, kPadCoord, kPadCoord}, ny[3] = {0, 0, 0};
-        double qr = 0.0, qi = 0.0;
-        if (j < j1) {
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                y[d] = a.sx[d * n + j];
-                ny[d] = a.sn ? a.sn[d * n + j] : 0.0;
-            }
-            const double w = a.sw[j];
-            if constexpr (CPLX) {
-                qr = w * a.x[2 * j];
-                qi = w * a.x[2 * j + 1];
-            } else {
-                qr = w * a.x[j];
-            }
-        }
-        double2 *rec = s_rec + tid * NR2;
-        if constexpr (DIM == 3) {
-            rec[0] = make_double2(y[0], y[1]);
-            rec[1] = make_double2(y[2], ny[0]);
-            rec[2] = make_double2(ny[1], ny[2]);
-            rec[3] = make_double2(qr, qi);
-        } else {
-            rec[0] = make_double2(y[0], y[1]);
-            rec[1] = make_double2(ny[0], ny[1]);
-            rec[2] = make_double2(qr, qi);
-        }
-        __syncthreads();
-#pragma unroll 2
-        for (int jj = 0; jj < BS; ++jj) {
-            const double2 *rc = s_rec + jj * NR2;
-            double ys[DIM], ns[DIM], q_r, q_i;
-            if constexpr (DIM == 3) {
-                double2 v0 = rc[0], v1 = rc[1], v2 = rc[2], v3 = rc[3];
-                ys[0] = v0.x; ys[1] = v0.y; ys[2] = v1.x;
-                ns[0] = v1.y; ns[1] = v2.x; ns[2] = v2.y;
-                q_r = v3.x; q_i = v3.y;
-            } else {
-                double2 v0 = rc[0], v1 = rc[1], v2 = rc[2];
-                ys[0] = v0.x; ys[1] = v0.y;
-                ns[0] = v1.x; ns[1] = v1.y;
-                q_r = v2.x; q_i = v2.y;
-            }
-            double r[DIM];
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) r[d] = xt[d] - ys[d];
-            double kr, ki;
-            pair_raw<PDE, DIM, KIND, MIXED>(r, nt, ns, a.p, kr, ki);
-            if constexpr (CPLX) {
-                accr = __builtin_fma(kr, q_r, accr);
-                accr = __builtin_fma(-ki, q_i, accr);
-                acci = __builtin_fma(kr, q_i, acci);
-                acci = __builtin_fma(ki, q_r, acci);
-            } else {
-                accr = __builtin_fma(kr, q_r, accr);
-            }
-        }
-        __syncthreads();
-    }
-    if (i < m) {
-        if constexpr (CPLX) {
-            double2 *pp = reinterpret_cast<double2 *>(a.part);
-            pp[chunk * m + i] = make_double2(accr, acci);
-        } else {
-            a.part[chunk * m + i] = accr;
-        }
-    }
-}
 
 struct ReduceArgs {
     int64_t m;
     int split;
-    const double *part;
+    const double *part;  // nr x split x m
+    int64_t ldx, ldy;    // leading dimensions of x and y (elements)
     const int32_t *rowptr;
     const int32_t *colind;
     const double *cval;
@@ -138,20 +24,25 @@ struct ReduceArgs {
     int has_id;
 };
 
-// y_i = alpha * (sum_c part[c][i] + (dA x)_i + cid x_i) + beta * y_i
+// y_i = alpha * (post * sum_c part[c][i] + (dA x)_i + cid x_i) + beta * y_i,
+// one right-hand side per blockIdx.y
 template <bool CPLX>
 __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= a.m) return;
+    const int r = blockIdx.y;
+    const double *x = a.x + (CPLX ? 2 : 1) * (int64_t)r * a.ldx;
+    double *y = a.y + (CPLX ? 2 : 1) * (int64_t)r * a.ldy;
+    const int64_t pbase = (int64_t)r * a.split * a.m;
     double sr = 0.0, si = 0.0;
     if constexpr (CPLX) {
-        const double2 *pp = reinterpret_cast<const double2 *>(a.part);
+        const double2 *pp = reinterpret_cast<const double2 *>(a.part) + pbase;
         for (int c = 0; c < a.split; ++c) {
             double2 v = pp[(int64_t)c * a.m + i];
             sr += v.x; si += v.y;
         }
     } else {
-        for (int c = 0; c < a.split; ++c) sr += a.part[(int64_t)c * a.m + i];
+        for (int c = 0; c < a.split; ++c) sr += a.part[pbase + (int64_t)c * a.m + i];
     }
     {
         const double tr = a.pr * sr - a.pi * si, ti = a.pr * si + a.pi * sr;
@@ -164,70 +55,45 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
             const int32_t j = a.colind[p];
             if constexpr (CPLX) {
                 const double vr = a.cval[2 * p], vi = a.cval[2 * p + 1];
-                const double xr = a.x[2 * j], xi = a.x[2 * j + 1];
+                const double xr = x[2 * j], xi = x[2 * j + 1];
                 sr += vr * xr - vi * xi;
                 si += vr * xi + vi * xr;
             } else {
-                sr += a.cval[p] * a.x[j];
+                sr += a.cval[p] * x[j];
             }
         }
     }
     if (a.has_id) {
         if constexpr (CPLX) {
-            const double xr = a.x[2 * i], xi = a.x[2 * i + 1];
+            const double xr = x[2 * i], xi = x[2 * i + 1];
             sr += a.cr * xr - a.ci * xi;
             si += a.cr * xi + a.ci * xr;
         } else {
-            sr += a.cr * a.x[i];
+            sr += a.cr * x[i];
         }
     }
     if constexpr (CPLX) {
         double yr = a.ar * sr - a.ai * si, yi = a.ar * si + a.ai * sr;
         if (!a.beta_zero) {
-            const double or_ = a.y[2 * i], oi = a.y[2 * i + 1];
+            const double or_ = y[2 * i], oi = y[2 * i + 1];
             yr += a.br * or_ - a.bi * oi;
             yi += a.br * oi + a.bi * or_;
         }
-        a.y[2 * i] = yr;
-        a.y[2 * i + 1] = yi;
+        y[2 * i] = yr;
+        y[2 * i + 1] = yi;
     } else {
         double yr = a.ar * sr;
-        if (!a.beta_zero) yr += a.br * a.y[i];
-        a.y[i] = yr;
+        if (!a.beta_zero) yr += a.br * y[i];
+        y[i] = yr;
     }
 }
 
 // ------------------------------------------------------------- dispatch
-constexpr int kApplyBS = 256;
-using ApplyFn = void (*)(const ApplyArgs &, dim3, hipStream_t);
-
-template <int PDE, int DIM, int KIND, bool MIXED>
-static void apply_launcher(const ApplyArgs &a, dim3 grid, hipStream_t s) {
-    hipLaunchKernelGGL((k_apply<PDE, DIM, KIND, MIXED, kApplyBS>), grid, dim3(kApplyBS), 0, s, a);
-}
-
-template <int PDE, int DIM, bool MIXED>
-static ApplyFn pick_kind(int kind) {
-    switch (kind) {
-        case INTI_SL: return apply_launcher<PDE, DIM, INTI_SL, MIXED>;
-        case INTI_DL: return apply_launcher<PDE, DIM, INTI_DL, MIXED>;
-        case INTI_ADL: return apply_launcher<PDE, DIM, INTI_ADL, MIXED>;
-        case INTI_HS: return apply_launcher<PDE, DIM, INTI_HS, MIXED>;
-        case INTI_COMB_SD: return apply_launcher<PDE, DIM, INTI_COMB_SD, MIXED>;
-        case INTI_COMB_AH: return apply_launcher<PDE, DIM, INTI_COMB_AH, MIXED>;
-    }
-    return nullptr;
-}
-
-static ApplyFn pick_apply(int pde, int dim, int kind, int mixed) {
-#define INTI_PICK(P, D)                                                          \
-    if (pde == P && dim == D) return mixed ? pick_kind<P, D, true>(kind)          \
-                                           : pick_kind<P, D, false>(kind);
-    INTI_PICK(INTI_LAPLACE, 2)
-    INTI_PICK(INTI_LAPLACE, 3)
-    INTI_PICK(INTI_HELMHOLTZ, 2)
-    INTI_PICK(INTI_HELMHOLTZ, 3)
-#undef INTI_PICK
+static ApplyFn pick_apply(int pde, int dim, int kind, int mixed, int nr) {
+    if (pde == INTI_LAPLACE && dim == 2) return apply_pick_lap2(kind, mixed, nr);
+    if (pde == INTI_LAPLACE && dim == 3) return apply_pick_lap3(kind, mixed, nr);
+    if (pde == INTI_HELMHOLTZ && dim == 2) return apply_pick_helm2(kind, mixed, nr);
+    if (pde == INTI_HELMHOLTZ && dim == 3) return apply_pick_helm3(kind, mixed, nr);
     return nullptr;
 }
 
@@ -273,6 +139,8 @@ KParams op_params(const Op *op, int *eff_kind) {
 // number of source chunks: enough workgroups to fill 256 CUs several
 // times over, a multiple of the 8 XCDs when more than 8.
 static void plan_split(const Op *op, int64_t m, int64_t n, int &split, int64_t &chunk) {
+    // chunks are whole 256-source tiles, which also covers the smaller
+    // tiles of the multi-right-hand-side kernels
     const int64_t ntiles = (m + kApplyBS - 1) / kApplyBS;
     const int64_t stiles = (n + kApplyBS - 1) / kApplyBS;
     int64_t s = op->split;
@@ -297,25 +165,28 @@ int launch_apply(const DirectLaunch &L) {
     if (m == 0 || L.nrhs == 0) return INTI_OK;
     int kind = op->kind;
     const KParams kp = op_params(op, &kind);
-    ApplyFn fn = pick_apply(op->pde, op->dim, kind, op->precision == INTI_MIXED);
-    if (!fn) return set_error(ctx, INTI_ERR_UNSUPPORTED, "no direct-sum kernel for this operator");
     int split;
     int64_t chunk;
     plan_split(op, m, n, split, chunk);
+    const int nrmax = L.nrhs >= 16 ? 16 : L.nrhs >= 4 ? 4 : 1;
     void *scr = nullptr;
-    int st = ctx_scratch(ctx, kScrPartial, sizeof(double) * cw * (size_t)m * split, &scr);
+    int st = ctx_scratch(ctx, kScrPartial, sizeof(double) * cw * (size_t)m * split * nrmax, &scr);
     if (st) return st;
     double *part = (double *)scr;
     const int64_t ntiles = (m + kApplyBS - 1) / kApplyBS;
-    for (int r = 0; r < L.nrhs; ++r) {
-        const double *xr = L.x + (int64_t)r * L.ldx * cw;
-        double *yr = L.y + (int64_t)r * L.ldy * cw;
+    for (int r0 = 0; r0 < L.nrhs;) {
+        const int left = L.nrhs - r0;
+        const int nr = left >= 16 ? 16 : left >= 4 ? 4 : 1;
+        const double *xr = L.x + (int64_t)r0 * L.ldx * cw;
+        double *yr = L.y + (int64_t)r0 * L.ldy * cw;
         if (n > 0) {
+            ApplyFn fn = pick_apply(op->pde, op->dim, kind, op->precision == INTI_MIXED, nr);
+            if (!fn) return set_error(ctx, INTI_ERR_UNSUPPORTED, "no direct-sum kernel for this operator");
             ApplyArgs a;
             a.m = m; a.n = n;
             a.tx = op->trg->coords; a.tn = op->trg->normals;
             a.sx = op->src->coords; a.sn = op->src->normals; a.sw = op->src->weights;
-            a.x = xr; a.part = part; a.split = split; a.chunk = chunk;
+            a.x = xr; a.ldx = L.ldx; a.part = part; a.split = split; a.chunk = chunk;
             a.p = kp;
             dim3 grid((unsigned)(ntiles * split));
             if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
@@ -325,6 +196,7 @@ int launch_apply(const DirectLaunch &L) {
         }
         ReduceArgs ra;
         ra.m = m; ra.split = n > 0 ? split : 0; ra.part = part;
+        ra.ldx = L.ldx; ra.ldy = L.ldy;
         ra.rowptr = op->corr ? op->corr->rowptr : nullptr;
         ra.colind = op->corr ? op->corr->colind : nullptr;
         ra.cval = op->corr ? op->corr->val : nullptr;
@@ -335,12 +207,11 @@ int launch_apply(const DirectLaunch &L) {
         ra.cr = op->cid[0]; ra.ci = op->cid[1];
         ra.pr = kp.postr; ra.pi = kp.posti;
         ra.has_id = (op->cid[0] != 0.0 || op->cid[1] != 0.0);
-        dim3 rg((unsigned)((m + 255) / 256));
-        if (m > 0) {
-            if (cplx) hipLaunchKernelGGL(k_reduce<true>, rg, dim3(256), 0, ctx->stream, ra);
-            else hipLaunchKernelGGL(k_reduce<false>, rg, dim3(256), 0, ctx->stream, ra);
-            INTI_HIP(ctx, hipGetLastError());
-        }
+        dim3 rg((unsigned)((m + 255) / 256), (unsigned)nr);
+        if (cplx) hipLaunchKernelGGL(k_reduce<true>, rg, dim3(256), 0, ctx->stream, ra);
+        else hipLaunchKernelGGL(k_reduce<false>, rg, dim3(256), 0, ctx->stream, ra);
+        INTI_HIP(ctx, hipGetLastError());
+        r0 += nr;
     }
     return INTI_OK;
 }

@@ -88,6 +88,26 @@ def test_apply_alpha_beta_identity_multirhs(inti, oracle, geoms):
     assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) <= 1e-12
 
 
+@pytest.mark.parametrize("pde,dim,kind", [(HEL, 2, 4), (HEL, 3, 4), (LAP, 3, 0), (LAP, 2, 5)])
+@pytest.mark.parametrize("nrhs", [5, 16, 21])
+def test_apply_multi_rhs(inti, oracle, geoms, pde, dim, kind, nrhs):
+    """Blocks of 16, 4 and 1 right-hand sides sharing each kernel value."""
+    say(f"[gpu] multi-rhs pde={pde} dim={dim} kind={KNAMES[kind]} nrhs={nrhs}")
+    Q = geoms[dim]
+    k = _k(pde, dim)
+    K, coefs = _kernel(inti, pde, dim, kind, k)
+    rng = np.random.default_rng(nrhs)
+    n = len(Q)
+    X = rng.normal(size=(n, nrhs))
+    if pde == HEL:
+        X = X + 1j * rng.normal(size=(n, nrhs))
+    ref = oracle.matvec(pde, dim, kind, k, Q, Q, X, coefs=coefs)
+    for prec, tol in (("fp64", 1e-12), ("mixed", 1e-6)):
+        Y = inti.IntegralOperator(K, Q, Q, precision=prec).apply(X)
+        err = np.linalg.norm(Y - ref, axis=0) / np.linalg.norm(ref, axis=0)
+        assert err.max() <= tol, (prec, err)
+
+
 def test_apply_split_and_padding(inti, oracle):
     """Several source chunks and a ragged last tile (N not a multiple of 256)."""
     say("[gpu] apply split/padding N=4320")
