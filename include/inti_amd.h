@@ -151,6 +151,28 @@ int inti_bdim_build(inti_op *opS, inti_op *opD, int derivative, int nq,
                     const double *params, inti_csr **dS, inti_csr **dD,
                     double *diag);
 
+/* ----------------------------------------------------------------- solvers */
+/* Restarted GMRES(restart) for the square operator op (with its attached
+ * correction and identity term), the solve of user code such as
+ * gmres(L, rhs; restart, maxiter) on L = I/2 + D - i k S
+ * (docs/src/examples/helmholtz_scattering.jl:230,494-495). Stops when
+ * ||b - A x|| <= max(rtol ||b||, atol) or after maxiter Arnoldi steps.
+ * x holds the initial guess and receives the solution. on_device: b and x
+ * are device pointers (else host). use_graph: replay one Arnoldi step as a
+ * captured hipGraph. iters/relres (optional): steps taken, final
+ * ||b - A x|| / ||b||. */
+int inti_gmres(inti_op *op, int on_device, const double *b, double *x,
+               int restart, int maxiter, double rtol, double atol,
+               int use_graph, int *iters, double *relres);
+/* Iterative refinement: residuals with op_hi (e.g. FP64), corrections by
+ * GMRES on op_lo (e.g. MIXED) to inner_rtol relative to the current
+ * residual, until ||b - A_hi x|| <= rtol ||b|| or max_outer sweeps. */
+int inti_refine_solve(inti_op *op_hi, inti_op *op_lo, int on_device,
+                      const double *b, double *x, int restart,
+                      int maxiter_inner, double inner_rtol, int max_outer,
+                      double rtol, int use_graph, int *outer,
+                      int *inner_total, double *relres);
+
 /* --------------------------------------------------------------------- CSR */
 int inti_csr_info(const inti_csr *csr, int64_t *m, int64_t *n, int64_t *nnz,
                   int *is_complex);

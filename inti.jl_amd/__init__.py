@@ -13,7 +13,8 @@ from .bdim import DimParameters, bdim_correction, etype_to_nearest_points
 from .kernels import (AdjointDoubleLayerKernel, CombinedFieldKernel, DoubleLayerKernel,
                       Helmholtz, HyperSingularKernel, Laplace, SingleLayerKernel,
                       default_density_eltype, default_kernel_eltype)
-from .nystrom import IntegralOperator, SparseCorrection, _green_multiplier, assemble_matrix
+from .nystrom import (IntegralOperator, SparseCorrection, _green_multiplier, assemble_matrix, gmres,
+                      refine_solve)
 from .quadrature import (Quadrature, circle_quadrature, ellipse_quadrature, gauss_legendre,
                          gauss_legendre_order, points, sphere_quadrature,
                          vioreanu_rokhlin_triangle)
@@ -29,6 +30,7 @@ __all__ = [
     "SparseCorrection", "_green_multiplier", "adj_double_layer_hypersingular",
     "assemble_matrix", "bdim_correction", "circle_quadrature", "combined_field",
     "default_context", "default_density_eltype", "default_kernel_eltype",
-    "ellipse_quadrature", "etype_to_nearest_points", "gauss_legendre", "gauss_legendre_order",
+    "ellipse_quadrature", "etype_to_nearest_points", "gauss_legendre", "gauss_legendre_order", "gmres",
+    "refine_solve",
     "points", "single_double_layer", "sphere_quadrature", "vioreanu_rokhlin_triangle",
 ]

@@ -476,6 +476,67 @@ int inti_bdim_build(inti_op *opS, inti_op *opD, int derivative, int nq, int64_t 
     return INTI_OK;
 }
 
+// ----------------------------------------------------------------- solvers
+// host pointers: stage b and x on the device for the solve
+struct Staged {
+    void *b = nullptr, *x = nullptr;
+    ~Staged() {
+        if (b) (void)hipFree(b);
+        if (x) (void)hipFree(x);
+    }
+};
+
+static int stage_in(Ctx *ctx, Staged &S, size_t bytes, const double *b, const double *x) {
+    if (hipMalloc(&S.b, bytes) != hipSuccess || hipMalloc(&S.x, bytes) != hipSuccess)
+        return set_error(ctx, INTI_ERR_NOMEM, "solve: allocation failed");
+    INTI_HIP(ctx, hipMemcpyAsync(S.b, b, bytes, hipMemcpyHostToDevice, ctx->stream));
+    INTI_HIP(ctx, hipMemcpyAsync(S.x, x, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return INTI_OK;
+}
+
+int inti_gmres(inti_op *op, int on_device, const double *b, double *x, int restart, int maxiter,
+               double rtol, double atol, int use_graph, int *iters, double *relres) {
+    if (!op) return INTI_ERR_ARG;
+    Ctx *ctx = op->ctx;
+    REQUIRE(ctx, b && x, "gmres: null vector");
+    REQUIRE(ctx, rtol >= 0 && atol >= 0, "gmres: negative tolerance");
+    LOCK(ctx);
+    if (on_device) return gmres_solve(op, b, x, restart, maxiter, rtol, atol, use_graph, iters, relres);
+    const size_t bytes = sizeof(double) * (op_is_complex(op) ? 2 : 1) * op->src->n;
+    Staged S;
+    int st = stage_in(ctx, S, bytes, b, x);
+    if (st) return st;
+    st = gmres_solve(op, (const double *)S.b, (double *)S.x, restart, maxiter, rtol, atol, use_graph, iters, relres);
+    if (st) return st;
+    INTI_HIP(ctx, hipMemcpyAsync(x, S.x, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return INTI_OK;
+}
+
+int inti_refine_solve(inti_op *op_hi, inti_op *op_lo, int on_device, const double *b, double *x,
+                      int restart, int maxiter_inner, double inner_rtol, int max_outer, double rtol,
+                      int use_graph, int *outer, int *inner_total, double *relres) {
+    if (!op_hi || !op_lo) return INTI_ERR_ARG;
+    Ctx *ctx = op_hi->ctx;
+    REQUIRE(ctx, op_lo->ctx == ctx, "refine: operators on different contexts");
+    REQUIRE(ctx, b && x, "refine: null vector");
+    REQUIRE(ctx, restart >= 1 && maxiter_inner >= 1 && max_outer >= 0, "refine: bad iteration limits");
+    LOCK(ctx);
+    if (on_device)
+        return refine_solve(op_hi, op_lo, b, x, restart, maxiter_inner, inner_rtol, max_outer, rtol, use_graph,
+                            outer, inner_total, relres);
+    const size_t bytes = sizeof(double) * (op_is_complex(op_hi) ? 2 : 1) * op_hi->src->n;
+    Staged S;
+    int st = stage_in(ctx, S, bytes, b, x);
+    if (st) return st;
+    st = refine_solve(op_hi, op_lo, (const double *)S.b, (double *)S.x, restart, maxiter_inner, inner_rtol,
+                      max_outer, rtol, use_graph, outer, inner_total, relres);
+    if (st) return st;
+    INTI_HIP(ctx, hipMemcpyAsync(x, S.x, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return INTI_OK;
+}
+
 // --------------------------------------------------------------------- CSR
 int inti_csr_info(const inti_csr *csr, int64_t *m, int64_t *n, int64_t *nnz, int *is_complex) {
     if (!csr) return INTI_ERR_ARG;

@@ -95,6 +95,13 @@ struct DirectLaunch {
     double alpha[2], beta[2];
 };
 int launch_apply(const DirectLaunch &L);
+// allocate the scratch an apply of nrhs columns needs (before graph capture)
+int apply_reserve(const Op *op, int nrhs);
+int gmres_solve(Op *op, const double *b, double *x, int restart, int maxiter, double rtol,
+                double atol, int use_graph, int *iters, double *relres);
+int refine_solve(Op *op_hi, Op *op_lo, const double *b, double *x, int restart, int maxiter_inner,
+                 double inner_rtol, int max_outer, double rtol, int use_graph, int *outer,
+                 int *inner_total, double *relres);
 int launch_assemble(const Op *op, double *out_dev);
 
 bool op_is_complex(const Op *op);

@@ -156,6 +156,17 @@ static void plan_split(const Op *op, int64_t m, int64_t n, int &split, int64_t &
     if (split < 1) split = 1;
 }
 
+int apply_reserve(const Op *op, int nrhs) {
+    const int cw = op_is_complex(op) ? 2 : 1;
+    const int64_t m = op->trg->n, n = op->src->n;
+    int split;
+    int64_t chunk;
+    plan_split(op, m, n, split, chunk);
+    const int nrmax = nrhs >= 16 ? 16 : nrhs >= 4 ? 4 : 1;
+    void *scr = nullptr;
+    return ctx_scratch(op->ctx, kScrPartial, sizeof(double) * cw * (size_t)m * split * nrmax, &scr);
+}
+
 int launch_apply(const DirectLaunch &L) {
     const Op *op = L.op;
     Ctx *ctx = op->ctx;

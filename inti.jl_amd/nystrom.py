@@ -160,3 +160,40 @@ def assemble_matrix(iop):
 
 def is_helmholtz(op):
     return isinstance(op, Helmholtz)
+
+
+def _iop(A):
+    return A.iop if hasattr(A, "iop") else A
+
+
+def gmres(A, b, x0=None, *, restart=100, maxiter=100, rtol=1e-6, atol=0.0, use_graph=True):
+    """Restarted GMRES on the GPU for the square operator A (IntegralOperator
+    or corrected operator): the `gmres(L, rhs; restart, maxiter)` of
+    IterativeSolvers used on Inti's operators
+    (docs/src/examples/helmholtz_scattering.jl:494-495). Stops when
+    ||b - A x|| <= max(rtol ||b||, atol). Returns (x, info)."""
+    A = _iop(A)
+    b = np.ascontiguousarray(b, dtype=A.dtype)
+    x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, dtype=A.dtype).copy()
+    it = C.c_int()
+    rr = C.c_double()
+    A.ctx.check(A.ctx.lib.inti_gmres(A.handle, 0, b.ctypes.data_as(C.c_void_p),
+                                     x.ctypes.data_as(C.c_void_p), restart, maxiter, rtol, atol,
+                                     1 if use_graph else 0, C.byref(it), C.byref(rr)))
+    return x, {"iterations": it.value, "relres": rr.value}
+
+
+def refine_solve(A_hi, A_lo, b, x0=None, *, restart=100, maxiter_inner=100, inner_rtol=1e-6,
+                 max_outer=10, rtol=1e-10, use_graph=True):
+    """Iterative refinement: FP64 residuals with A_hi, GMRES corrections with
+    A_lo (e.g. the MIXED operator). Returns (x, info)."""
+    A_hi, A_lo = _iop(A_hi), _iop(A_lo)
+    b = np.ascontiguousarray(b, dtype=A_hi.dtype)
+    x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, dtype=A_hi.dtype).copy()
+    o, tot = C.c_int(), C.c_int()
+    rr = C.c_double()
+    A_hi.ctx.check(A_hi.ctx.lib.inti_refine_solve(
+        A_hi.handle, A_lo.handle, 0, b.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p),
+        restart, maxiter_inner, inner_rtol, max_outer, rtol, 1 if use_graph else 0, C.byref(o),
+        C.byref(tot), C.byref(rr)))
+    return x, {"outer": o.value, "inner_iterations": tot.value, "relres": rr.value}

@@ -1,0 +1,90 @@
+"""GPU GMRES and mixed-precision iterative refinement against a dense FP64
+solve of the oracle's corrected matrix."""
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+HEL, LAP = 1, 0
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _dense_cfie(oracle, Q, k):
+    """I/2 + (D + dD) - i k (S + dS), all from the oracle."""
+    n = len(Q)
+    mu = np.full(n, -0.5)
+    rp, ci, vS, vD, _ = oracle.bdim(HEL, 3, k, Q, Q, mu)
+    A = oracle.assemble(HEL, 3, 4, k, Q, Q, coefs=(-1j * k, 1.0))
+    A = A + sp.csr_matrix((vD - 1j * k * vS, ci, rp), shape=(n, n)).toarray() + 0.5 * np.eye(n)
+    return A
+
+
+@pytest.fixture(scope="module")
+def cfie(inti, oracle):
+    Q = inti.sphere_quadrature(3, base="ico")
+    k = 2.0
+    op = inti.Helmholtz(k=k, dim=3)
+    L64, corr = inti.combined_field(op=op, quad=Q, c_single=-1j * k, c_double=1.0, c_identity=0.5)
+    Lmx, _ = inti.combined_field(op=op, quad=Q, c_single=-1j * k, c_double=1.0, c_identity=0.5,
+                                 precision="mixed", corrections=corr)
+    A = _dense_cfie(oracle, Q, k)
+    # plane-wave Dirichlet data of a sound-soft scatterer
+    b = -np.exp(1j * k * Q.coords[2])
+    x_ref = np.linalg.solve(A, b)
+    return Q, L64, Lmx, A, b, x_ref
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_gmres_fp64_matches_dense_solve(inti, cfie, use_graph):
+    say(f"[gpu] gmres fp64 graph={use_graph}")
+    Q, L64, Lmx, A, b, x_ref = cfie
+    x, info = inti.gmres(L64, b, restart=30, maxiter=200, rtol=1e-13, use_graph=use_graph)
+    assert info["relres"] <= 1e-13
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= 1e-10
+    # the residual the GPU reports is the true one
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) <= 2e-13
+
+
+def test_gmres_graph_replay_is_deterministic(inti, cfie):
+    say("[gpu] gmres graph vs eager")
+    Q, L64, Lmx, A, b, x_ref = cfie
+    x1, i1 = inti.gmres(L64, b, restart=20, maxiter=60, rtol=1e-9, use_graph=False)
+    x2, i2 = inti.gmres(L64, b, restart=20, maxiter=60, rtol=1e-9, use_graph=True)
+    x3, i3 = inti.gmres(L64, b, restart=20, maxiter=60, rtol=1e-9, use_graph=True)
+    assert i1["iterations"] == i2["iterations"] == i3["iterations"]
+    assert np.array_equal(x1, x2) and np.array_equal(x2, x3)
+
+
+def test_refined_mixed_solve_matches_fp64_oracle(inti, cfie):
+    """FP64 residual + MIXED GMRES corrections reach the FP64 solution to 1e-10."""
+    say("[gpu] iterative refinement")
+    Q, L64, Lmx, A, b, x_ref = cfie
+    x, info = inti.refine_solve(L64, Lmx, b, restart=30, maxiter_inner=100, inner_rtol=1e-6,
+                                max_outer=10, rtol=1e-12)
+    assert info["relres"] <= 1e-12
+    assert info["outer"] >= 2  # the mixed operator alone cannot reach 1e-12
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= 1e-10
+
+
+def test_gmres_laplace_first_kind(inti, oracle):
+    """Real arithmetic path: 3D Laplace single layer S sigma = f."""
+    say("[gpu] gmres laplace SL")
+    Q = inti.sphere_quadrature(2, base="ico")
+    op = inti.Laplace(dim=3)
+    S, D = inti.single_double_layer(op=op, target=Q, source=Q, compression={"method": "gpu"},
+                                    correction={"method": "dim"})
+    n = len(Q)
+    f = 1.0 + Q.coords[0] + 0.5 * Q.coords[1] * Q.coords[2]
+    x, info = inti.gmres(S, f, restart=50, maxiter=400, rtol=1e-12)
+    mu = np.full(n, -0.5)
+    rp, ci, vS, vD, _ = oracle.bdim(LAP, 3, 0.0, Q, Q, mu)
+    Sd = oracle.assemble(LAP, 3, 0, 0.0, Q, Q).real + sp.csr_matrix((vS.real, ci, rp), shape=(n, n)).toarray()
+    x_ref = np.linalg.solve(Sd, f)
+    assert info["relres"] <= 1e-12
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= 1e-9
