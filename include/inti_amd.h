@@ -151,6 +151,25 @@ int inti_bdim_build(inti_op *opS, inti_op *opD, int derivative, int nq,
                     const double *params, inti_csr **dS, inti_csr **dD,
                     double *diag);
 
+/* Batched per-element least-squares correction: the solve-and-scatter step
+ * of vdim_correction (src/vdim.jl:84-127). Element e has a real
+ * nrows x ncols matrix A_e (column-major, elements consecutive in A; in
+ * vdim the interpolation basis at the element's nodes), and every target i
+ * listed for e has the right-hand side row rhs[i, :] of the m x nrows
+ * column-major rhs (complex interleaved if is_complex). The least-squares
+ * solution w of A_e w = rhs[i, :] gives the CSR entries (i, cols[c, e]) =
+ * w[c] (duplicates summed). The reference solves with an SVD; for a
+ * full-rank A_e Householder QR gives the same solution: the least-squares one
+ * when ncols <= nrows, the minimum-norm one (QR of A_e^T) when ncols > nrows.
+ * Sizes: min(nrows, ncols) <= 24, max(nrows, ncols) <= 32.
+ * near_ptr/near_idx: element -> targets, or NULL for
+ * the self pattern (m == n, the targets of e are cols[:, e]).
+ * diag (optional, 2 doubles): max R-diagonal ratio, max |w|. */
+int inti_lsq_build(inti_ctx *ctx, int is_complex, int nrows, int ncols,
+                   int64_t ne, const double *A, const int64_t *cols, int64_t m,
+                   int64_t n, const int64_t *near_ptr, const int64_t *near_idx,
+                   const double *rhs, inti_csr **out, double *diag);
+
 /* ----------------------------------------------------------------- solvers */
 /* Restarted GMRES(restart) for the square operator op (with its attached
  * correction and identity term), the solve of user code such as

@@ -2,22 +2,27 @@
 
 Host-side mirror of the Inti.jl interface for this path (Quadrature,
 Laplace/Helmholtz kernels, IntegralOperator, assemble_matrix,
-bdim_correction, single_double_layer) over the C-ABI library
+bdim_correction, single_double_layer, vdim_correction, volume_potential,
+gmres) over the C-ABI library
 libinti_amd.so (hand-written gfx950 HIP kernels). The directory name is not a
 Python identifier; load it with load_package() from __graft_entry__.
 """
 from . import _lib
 from .api import (CorrectedOperator, adj_double_layer_hypersingular, combined_field,
-                  single_double_layer)
+                  single_double_layer, single_double_layer_potential)
 from .bdim import DimParameters, bdim_correction, etype_to_nearest_points
 from .kernels import (AdjointDoubleLayerKernel, CombinedFieldKernel, DoubleLayerKernel,
                       Helmholtz, HyperSingularKernel, Laplace, SingleLayerKernel,
                       default_density_eltype, default_kernel_eltype)
-from .nystrom import (IntegralOperator, SparseCorrection, _green_multiplier, assemble_matrix, gmres,
-                      refine_solve)
-from .quadrature import (Quadrature, circle_quadrature, ellipse_quadrature, gauss_legendre,
-                         gauss_legendre_order, points, sphere_quadrature,
+from .nystrom import (DoubleLayerPotential, IntegralOperator, IntegralPotential,
+                      SingleLayerPotential, SparseCorrection, _green_multiplier, assemble_matrix,
+                      gmres, refine_solve)
+from .quadrature import (Quadrature, boundary_quadrature, circle_quadrature, curve_quadrature,
+                         disk_quadratures, ellipse_disk_quadratures, ellipse_quadrature,
+                         gauss_legendre, gauss_legendre_order, points, sphere_quadrature,
                          vioreanu_rokhlin_triangle)
+from . import polynomials
+from .vdim import vdim_auxiliary_quantities, vdim_correction, volume_potential
 
 IntiError = _lib.IntiError
 default_context = _lib.default_context
@@ -29,6 +34,10 @@ __all__ = [
     "IntegralOperator", "IntiError", "Laplace", "Quadrature", "SingleLayerKernel",
     "SparseCorrection", "_green_multiplier", "adj_double_layer_hypersingular",
     "assemble_matrix", "bdim_correction", "circle_quadrature", "combined_field",
+    "disk_quadratures", "ellipse_disk_quadratures", "boundary_quadrature", "curve_quadrature",
+    "polynomials", "vdim_auxiliary_quantities", "vdim_correction",
+    "volume_potential", "IntegralPotential", "SingleLayerPotential", "DoubleLayerPotential",
+    "single_double_layer_potential",
     "default_context", "default_density_eltype", "default_kernel_eltype",
     "ellipse_quadrature", "etype_to_nearest_points", "gauss_legendre", "gauss_legendre_order", "gmres",
     "refine_solve",

@@ -57,6 +57,8 @@ SIGNATURES = {
     "inti_op_detach_correction": (C.c_int, [_vp]),
     "inti_bdim_build": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int64, _i64p, _i64p, _i64p,
                                   _dp, _dp, C.POINTER(_vp), C.POINTER(_vp), _dp]),
+    "inti_lsq_build": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int64, _dp, _i64p, C.c_int64,
+                                 C.c_int64, _i64p, _i64p, _dp, C.POINTER(_vp), _dp]),
     "inti_gmres": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_double, C.c_double,
                              C.c_int, C.POINTER(C.c_int), _dp]),
     "inti_refine_solve": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_double,

@@ -15,7 +15,8 @@ import numpy as np
 from .bdim import DimParameters, bdim_correction, etype_to_nearest_points
 from .kernels import (AdjointDoubleLayerKernel, CombinedFieldKernel, DoubleLayerKernel,
                       Helmholtz, HyperSingularKernel, SingleLayerKernel)
-from .nystrom import IntegralOperator, _green_multiplier, assemble_matrix
+from .nystrom import (DoubleLayerPotential, IntegralOperator, SingleLayerPotential,
+                      _green_multiplier, assemble_matrix)
 
 COMPRESSION_METHODS = ("none", "gpu")
 CORRECTION_METHODS = ("none", "dim")
@@ -102,6 +103,11 @@ def single_double_layer(*, op, target, source, compression=None, correction=None
         Sop.attach(dS)
         Dop.attach(dD)
     return CorrectedOperator(Sop), CorrectedOperator(Dop)
+
+
+def single_double_layer_potential(*, op, source):
+    """(single-layer, double-layer) IntegralPotentials (api.jl:220-226)."""
+    return SingleLayerPotential(op, source), DoubleLayerPotential(op, source)
 
 
 def adj_double_layer_hypersingular(*, op, target, source=None, compression=None,

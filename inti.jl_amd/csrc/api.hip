@@ -63,6 +63,9 @@ void csr_free(Csr *c) {
 int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *qtags,
                const int64_t *near_ptr, const int64_t *near_idx, const double *mu,
                const double *params, Csr **outS, Csr **outD, double *diag_out);
+int lsq_build(Ctx *ctx, int cplx, int nrows, int ncols, int64_t ne, const double *A, const int64_t *cols,
+              int64_t m, int64_t n, const int64_t *near_ptr, const int64_t *near_idx, const double *rhs,
+              Csr **out, double *diag_out);
 
 }  // namespace inti
 
@@ -473,6 +476,21 @@ int inti_bdim_build(inti_op *opS, inti_op *opD, int derivative, int nq, int64_t 
     if (st) return st;
     *dS = static_cast<inti_csr *>(s);
     *dD = static_cast<inti_csr *>(d);
+    return INTI_OK;
+}
+
+int inti_lsq_build(inti_ctx *ctx, int is_complex, int nrows, int ncols, int64_t ne, const double *A,
+                   const int64_t *cols, int64_t m, int64_t n, const int64_t *near_ptr,
+                   const int64_t *near_idx, const double *rhs, inti_csr **out, double *diag) {
+    if (!ctx) return INTI_ERR_ARG;
+    REQUIRE(ctx, A && cols && rhs && out, "lsq: null argument");
+    REQUIRE(ctx, ne > 0 && m > 0 && n > 0, "lsq: empty problem");
+    REQUIRE(ctx, (near_ptr == nullptr) == (near_idx == nullptr), "lsq: near_ptr and near_idx go together");
+    LOCK(ctx);
+    Csr *c = nullptr;
+    int st = lsq_build(ctx, is_complex ? 1 : 0, nrows, ncols, ne, A, cols, m, n, near_ptr, near_idx, rhs, &c, diag);
+    if (st) return st;
+    *out = static_cast<inti_csr *>(c);
     return INTI_OK;
 }
 

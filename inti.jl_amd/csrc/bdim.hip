@@ -13,17 +13,14 @@
 //      bdim.jl:193-194): dD = W[1:nq], dS = -W[nq+1:2nq] (bdim.jl:175-182).
 #include <algorithm>
 #include <cmath>
-#include <numeric>
 
-#include "common.hpp"
+#include "lsq.hpp"
 #include "pair.hpp"
 
 namespace inti {
 
 constexpr int kThetaTT = 32;   // sources per LDS tile of the Theta kernel
 constexpr int kThetaBS = 256;  // targets per block
-constexpr int kMaxNs = 32;     // monopoles supported by the kernels
-constexpr int kMaxNc = 24;     // 2*nq supported by the LS kernel
 
 // ----------------------------------------------------------------- traces
 struct TraceArgs {
@@ -197,22 +194,6 @@ struct LsArgs {
     double *diag;             // per element: Rdiag ratio, max |W|, max |Theta|
 };
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 cconjmul(double2 a, double2 b) {  // conj(a)*b
-    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
-}
-__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
-    // scaled division (Smith)
-    if (fabs(b.x) >= fabs(b.y)) {
-        double t = b.y / b.x, den = b.x + b.y * t;
-        return make_double2((a.x + a.y * t) / den, (a.y - a.x * t) / den);
-    }
-    double t = b.x / b.y, den = b.y + b.x * t;
-    return make_double2((a.x * t + a.y) / den, (a.y * t - a.x) / den);
-}
-
 __global__ __launch_bounds__(64) void k_ls(LsArgs a) {
     __shared__ double2 A[kMaxNs * kMaxNc];  // ns x nc column-major
     __shared__ double2 tau[kMaxNc];
@@ -230,58 +211,8 @@ __global__ __launch_bounds__(64) void k_ls(LsArgs a) {
         A[kk + ns * r] = B[jg[rr] + a.ldb * kk];
     }
     __syncthreads();
-    for (int kk = 0; kk < nc; ++kk) {
-        const int len = ns - kk;
-        double2 *x = A + kk + ns * kk;
-        if (lane == 0) {
-            double s = 0.0;
-            for (int q = 0; q < len; ++q) s += x[q].x * x[q].x + x[q].y * x[q].y;
-            const double normu = sqrt(s);
-            double2 t = make_double2(0.0, 0.0);
-            if (normu != 0.0) {
-                double2 xi1 = x[0];
-                const double nu = copysign(normu, xi1.x);
-                xi1.x += nu;
-                x[0] = make_double2(-nu, 0.0);
-                // x[1:] /= xi1 is done below by all lanes
-                t = make_double2(xi1.x / nu, xi1.y / nu);
-                s_xi1 = xi1;
-            }
-            tau[kk] = t;
-        }
-        __syncthreads();
-        const double2 t = tau[kk];
-        if (t.x != 0.0 || t.y != 0.0) {
-            const double2 xi1 = s_xi1;
-            for (int q = 1 + lane; q < len; q += 64) x[q] = cdiv(x[q], xi1);
-        }
-        __syncthreads();
-        // apply the reflector to columns kk+1 .. nc-1, one lane per column
-        const int j = kk + 1 + lane;
-        if (j < nc) {
-            double2 *Aj = A + kk + ns * j;
-            double2 v = Aj[0];
-            for (int q = 1; q < len; ++q) {
-                const double2 c = cconjmul(x[q], Aj[q]);
-                v.x += c.x; v.y += c.y;
-            }
-            v = cconjmul(t, v);
-            Aj[0].x -= v.x; Aj[0].y -= v.y;
-            for (int q = 1; q < len; ++q) {
-                const double2 c = cmul(x[q], v);
-                Aj[q].x -= c.x; Aj[q].y -= c.y;
-            }
-        }
-        __syncthreads();
-    }
-    // conditioning indicator: ratio of extreme |R_ii|
-    double rmax = 0.0, rmin = INFINITY;
-    for (int q = 0; q < nc; ++q) {
-        const double2 v = A[q + ns * q];
-        const double av = hypot(v.x, v.y);
-        rmax = fmax(rmax, av);
-        rmin = fmin(rmin, av);
-    }
+    wave_householder_qr(A, ns, nc, tau, &s_xi1);
+    const double ratio = rdiag_ratio(A, ns, nc);
     // least-squares solves, one lane per near target
     const int p0 = a.pair_ptr[e], p1 = a.pair_ptr[e + 1];
     double wmax = 0.0, bmax = 0.0;
@@ -296,34 +227,14 @@ __global__ __launch_bounds__(64) void k_ls(LsArgs a) {
                 bn += b[q].x * b[q].x + b[q].y * b[q].y;
             }
             bmax = fmax(bmax, sqrt(bn));
-            for (int kk = 0; kk < nc; ++kk) {
-                const double2 *x = A + kk + ns * kk;
-                double2 v = b[kk];
-                for (int q = kk + 1; q < ns; ++q) {
-                    const double2 c = cconjmul(x[q - kk], b[q]);
-                    v.x += c.x; v.y += c.y;
-                }
-                v = cconjmul(tau[kk], v);
-                b[kk].x -= v.x; b[kk].y -= v.y;
-                for (int q = kk + 1; q < ns; ++q) {
-                    const double2 c = cmul(x[q - kk], v);
-                    b[q].x -= c.x; b[q].y -= c.y;
-                }
-            }
-            for (int jj = nc - 1; jj >= 0; --jj) {
-                const double2 xj = cdiv(b[jj], A[jj + ns * jj]);
-                b[jj] = xj;
-                for (int q = jj - 1; q >= 0; --q) {
-                    const double2 c = cmul(A[q + ns * jj], xj);
-                    b[q].x -= c.x; b[q].y -= c.y;
-                }
-            }
+            lane_qr_solve(A, ns, nc, tau, b);
             double wn = 0.0;
             for (int q = 0; q < nc; ++q) wn += b[q].x * b[q].x + b[q].y * b[q].y;
             wmax = fmax(wmax, sqrt(wn));
+            // dD = W[1:nq], dS = -W[nq+1:2nq] (the sign is applied by the gather)
             for (int r = 0; r < nq; ++r) {
                 a.triD[(int64_t)p * nq + r] = b[r];
-                a.triS[(int64_t)p * nq + r] = make_double2(-b[nq + r].x, -b[nq + r].y);
+                a.triS[(int64_t)p * nq + r] = b[nq + r];
             }
         }
     }
@@ -333,37 +244,9 @@ __global__ __launch_bounds__(64) void k_ls(LsArgs a) {
         bmax = fmax(bmax, __shfl_xor(bmax, off));
     }
     if (lane == 0) {
-        a.diag[3 * (int64_t)blockIdx.x + 0] = rmax / rmin;
+        a.diag[3 * (int64_t)blockIdx.x + 0] = ratio;
         a.diag[3 * (int64_t)blockIdx.x + 1] = wmax;
         a.diag[3 * (int64_t)blockIdx.x + 2] = bmax;
-    }
-}
-
-// CSR values from the triplets: val[pos] = sum of the triplets mapped to pos.
-struct GatherArgs {
-    int64_t nnz;
-    const int32_t *seg;   // nnz+1 -> range in perm
-    const int32_t *perm;  // triplet indices ordered by CSR position
-    const double2 *triS, *triD;
-    double *valS, *valD;
-    int cplx;
-};
-
-__global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
-    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (q >= a.nnz) return;
-    double sr = 0, si = 0, dr = 0, di = 0;
-    for (int32_t t = a.seg[q]; t < a.seg[q + 1]; ++t) {
-        const int32_t u = a.perm[t];
-        sr += a.triS[u].x; si += a.triS[u].y;
-        dr += a.triD[u].x; di += a.triD[u].y;
-    }
-    if (a.cplx) {
-        a.valS[2 * q] = sr; a.valS[2 * q + 1] = si;
-        a.valD[2 * q] = dr; a.valD[2 * q + 1] = di;
-    } else {
-        a.valS[q] = sr;
-        a.valD[q] = dr;
     }
 }
 
@@ -519,10 +402,9 @@ int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *q
     std::vector<int32_t> elems;
     for (int64_t e = 0; e < ne; ++e)
         if (pair_ptr[e + 1] > pair_ptr[e]) elems.push_back((int32_t)e);
-    // triplets (pair*nq + r) -> (row, col); order by (row, col)
+    // triplets (pair*nq + r) -> (row, col), CSR pattern
     const int64_t ntri = npairs * nq;
-    std::vector<int64_t> tri_order(ntri);
-    std::iota(tri_order.begin(), tri_order.end(), 0);
+    if (ntri >= (int64_t)INT32_MAX) return set_error(ctx, INTI_ERR_SIZE, "bdim: too many triplets");
     std::vector<int32_t> tri_row(ntri), tri_col(ntri);
     for (int64_t e = 0; e < ne; ++e)
         for (int32_t p = pair_ptr[e]; p < pair_ptr[e + 1]; ++p)
@@ -530,24 +412,10 @@ int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *q
                 tri_row[(int64_t)p * nq + r] = pair_tgt[p];
                 tri_col[(int64_t)p * nq + r] = (int32_t)qtags[r + (int64_t)nq * e];
             }
-    std::stable_sort(tri_order.begin(), tri_order.end(), [&](int64_t u, int64_t v) {
-        if (tri_row[u] != tri_row[v]) return tri_row[u] < tri_row[v];
-        return tri_col[u] < tri_col[v];
-    });
-    std::vector<int32_t> seg, perm(ntri), rowptr(m + 1, 0), colind;
-    for (int64_t q = 0; q < ntri; ++q) {
-        const int64_t u = tri_order[q];
-        perm[q] = (int32_t)u;
-        if (q == 0 || tri_row[u] != tri_row[tri_order[q - 1]] || tri_col[u] != tri_col[tri_order[q - 1]]) {
-            seg.push_back((int32_t)q);
-            colind.push_back(tri_col[u]);
-            rowptr[tri_row[u] + 1]++;
-        }
-    }
-    seg.push_back((int32_t)ntri);
-    for (int64_t i = 0; i < m; ++i) rowptr[i + 1] += rowptr[i];
+    TripletPattern pat;
+    build_pattern(m, tri_row, tri_col, pat);
+    const std::vector<int32_t> &rowptr = pat.rowptr, &colind = pat.colind, &seg = pat.seg, &perm = pat.perm;
     const int64_t nnz = (int64_t)colind.size();
-    if (ntri >= (int64_t)INT32_MAX) return set_error(ctx, INTI_ERR_SIZE, "bdim: too many triplets");
 
     // ---- device buffers
     std::vector<double> zs;
@@ -659,13 +527,15 @@ int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *q
         up(cD->rowptr, rowptr.data(), sizeof(int32_t) * (m + 1)) != hipSuccess ||
         up(cD->colind, colind.data(), sizeof(int32_t) * nnz) != hipSuccess)
         return fail(set_error(ctx, INTI_ERR_HIP, "bdim: CSR upload failed"));
-    if (nnz > 0) {
-        GatherArgs g;
+    {
+        Gather1Args g;
         g.nnz = nnz;
         g.seg = (const int32_t *)dseg.p; g.perm = (const int32_t *)dperm.p;
-        g.triS = (const double2 *)dtriS.p; g.triD = (const double2 *)dtriD.p;
-        g.valS = cS->val; g.valD = cD->val; g.cplx = cplx;
-        hipLaunchKernelGGL(k_gather, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, g);
+        g.cplx = cplx;
+        g.tri = (const double2 *)dtriS.p; g.val = cS->val; g.sign = -1.0;
+        launch_gather1(g, st);
+        g.tri = (const double2 *)dtriD.p; g.val = cD->val; g.sign = 1.0;
+        launch_gather1(g, st);
         if (hipGetLastError() != hipSuccess) return fail(set_error(ctx, INTI_ERR_HIP, "bdim: gather launch failed"));
     }
     std::vector<double> dg(3 * elems.size());

@@ -11,7 +11,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from . import _lib
-from .kernels import CombinedFieldKernel, Helmholtz
+from .kernels import CombinedFieldKernel, DoubleLayerKernel, Helmholtz, SingleLayerKernel
 
 
 def _green_multiplier(s):
@@ -57,6 +57,45 @@ class SparseCorrection:
     def to_scipy(self):
         rowptr, colind, val = self.export()
         return sp.csr_matrix((val, colind, rowptr), shape=self.shape)
+
+
+class IntegralPotential:
+    """IntegralPotential(kernel, quadrature) (nystrom.jl:20-38): pot[sigma] is
+    a function of evaluation points x, (dim,) or (dim, m), returning
+    sum_j K(x, y_j) sigma_j w_j; the sum runs as a GPU direct sum over all
+    points of one call. No singular correction: x must lie off the surface."""
+
+    def __init__(self, kernel, quadrature, precision="fp64", ctx=None):
+        self.kernel = kernel
+        self.quadrature = quadrature
+        self.precision = precision
+        self.ctx = ctx
+
+    def __getitem__(self, sigma):
+        sigma = np.asarray(sigma)
+        if sigma.shape[0] != len(self.quadrature):
+            raise ValueError("the density needs one value per quadrature node")
+
+        def evaluate(x):
+            from .quadrature import points
+            x = np.asarray(x, dtype=np.float64)
+            single = x.ndim == 1
+            X = x.reshape(self.quadrature.dim, -1)
+            op = IntegralOperator(self.kernel, points(X), self.quadrature, precision=self.precision,
+                                  ctx=self.ctx)
+            out = op.apply(sigma)
+            return out[0] if single and out.ndim == 1 else out
+        return evaluate
+
+
+def SingleLayerPotential(op, source):
+    """nystrom.jl:40-47."""
+    return IntegralPotential(SingleLayerKernel(op), source)
+
+
+def DoubleLayerPotential(op, source):
+    """nystrom.jl:49-56."""
+    return IntegralPotential(DoubleLayerKernel(op), source)
 
 
 class IntegralOperator:

@@ -66,8 +66,38 @@ VR4N6_NODES = np.array([
 ])
 VR4N6_WEIGHTS = np.array([0.0549758718276609] * 3 + [0.1116907948390057] * 3)
 
-# TRIANGLE_VR_ORDER_TO_NPTS (src/quad_rules_tables.jl:66-78), tabulated here
-TRIANGLE_VR_RULES = {4: (VR4N6_NODES, VR4N6_WEIGHTS)}
+# TRIANGLE_VR2N3 and TRIANGLE_VR5N10 (src/quad_rules_tables_triangle.jl:250-255, 277-288)
+VR2N3_NODES = np.array([
+    [0.16666666666666663, 0.6666666666666667],
+    [0.16666666666666663, 0.16666666666666674],
+    [0.6666666666666666, 0.16666666666666674],
+])
+VR2N3_WEIGHTS = np.array([0.1666666666666667] * 3)
+VR5N10_NODES = np.array([
+    [0.31397939767698096, 0.07109443567989363],
+    [0.6149261666431254, 0.3139793976769811],
+    [0.07109443567989354, 0.6149261666431255],
+    [0.6149261666431254, 0.07109443567989363],
+    [0.31397939767698096, 0.6149261666431255],
+    [0.07109443567989351, 0.3139793976769811],
+    [0.07111299453344089, 0.8577740109331182],
+    [0.07111299453344097, 0.07111299453344105],
+    [0.857774010933118, 0.07111299453344105],
+    [0.33333333333333326, 0.3333333333333334],
+])
+VR5N10_WEIGHTS = np.array([0.051683140693245144] * 6 + [0.030139490203570284] * 3 + [0.09948268522981828])
+
+# TRIANGLE_VR_ORDER_TO_NPTS (src/quad_rules_tables.jl:66-78), the rules tabulated here
+TRIANGLE_VR_RULES = {2: (VR2N3_NODES, VR2N3_WEIGHTS), 4: (VR4N6_NODES, VR4N6_WEIGHTS),
+                     5: (VR5N10_NODES, VR5N10_WEIGHTS)}
+# TRIANGLE_VR_IORDER_TO_QORDER (src/quad_rules_tables.jl:83-95), restricted to them
+TRIANGLE_VR_IORDER_TO_QORDER = {1: 2, 2: 4, 3: 5}
+
+
+def triangle_vr_interpolation_order_to_quadrature_order(i):
+    """Triangle_VR_interpolation_order_to_quadrature_order
+    (src/reference_integration.jl:320-322)."""
+    return TRIANGLE_VR_IORDER_TO_QORDER[i]
 
 
 def vioreanu_rokhlin_triangle(order):
@@ -117,7 +147,8 @@ class Quadrature:
     """SoA quadrature: coords (dim, N), normals (dim, N), weights (N,),
     qtags (nq, ne) element-major (node i of element e is qtags[i, e])."""
 
-    def __init__(self, coords, normals, weights, qtags, elements=None, name=""):
+    def __init__(self, coords, normals, weights, qtags, elements=None, name="", qorder=None):
+        self.qorder = qorder
         self.coords = np.ascontiguousarray(coords, dtype=np.float64)
         self.normals = None if normals is None else np.ascontiguousarray(normals, dtype=np.float64)
         self.weights = np.ascontiguousarray(weights, dtype=np.float64)
@@ -216,7 +247,8 @@ def ellipse_quadrature(a, b, nel, qorder=8, center=(0.0, 0.0), rotation=0.0):
     vals = np.stack([c * px - s * py + center[0], s * px + c * py + center[1]], axis=2)
     coords, normals, w, qtags = _line_quadrature(vals, qorder)
     return Quadrature(coords, normals, w, qtags, elements=vals,
-                      name=f"ellipse(a={a},b={b},nel={nel},q={qorder})")
+                      name=f"ellipse(a={a},b={b},nel={nel},q={qorder})",
+                      qorder=2 * len(gauss_legendre_order(qorder)[0]) - 1)
 
 
 def circle_quadrature(nel, qorder=8, radius=1.0, center=(0.0, 0.0)):
@@ -294,4 +326,167 @@ def sphere_quadrature(freq, qorder=4, base="ico", radius=1.0, radial=None, cente
         flip = np.repeat(np.where(s < 0, -1.0, 1.0), nq)
         normals = normals * flip[None, :]
     return Quadrature(coords, normals, w, qtags, elements=vals,
-                      name=f"sphere({base},freq={freq},q={qorder})")
+                      name=f"sphere({base},freq={freq},q={qorder})", qorder=qorder)
+
+
+def _triangle_quadrature_2d(vals, qorder):
+    """Planar P2 triangles: weights |det J| w_ref, no normals."""
+    uv, w_ref = vioreanu_rokhlin_triangle(qorder)
+    x, xu, xv = p2_triangle_eval(vals, uv)
+    ne, nq, _ = x.shape
+    det = xu[..., 0] * xv[..., 1] - xu[..., 1] * xv[..., 0]
+    w = np.abs(det) * w_ref[None, :]
+    coords = x.reshape(ne * nq, 2).T
+    qtags = np.arange(ne * nq, dtype=np.int64).reshape(ne, nq).T
+    return coords, w.reshape(-1), qtags
+
+
+def _zip_rings(ia, ta, ib, tb):
+    """Triangulate the band between two closed rings of point indices with
+    angles ta (inner) and tb (outer), both increasing, counter-clockwise."""
+    tris = []
+    na, nb = len(ia), len(ib)
+    if na == 1:  # a fan around the centre
+        return [(ia[0], ib[j], ib[(j + 1) % nb]) for j in range(nb)]
+    a = b = 0
+    while a < na or b < nb:
+        ta_next = ta[(a + 1) % na] + (2 * np.pi if a + 1 >= na else 0.0)
+        tb_next = tb[(b + 1) % nb] + (2 * np.pi if b + 1 >= nb else 0.0)
+        if b < nb and (a >= na or tb_next <= ta_next):
+            tris.append((ia[a % na], ib[b % nb], ib[(b + 1) % nb]))
+            b += 1
+        else:
+            tris.append((ia[a % na], ib[b % nb], ia[(a + 1) % na]))
+            a += 1
+    return tris
+
+
+def _p2_disk(V, T, tpar, a, b, qorder, boundary_qorder, name):
+    """Volume and boundary quadratures of a triangulated elliptical disk.
+    V: vertices (nv, 2); T: triangles (ne, 3); tpar: the ellipse parameter t
+    of each boundary vertex (NaN for interior ones). Edges between two
+    consecutive boundary vertices become curved P2 edges (midpoint at the mean
+    parameter, on the ellipse); all other edges are straight."""
+    def xy(t):
+        return np.array([a * np.cos(t), b * np.sin(t)])
+
+    def tmid(tu, tv):
+        if abs(tv - tu) > np.pi:
+            tv += 2 * np.pi if tv < tu else -2 * np.pi
+        return 0.5 * (tu + tv)
+
+    vals = np.zeros((len(T), 6, 2))
+    for e, (i0, i1, i2) in enumerate(T):
+        p0, p1, p2 = V[i0], V[i1], V[i2]
+        if (p1[0] - p0[0]) * (p2[1] - p0[1]) - (p1[1] - p0[1]) * (p2[0] - p0[0]) < 0:
+            i1, i2 = i2, i1  # counter-clockwise in the (x, y) plane
+
+        def mid(u, v):
+            if not (np.isnan(tpar[u]) or np.isnan(tpar[v])):
+                return xy(tmid(tpar[u], tpar[v]))
+            return 0.5 * (V[u] + V[v])
+
+        vals[e] = [V[i0], mid(i0, i1), V[i1], mid(i0, i2), mid(i1, i2), V[i2]]
+    coords, w, qtags = _triangle_quadrature_2d(vals, qorder)
+    vol = Quadrature(coords, None, w, qtags, elements=vals, name=name, qorder=qorder)
+    # boundary: the same curved edges as P2 line elements, counter-clockwise
+    bidx = np.nonzero(~np.isnan(tpar))[0]
+    tb = np.sort(tpar[bidx])
+    bvals = np.zeros((len(tb), 3, 2))
+    for j in range(len(tb)):
+        t0, t1 = tb[j], tb[j + 1] if j + 1 < len(tb) else tb[0] + 2 * np.pi
+        bvals[j] = [xy(t0), xy(0.5 * (t0 + t1)), xy(t1)]
+    vol.boundary_elements = bvals
+    bq = qorder if boundary_qorder is None else boundary_qorder
+    return vol, curve_quadrature(bvals, bq, name + "-boundary")
+
+
+def curve_quadrature(elements, qorder, name="curve"):
+    """Gauss-Legendre quadrature (order qorder) on P2 line elements
+    (ne, 3, 2), nodes in element order; normals (t2, -t1)/|t|."""
+    bc, bn, bw, bqt = _line_quadrature(elements, qorder)
+    return Quadrature(bc, bn, bw, bqt, elements=elements, name=name,
+                      qorder=2 * len(gauss_legendre_order(qorder)[0]) - 1)
+
+
+def boundary_quadrature(volume, qorder):
+    """Quadrature of order qorder on the outer boundary of a meshed 2D volume
+    (the boundary the reference's volume_potential builds from the parent mesh
+    when none is given, api.jl:314-322)."""
+    el = getattr(volume, "boundary_elements", None)
+    if el is None:
+        raise ValueError("the volume quadrature carries no boundary mesh")
+    return curve_quadrature(el, qorder, volume.name + "-boundary")
+
+
+def disk_quadratures(nrings, a=1.0, b=1.0, qorder=5, boundary_qorder=None):
+    """(volume, boundary) quadratures of the elliptical disk x^2/a^2 + y^2/b^2 <= 1:
+    P2 triangles on concentric rings (ring i has 6 i vertices), interior edges
+    straight, the outer edges curved (midpoints on the ellipse); the boundary
+    quadrature uses the same outer edges as P2 line elements, so both discretise
+    the same curve. qorder: Vioreanu-Rokhlin order of the volume rule;
+    boundary_qorder: Gauss-Legendre order of the boundary rule (default qorder)."""
+    pts = [(0.0, 0.0)]
+    rings, angles = [[0]], [np.array([0.0])]
+    for i in range(1, nrings + 1):
+        t = 2 * np.pi * np.arange(6 * i) / (6 * i)
+        idx = list(range(len(pts), len(pts) + 6 * i))
+        pts.extend((i / nrings * np.cos(tt), i / nrings * np.sin(tt)) for tt in t)
+        rings.append(idx)
+        angles.append(t)
+    P = np.array(pts)
+    rho = np.hypot(P[:, 0], P[:, 1])
+    theta = np.arctan2(P[:, 1], P[:, 0])
+    tris = []
+    for i in range(1, nrings + 1):
+        tris += _zip_rings(rings[i - 1], angles[i - 1], rings[i], angles[i])
+    V = np.stack([a * rho * np.cos(theta), b * rho * np.sin(theta)], axis=1)
+    tpar = np.full(len(P), np.nan)
+    tpar[rings[-1]] = np.mod(angles[-1], 2 * np.pi)
+    return _p2_disk(V, np.array(tris), tpar, a, b, qorder, boundary_qorder,
+                    f"disk(a={a},b={b},rings={nrings},q={qorder})")
+
+
+def ellipse_disk_quadratures(meshsize, a=1.0, b=1.0, qorder=5, boundary_qorder=None):
+    """(volume, boundary) quadratures of the elliptical disk with a quasi-uniform
+    mesh of size about `meshsize` (what gmsh's addDisk + MeshSizeMax/Min gives
+    the reference's examples): boundary vertices equally spaced in arc length,
+    interior vertices on a hexagonal lattice kept at least 0.6 meshsize from
+    the boundary, Delaunay triangulation, P2 with curved boundary edges."""
+    from scipy.spatial import Delaunay, cKDTree
+    h = float(meshsize)
+    ts = np.linspace(0.0, 2 * np.pi, 20001)
+    arc = np.concatenate([[0.0], np.cumsum(np.hypot(np.diff(a * np.cos(ts)), np.diff(b * np.sin(ts))))])
+    nb = max(6, int(round(arc[-1] / h)))
+    tb = np.interp(np.arange(nb) * arc[-1] / nb, arc, ts)
+    B = np.stack([a * np.cos(tb), b * np.sin(tb)], axis=1)
+    dense = cKDTree(np.stack([a * np.cos(ts), b * np.sin(ts)], axis=1))
+    ny = int(np.ceil(b / (h * np.sqrt(3) / 2))) + 1
+    nx = int(np.ceil(a / h)) + 1
+    I = []
+    for j in range(-ny, ny + 1):
+        y = j * h * np.sqrt(3) / 2
+        for i in range(-nx, nx + 1):
+            x = (i + 0.5 * (j % 2)) * h
+            if (x / a) ** 2 + (y / b) ** 2 < 1.0:
+                I.append((x, y))
+    I = np.array(I)
+    I = I[dense.query(I)[0] > 0.6 * h]
+    V = np.concatenate([B, I], axis=0)
+    T = Delaunay(V).simplices
+    # drop zero-area triangles (collinear boundary triples cannot occur on a
+    # strictly convex curve, but guard anyway)
+    p0, p1, p2 = V[T[:, 0]], V[T[:, 1]], V[T[:, 2]]
+    area = 0.5 * np.abs((p1[:, 0] - p0[:, 0]) * (p2[:, 1] - p0[:, 1]) - (p1[:, 1] - p0[:, 1]) * (p2[:, 0] - p0[:, 0]))
+    T = T[area > 1e-12 * h * h]
+    tpar = np.full(len(V), np.nan)
+    tpar[:nb] = tb
+    # every boundary edge must join consecutive boundary vertices
+    for tri in T:
+        on = [v for v in tri if v < nb]
+        if len(on) == 3:
+            raise RuntimeError("triangle with three boundary vertices")
+        if len(on) == 2 and (on[0] - on[1]) % nb not in (1, nb - 1):
+            raise RuntimeError("boundary chord in the triangulation")
+    return _p2_disk(V, T, tpar, a, b, qorder, boundary_qorder,
+                    f"ellipse-disk(a={a},b={b},h={h},q={qorder})")

@@ -35,7 +35,31 @@ def test_vr4n6_exactness(inti):
                 worst5 = max(worst5, abs(got - exact))
     assert worst5 > 1e-6
     with pytest.raises(ValueError):
-        inti.vioreanu_rokhlin_triangle(5)
+        inti.vioreanu_rokhlin_triangle(3)  # no 3rd-order rule in the table
+
+
+@pytest.mark.parametrize("order", [2, 5])
+def test_other_vr_rules_exactness(inti, order):
+    uv, w = inti.vioreanu_rokhlin_triangle(order)
+    assert abs(w.sum() - 0.5) < 1e-15
+    for a in range(order + 1):
+        for b in range(order + 1 - a):
+            exact = math.factorial(a) * math.factorial(b) / math.factorial(a + b + 2)
+            assert abs(np.dot(uv[:, 0] ** a * uv[:, 1] ** b, w) - exact) < 1e-15
+
+
+def test_disk_quadrature(inti):
+    from inti_jl_amd import quadrature as q
+    errs = []
+    for R in (4, 8):
+        V, B = q.disk_quadratures(R, 2.0, 1.0, qorder=5)
+        assert V.ne == 6 * R * R and V.nq == 10 and B.ne == 6 * R
+        assert np.all(V.weights > 0)
+        errs.append(abs(V.weights.sum() - 2 * np.pi))
+        # volume and boundary discretise the same curve: divergence theorem
+        flux = np.dot(np.sum(B.coords * B.normals, axis=0), B.weights)
+        assert abs(flux - 2 * V.weights.sum()) < 1e-9
+    assert errs[1] < errs[0] / 12
 
 
 def test_circle_geometry(inti):
@@ -73,3 +97,24 @@ def test_sphere_counts_and_orientation(inti):
     assert np.all(np.sum(bumpy.coords * bumpy.normals, axis=0) > 0)
     val = np.dot(np.sum(bumpy.coords * bumpy.normals, axis=0), bumpy.weights)
     assert val > 0
+
+
+def test_ellipse_disk_mesh(inti):
+    """Quasi-uniform Delaunay P2 mesh of the elliptical disk: area and
+    perimeter of the 2 x 1 ellipse (curved P2 edges: h^4-accurate), every
+    triangle counter-clockwise with a positive Jacobian at the nodes, the
+    boundary rebuilt from the volume mesh equals the one returned."""
+    from scipy import special
+    a, b = 2.0, 1.0
+    perim = 4 * a * special.ellipe(1 - (b / a) ** 2)
+    V, B = inti.ellipse_disk_quadratures(0.1, a=a, b=b, qorder=5, boundary_qorder=10)
+    assert abs(V.weights.sum() - np.pi * a * b) < 1e-5
+    assert abs(B.weights.sum() - perim) < 1e-6
+    from inti_jl_amd.quadrature import p2_triangle_eval, vioreanu_rokhlin_triangle
+    uv, _ = vioreanu_rokhlin_triangle(5)
+    _, xu, xv = p2_triangle_eval(V.elements, uv)
+    assert (xu[..., 0] * xv[..., 1] - xu[..., 1] * xv[..., 0]).min() > 0
+    B2 = inti.boundary_quadrature(V, 10)
+    np.testing.assert_array_equal(B2.coords, B.coords)
+    # divergence theorem on the boundary: int x . n = 2 |Omega|
+    assert abs(np.sum(B.weights * np.sum(B.coords * B.normals, axis=0)) - 2 * np.pi * a * b) < 1e-5
