@@ -53,6 +53,15 @@ typedef struct inti_csr inti_csr;
 #define INTI_HS 3
 #define INTI_COMB_SD 4
 #define INTI_COMB_AH 5
+/* Helmholtz far-field patterns of the single- and double-layer potentials
+ * and of c0*SL + c1*DL: targets are unit directions xh, K(xh, y) =
+ * P e^{-i k xh.y} B with B = 1 (SL) or -i k xh.n_y (DL), P = 1/(4 pi) in
+ * 3D and e^{i pi/4}/sqrt(8 pi k) in 2D (u_s ~ e^{ikr} r^{-(dim-1)/2} u_inf).
+ * No counterpart in the reference; the objective of the shape-gradient
+ * workload (squared far-field magnitude over a set of directions). */
+#define INTI_FF_SL 6
+#define INTI_FF_DL 7
+#define INTI_FF_COMB 8
 
 /* arithmetic of the direct sum */
 #define INTI_FP64 0  /* FP64 throughout                                    */
@@ -123,6 +132,26 @@ int inti_op_apply_device(inti_op *op, int nrhs, const double *x, int64_t ldx,
 /* Dense column-major m x n matrix of the operator, written to host memory
  * (assemble_matrix, src/nystrom.jl:103-124). */
 int inti_op_assemble_dense(inti_op *op, double *out);
+/* Pullback of y = A sigma (reverse mode; the rrule of `*` on an
+ * IntegralOperator, for objectives J(y) with cotangent ubar = dJ/dRe y +
+ * i dJ/dIm y). Kernel values are recomputed, never stored. Outputs (host,
+ * NULL to skip):
+ *   sigma_bar (n, complex for Helmholtz) = A^H ubar (corrections and the
+ *     identity term included);
+ *   src_xyz_bar (dim x n SoA), src_normal_bar (dim x n), src_weight_bar (n):
+ *     dJ/d(source nodes, normals, weights);
+ *   trg_xyz_bar (dim x m): dJ/d(target nodes) (not for far-field kinds);
+ *   k_bar: dJ/dk. dcoefs = {Re, Im dc0/dk, Re, Im dc1/dk} for combined kinds
+ *     whose coefficients depend on k (e.g. -i k), NULL for constants.
+ * For an operator on one surface the gradient of a node is the sum of its
+ * source and target parts. Kinds: SL, DL, COMB_SD and the far fields, FP64.
+ * Geometry and k gradients are of the direct sum: they are refused while a
+ * correction is attached (its weights depend on the geometry).
+ * No counterpart in the reference, which differentiates with ForwardDiff. */
+int inti_op_adjoint(inti_op *op, const double *ubar, const double *sigma,
+                    double *sigma_bar, double *src_xyz_bar,
+                    double *src_normal_bar, double *src_weight_bar,
+                    double *trg_xyz_bar, const double *dcoefs, double *k_bar);
 /* Fuse coef * csr into every following apply of op (the sparse correction
  * of src/api.jl:171-185). Several attachments with the same pattern are
  * merged into one CSR. */

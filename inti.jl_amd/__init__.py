@@ -12,11 +12,12 @@ from .api import (CorrectedOperator, adj_double_layer_hypersingular, combined_fi
                   single_double_layer, single_double_layer_potential)
 from .bdim import DimParameters, bdim_correction, etype_to_nearest_points
 from .kernels import (AdjointDoubleLayerKernel, CombinedFieldKernel, DoubleLayerKernel,
-                      Helmholtz, HyperSingularKernel, Laplace, SingleLayerKernel,
-                      default_density_eltype, default_kernel_eltype)
+                      FarFieldDoubleLayerKernel, FarFieldSingleLayerKernel, Helmholtz,
+                      HyperSingularKernel, Laplace, SingleLayerKernel, default_density_eltype,
+                      default_kernel_eltype)
 from .nystrom import (DoubleLayerPotential, IntegralOperator, IntegralPotential,
                       SingleLayerPotential, SparseCorrection, _green_multiplier, assemble_matrix,
-                      gmres, refine_solve)
+                      gmres, refine_solve, rrule)
 from .quadrature import (Quadrature, boundary_quadrature, circle_quadrature, curve_quadrature,
                          disk_quadratures, ellipse_disk_quadratures, ellipse_quadrature,
                          gauss_legendre, gauss_legendre_order, points, sphere_quadrature,
@@ -37,7 +38,8 @@ __all__ = [
     "disk_quadratures", "ellipse_disk_quadratures", "boundary_quadrature", "curve_quadrature",
     "polynomials", "vdim_auxiliary_quantities", "vdim_correction",
     "volume_potential", "IntegralPotential", "SingleLayerPotential", "DoubleLayerPotential",
-    "single_double_layer_potential",
+    "single_double_layer_potential", "FarFieldSingleLayerKernel", "FarFieldDoubleLayerKernel",
+    "rrule",
     "default_context", "default_density_eltype", "default_kernel_eltype",
     "ellipse_quadrature", "etype_to_nearest_points", "gauss_legendre", "gauss_legendre_order", "gmres",
     "refine_solve",

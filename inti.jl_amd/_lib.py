@@ -22,6 +22,7 @@ INTI_ERR_SIZE = 5
 
 LAPLACE, HELMHOLTZ = 0, 1
 SL, DL, ADL, HS, COMB_SD, COMB_AH = 0, 1, 2, 3, 4, 5
+FF_SL, FF_DL, FF_COMB = 6, 7, 8
 FP64, MIXED = 0, 1
 
 _vp = C.c_void_p
@@ -53,6 +54,7 @@ SIGNATURES = {
     "inti_op_apply": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
     "inti_op_apply_device": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64, _dp, _dp]),
     "inti_op_assemble_dense": (C.c_int, [_vp, _dp]),
+    "inti_op_adjoint": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "inti_op_attach_correction": (C.c_int, [_vp, _vp, C.c_double, C.c_double]),
     "inti_op_detach_correction": (C.c_int, [_vp]),
     "inti_bdim_build": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int64, _i64p, _i64p, _i64p,

@@ -1,0 +1,598 @@
+// Reverse mode of the direct sum y_i = sum_j K(x_i, y_j) w_j sigma_j: given
+// the cotangent ubar of y (ChainRules convention, the gradient of a real
+// objective J with respect to Re y + i Im y), the pullback returns
+//   sigma_bar_j = w_j conj(sum_i c_i K_ij)              (= A^H ubar)
+//   ybar_j      = Re(s_j sum_i c_i dK_ij/dy_j)          (source nodes)
+//   nbar_j      = Re(s_j sum_i c_i dK_ij/dn_j)          (source normals)
+//   wbar_j      = Re(sigma_j sum_i c_i K_ij)            (source weights)
+//   xbar_i      = Re(c_i sum_j s_j dK_ij/dx_i)          (target nodes)
+//   kbar        = Re(sum_ij c_i dK_ij/dk s_j)           (wavenumber)
+// with c_i = conj(ubar_i), s_j = w_j sigma_j. Kernel values are recomputed,
+// never stored: the source-side sums run source-major (one thread per
+// source, targets staged in LDS), the target-side sums target-major, each
+// with the other index range cut into chunks whose partial sums a second
+// kernel adds in a fixed order (deterministic, no atomics).
+//
+// Kernels differentiated (src/kernels.jl:137-204, 347-413): single layer,
+// double layer, c0 SL + c1 DL, and the Helmholtz far-field patterns. With
+// r = x - y, d = |r|, the translation-invariant ones are
+//   SL: K = g(d),            DL: K = h(d) (r . n_y),   h = -g'(d)/d,
+// so dK/dy = -g'(d) r/d (SL), -(h'(d) (r.n_y) r/d + h(d) n_y) (DL),
+// dK/dn_y = h(d) r, dK/dx = -dK/dy, with (E = e^{ikd}/(4 pi)):
+//   Laplace 2D   g = -log(d)/(2 pi)     h = 1/(2 pi d^2)
+//   Laplace 3D   g = 1/(4 pi d)         h = 1/(4 pi d^3)
+//   Helmholtz 3D g = E/d, g_k = i E     h = E (1 - ikd)/d^3, h_k = E k/d,
+//                h'/d = E (k^2 d^2 - 3 + 3ikd)/d^5
+//   Helmholtz 2D g = (i/4) H0(kd), g_k = -(i/4) d H1(kd),
+//                h = (ik/4) H1(kd)/d, h_k = (ik/4) H0(kd),
+//                h'/d = (ik/4) (k H0(kd)/d^2 - 2 H1(kd)/d^3).
+#include <complex>
+#include <vector>
+
+#include "common.hpp"
+#include "pair.hpp"
+
+namespace inti {
+
+namespace {
+
+__device__ __forceinline__ double2 c2(double r, double i) { return make_double2(r, i); }
+__device__ __forceinline__ double2 cm(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cs(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double2 ca(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ void cfma(double2 &acc, double2 a, double2 b) {  // acc += a*b
+    acc.x = __builtin_fma(a.x, b.x, __builtin_fma(-a.y, b.y, acc.x));
+    acc.y = __builtin_fma(a.x, b.y, __builtin_fma(a.y, b.x, acc.y));
+}
+
+struct AdjParams {
+    double k, kappa;
+    double2 c0, c1, dc0, dc1;  // combined-field coefficients and d/dk
+    double2 P, Pk;             // far-field prefactor and dP/dk
+};
+
+// g, g'/d, g_k (SL) and h, h'/d, h_k (DL) at distance d (d2 = d^2 > 0)
+template <int PDE, int DIM, bool WANT_G, bool WANT_H>
+__device__ __forceinline__ void radial(double d2, const AdjParams &p, double2 &g, double2 &gpd,
+                                       double2 &gk, double2 &h, double2 &hpd, double2 &hk) {
+    const double id = rsq64(d2);
+    const double d = d2 * id;
+    const double id2 = id * id;
+    if constexpr (PDE == INTI_LAPLACE) {
+        gk = hk = c2(0, 0);
+        if constexpr (DIM == 2) {
+            g = c2(-0.5 * kInv2Pi * log(d2), 0);
+            gpd = c2(-kInv2Pi * id2, 0);
+            h = c2(kInv2Pi * id2, 0);
+            hpd = c2(-2.0 * kInv2Pi * id2 * id2, 0);
+        } else {
+            const double id3 = id2 * id;
+            g = c2(kInv4Pi * id, 0);
+            gpd = c2(-kInv4Pi * id3, 0);
+            h = c2(kInv4Pi * id3, 0);
+            hpd = c2(-3.0 * kInv4Pi * id3 * id2, 0);
+        }
+    } else if constexpr (DIM == 3) {
+        const double k = p.k;
+        double s, c;
+        sincos_rev<false>(p.kappa * d, s, c);
+        const double2 E = c2(kInv4Pi * c, kInv4Pi * s);
+        const double id3 = id2 * id;
+        if constexpr (WANT_G) {
+            g = cs(E, id);
+            gpd = cs(cm(E, c2(-1.0, k * d)), id3);
+            gk = c2(-E.y, E.x);
+        }
+        if constexpr (WANT_H) {
+            h = cs(cm(E, c2(1.0, -k * d)), id3);
+            hpd = cs(cm(E, c2(k * k * d2 - 3.0, 3.0 * k * d)), id3 * id2);
+            hk = cs(E, k * id);
+        }
+    } else {
+        const double k = p.k;
+        double h0r, h0i, h1r, h1i;
+        hankel01<false>(k * d, h0r, h0i, h1r, h1i);
+        // (i/4) z = (-z.y/4, z.x/4)
+        if constexpr (WANT_G) {
+            g = c2(-0.25 * h0i, 0.25 * h0r);
+            gpd = c2(0.25 * k * h1i * id, -0.25 * k * h1r * id);  // -(ik/4) H1 / d
+            gk = c2(0.25 * d * h1i, -0.25 * d * h1r);            // -(i/4) d H1
+        }
+        if constexpr (WANT_H) {
+            const double q = 0.25 * k;
+            h = c2(-q * h1i * id, q * h1r * id);  // (ik/4) H1 / d
+            const double ar = k * h0r * id2 - 2.0 * h1r * id2 * id;
+            const double ai = k * h0i * id2 - 2.0 * h1i * id2 * id;
+            hpd = c2(-q * ai, q * ar);
+            hk = c2(-q * h0i, q * h0r);
+        }
+    }
+}
+
+// Value and derivatives of one pair; x the target (or far-field direction).
+template <int PDE, int DIM, int KIND>
+__device__ __forceinline__ void pair_grad(const double *x, const double *y, const double *ny,
+                                          const AdjParams &p, double2 &K, double2 (&gy)[DIM],
+                                          double2 (&gn)[DIM], double2 &gk) {
+    if constexpr (KIND >= INTI_FF_SL) {
+        const double k = p.k;
+        const double xy = dotv<DIM>(x, y);
+        double s, c;
+        sincos_rev<false>(-p.kappa * xy, s, c);
+        const double2 E = c2(c, s);  // e^{-ik xh.y}
+        const double2 PE = cm(p.P, E);
+        const double2 mixy = c2(0.0, -xy);  // -i xh.y
+        double2 Ks = PE, Kd = c2(0, 0), gkS = ca(cm(p.Pk, E), cm(PE, mixy)), gkD = c2(0, 0);
+        double xn = 0.0;
+        if constexpr (KIND != INTI_FF_SL) {
+            xn = dotv<DIM>(x, ny);
+            const double2 B = c2(0.0, -k * xn);  // -ik xh.n
+            Kd = cm(PE, B);
+            gkD = ca(ca(cm(cm(p.Pk, E), B), cm(PE, c2(0.0, -xn))), cm(Kd, mixy));
+        }
+        double2 kn;  // coefficient of xh in dK/dn_y
+        if constexpr (KIND == INTI_FF_SL) {
+            K = Ks; gk = gkS; kn = c2(0, 0);
+        } else if constexpr (KIND == INTI_FF_DL) {
+            K = Kd; gk = gkD; kn = cm(PE, c2(0.0, -k));
+        } else {
+            K = ca(cm(p.c0, Ks), cm(p.c1, Kd));
+            gk = ca(ca(cm(p.dc0, Ks), cm(p.c0, gkS)), ca(cm(p.dc1, Kd), cm(p.c1, gkD)));
+            kn = cm(p.c1, cm(PE, c2(0.0, -k)));
+        }
+        const double2 mik_K = cm(c2(0.0, -k), K);  // dK/dy = -ik xh K
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            gy[d] = cs(mik_K, x[d]);
+            gn[d] = cs(kn, x[d]);
+        }
+    } else {
+        double r[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) r[d] = x[d] - y[d];
+        const double d2 = dotv<DIM>(r, r);
+        const bool ok = d2 > kSamePointTol2;
+        const double d2s = fmax(d2, kClampD2);
+        constexpr bool WG = KIND != INTI_DL, WH = KIND != INTI_SL;
+        double2 g = c2(0, 0), gpd = c2(0, 0), gkg = c2(0, 0), h = c2(0, 0), hpd = c2(0, 0), hkk = c2(0, 0);
+        radial<PDE, DIM, WG, WH>(d2s, p, g, gpd, gkg, h, hpd, hkk);
+        const double rn = WH ? dotv<DIM>(r, ny) : 0.0;
+        // SL part: K = g, dK/dy = -gpd r; DL part: K = h rn,
+        // dK/dy = -(hpd rn r + h n), dK/dn = h r
+        double2 a_r, a_n, Kv;  // dK/dy = a_r r + a_n n;  dK/dn = b_r r
+        double2 b_r = c2(0, 0);
+        if constexpr (KIND == INTI_SL) {
+            Kv = g; gk = gkg; a_r = cs(gpd, -1.0); a_n = c2(0, 0);
+        } else if constexpr (KIND == INTI_DL) {
+            Kv = cs(h, rn); gk = cs(hkk, rn); a_r = cs(hpd, -rn); a_n = cs(h, -1.0); b_r = h;
+        } else {  // c0 SL + c1 DL
+            const double2 KS = g, KD = cs(h, rn);
+            Kv = ca(cm(p.c0, KS), cm(p.c1, KD));
+            gk = ca(ca(cm(p.dc0, KS), cm(p.c0, gkg)), ca(cm(p.dc1, KD), cm(p.c1, cs(hkk, rn))));
+            a_r = ca(cm(p.c0, cs(gpd, -1.0)), cm(p.c1, cs(hpd, -rn)));
+            a_n = cm(p.c1, cs(h, -1.0));
+            b_r = cm(p.c1, h);
+        }
+        const double mk = ok ? 1.0 : 0.0;
+        K = cs(Kv, mk);
+        gk = cs(gk, mk);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            gy[d] = cs(ca(cs(a_r, r[d]), cs(a_n, ny ? ny[d] : 0.0)), mk);
+            gn[d] = cs(b_r, r[d] * mk);
+        }
+    }
+}
+
+constexpr int kAdjBS = 256;
+
+struct AdjArgs {
+    int64_t m, n;              // targets, sources
+    const double *tx, *sx, *sn, *sw;
+    const double2 *c;          // conj(ubar), m
+    const double2 *sig;        // sigma, n (complex; Laplace has zero imaginary parts)
+    double2 *part;             // partial sums
+    int split;
+    int64_t chunk;
+    AdjParams p;
+};
+
+// Source-major pass: NV = 2 + 2 DIM complex sums per source:
+// [sum c K, sum c dK/dy (DIM), sum c dK/dn (DIM), sum c dK/dk]
+template <int PDE, int DIM, int KIND>
+__global__ __launch_bounds__(kAdjBS) void k_adj_src(AdjArgs a) {
+    constexpr int NV = 2 + 2 * DIM;
+    __shared__ double s_x[kAdjBS * DIM];
+    __shared__ double2 s_c[kAdjBS];
+    const int tid = threadIdx.x;
+    const int64_t chunk = blockIdx.x % a.split, tile = blockIdx.x / a.split;
+    const int64_t j = tile * kAdjBS + tid;
+    const int64_t jc = j < a.n ? j : a.n - 1;
+    double y[DIM], ny[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        y[d] = a.sx[d * a.n + jc];
+        ny[d] = a.sn ? a.sn[d * a.n + jc] : 0.0;
+    }
+    double2 acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = c2(0, 0);
+    const int64_t i0 = chunk * a.chunk;
+    const int64_t i1 = i0 + a.chunk < a.m ? i0 + a.chunk : a.m;
+    for (int64_t t0 = i0; t0 < i1; t0 += kAdjBS) {
+        const int64_t i = t0 + tid;
+        const bool live = i < i1;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) s_x[tid * DIM + d] = live ? a.tx[d * a.m + i] : 0.0;  // unused: the loop stops at cnt
+        s_c[tid] = live ? a.c[i] : c2(0, 0);
+        __syncthreads();
+        const int cnt = (int)(i1 - t0 < kAdjBS ? i1 - t0 : kAdjBS);
+        for (int q = 0; q < cnt; ++q) {
+            double x[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) x[d] = s_x[q * DIM + d];
+            double2 K, gy[DIM], gn[DIM], gk;
+            pair_grad<PDE, DIM, KIND>(x, y, ny, a.p, K, gy, gn, gk);
+            const double2 cc = s_c[q];
+            cfma(acc[0], cc, K);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                cfma(acc[1 + d], cc, gy[d]);
+                cfma(acc[1 + DIM + d], cc, gn[d]);
+            }
+            cfma(acc[NV - 1], cc, gk);
+        }
+        __syncthreads();
+    }
+    if (j < a.n) {
+        double2 *out = a.part + (chunk * a.n + j) * NV;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) out[v] = acc[v];
+    }
+}
+
+// Target-major pass: DIM complex sums per target, sum_j s_j dK/dx_i.
+template <int PDE, int DIM, int KIND>
+__global__ __launch_bounds__(kAdjBS) void k_adj_trg(AdjArgs a) {
+    __shared__ double s_y[kAdjBS * DIM];
+    __shared__ double s_n[kAdjBS * DIM];
+    __shared__ double2 s_s[kAdjBS];
+    const int tid = threadIdx.x;
+    const int64_t chunk = blockIdx.x % a.split, tile = blockIdx.x / a.split;
+    const int64_t i = tile * kAdjBS + tid;
+    const int64_t ic = i < a.m ? i : a.m - 1;
+    double x[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) x[d] = a.tx[d * a.m + ic];
+    double2 acc[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) acc[d] = c2(0, 0);
+    const int64_t j0 = chunk * a.chunk;
+    const int64_t j1 = j0 + a.chunk < a.n ? j0 + a.chunk : a.n;
+    for (int64_t t0 = j0; t0 < j1; t0 += kAdjBS) {
+        const int64_t j = t0 + tid;
+        const bool live = j < j1;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            s_y[tid * DIM + d] = live ? a.sx[d * a.n + j] : 0.0;  // unused: the loop stops at cnt
+            s_n[tid * DIM + d] = live && a.sn ? a.sn[d * a.n + j] : 0.0;
+        }
+        s_s[tid] = live ? cs(a.sig[j], a.sw[j]) : c2(0, 0);
+        __syncthreads();
+        const int cnt = (int)(j1 - t0 < kAdjBS ? j1 - t0 : kAdjBS);
+        for (int q = 0; q < cnt; ++q) {
+            double y[DIM], ny[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                y[d] = s_y[q * DIM + d];
+                ny[d] = s_n[q * DIM + d];
+            }
+            double2 K, gy[DIM], gn[DIM], gk;
+            pair_grad<PDE, DIM, KIND>(x, y, ny, a.p, K, gy, gn, gk);
+            const double2 ss = s_s[q];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) cfma(acc[d], ss, cs(gy[d], -1.0));  // dK/dx = -dK/dy
+        }
+        __syncthreads();
+    }
+    if (i < a.m) {
+        double2 *out = a.part + (chunk * a.m + i) * DIM;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) out[d] = acc[d];
+    }
+}
+
+struct AdjOut {
+    int64_t m, n;
+    int split, dim, cplx;
+    const double2 *part, *c, *sig;
+    const double *sw;
+    double *sigma_bar, *ybar, *nbar, *wbar, *kpart, *xbar;
+};
+
+// sum the source-side partials over chunks and form the source gradients;
+// per-block partial sums of kbar
+template <int DIM>
+__global__ __launch_bounds__(256) void k_adj_src_reduce(AdjOut o) {
+    constexpr int NV = 2 + 2 * DIM;
+    __shared__ double s_k[256];
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double kb = 0.0;
+    if (j < o.n) {
+        double2 S[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) S[v] = c2(0, 0);
+        for (int c = 0; c < o.split; ++c) {
+            const double2 *p = o.part + ((int64_t)c * o.n + j) * NV;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) S[v] = ca(S[v], p[v]);
+        }
+        const double w = o.sw[j];
+        const double2 sg = o.sig[j];
+        const double2 s = cs(sg, w);
+        if (o.sigma_bar) {
+            if (o.cplx) {
+                o.sigma_bar[2 * j] = w * S[0].x;
+                o.sigma_bar[2 * j + 1] = -w * S[0].y;
+            } else {
+                o.sigma_bar[j] = w * S[0].x;
+            }
+        }
+        if (o.wbar) o.wbar[j] = sg.x * S[0].x - sg.y * S[0].y;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            if (o.ybar) o.ybar[d * o.n + j] = s.x * S[1 + d].x - s.y * S[1 + d].y;
+            if (o.nbar) o.nbar[d * o.n + j] = s.x * S[1 + DIM + d].x - s.y * S[1 + DIM + d].y;
+        }
+        kb = s.x * S[NV - 1].x - s.y * S[NV - 1].y;
+    }
+    s_k[threadIdx.x] = kb;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) s_k[threadIdx.x] += s_k[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && o.kpart) o.kpart[blockIdx.x] = s_k[0];
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_adj_trg_reduce(AdjOut o) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= o.m) return;
+    double2 S[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) S[d] = c2(0, 0);
+    for (int c = 0; c < o.split; ++c) {
+        const double2 *p = o.part + ((int64_t)c * o.m + i) * DIM;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) S[d] = ca(S[d], p[d]);
+    }
+    const double2 cc = o.c[i];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) o.xbar[d * o.m + i] = cc.x * S[d].x - cc.y * S[d].y;
+}
+
+// sigma_bar += corr^H ubar + conj(identity) ubar: one thread per row i
+// scatters conj(val_iq) ubar_i to column colind_q (atomics: rows share
+// columns only across elements, so the collisions are few)
+__global__ __launch_bounds__(256) void k_adj_corr(int64_t m, const int32_t *rowptr, const int32_t *colind,
+                                                  const double *val, int cplx, const double2 *c,
+                                                  double *sigma_bar) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const double2 cc = c[i];  // conj(ubar_i)
+    for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+        const int32_t j = colind[q];
+        if (cplx) {
+            // conj(v) ubar = conj(v c)
+            const double vr = val[2 * q], vi = val[2 * q + 1];
+            atomicAdd(&sigma_bar[2 * j], vr * cc.x - vi * cc.y);
+            atomicAdd(&sigma_bar[2 * j + 1], -(vr * cc.y + vi * cc.x));
+        } else {
+            atomicAdd(&sigma_bar[j], val[q] * cc.x);
+        }
+    }
+}
+
+using SrcFn = void (*)(const AdjArgs &, dim3, hipStream_t);
+
+template <int PDE, int DIM, int KIND>
+void src_launch(const AdjArgs &a, dim3 g, hipStream_t s) {
+    hipLaunchKernelGGL((k_adj_src<PDE, DIM, KIND>), g, dim3(kAdjBS), 0, s, a);
+}
+template <int PDE, int DIM, int KIND>
+void trg_launch(const AdjArgs &a, dim3 g, hipStream_t s) {
+    hipLaunchKernelGGL((k_adj_trg<PDE, DIM, KIND>), g, dim3(kAdjBS), 0, s, a);
+}
+
+template <int PDE, int DIM>
+void pick(int kind, SrcFn &fs, SrcFn &ft) {
+    fs = ft = nullptr;
+    switch (kind) {
+        case INTI_SL: fs = src_launch<PDE, DIM, INTI_SL>; ft = trg_launch<PDE, DIM, INTI_SL>; return;
+        case INTI_DL: fs = src_launch<PDE, DIM, INTI_DL>; ft = trg_launch<PDE, DIM, INTI_DL>; return;
+        case INTI_COMB_SD:
+            fs = src_launch<PDE, DIM, INTI_COMB_SD>; ft = trg_launch<PDE, DIM, INTI_COMB_SD>; return;
+    }
+    if constexpr (PDE == INTI_HELMHOLTZ) {
+        switch (kind) {
+            case INTI_FF_SL: fs = src_launch<PDE, DIM, INTI_FF_SL>; return;
+            case INTI_FF_DL: fs = src_launch<PDE, DIM, INTI_FF_DL>; return;
+            case INTI_FF_COMB: fs = src_launch<PDE, DIM, INTI_FF_COMB>; return;
+        }
+    }
+}
+
+struct Dev {
+    void *p = nullptr;
+    hipStream_t s = nullptr;
+    ~Dev() {
+        if (p) {
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(p);
+        }
+    }
+    bool alloc(hipStream_t st, size_t bytes) {
+        s = st;
+        return hipMalloc(&p, bytes ? bytes : 8) == hipSuccess;
+    }
+};
+
+// chunks of the inner index so that the grid has several blocks per CU
+void plan(int64_t own, int64_t inner, int num_cu, int &split, int64_t &chunk) {
+    const int64_t tiles = (own + kAdjBS - 1) / kAdjBS;
+    const int64_t itiles = (inner + kAdjBS - 1) / kAdjBS;
+    int64_t s = (8LL * num_cu + tiles - 1) / tiles;
+    if (s > itiles) s = itiles;
+    if (s < 1) s = 1;
+    chunk = (itiles + s - 1) / s * kAdjBS;
+    split = (int)((inner + chunk - 1) / chunk);
+    if (split < 1) split = 1;
+}
+
+}  // namespace
+
+int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_bar, double *ybar,
+               double *nbar, double *wbar, double *xbar, const double *dcoefs, double *kbar) {
+    Ctx *ctx = op->ctx;
+    const int kind = op->kind;
+    const bool ff = kind >= INTI_FF_SL;
+    if (!(kind == INTI_SL || kind == INTI_DL || kind == INTI_COMB_SD || ff))
+        return set_error(ctx, INTI_ERR_UNSUPPORTED, "adjoint: supported kinds are SL, DL, c0 SL + c1 DL and the far fields");
+    if (op->precision != INTI_FP64)
+        return set_error(ctx, INTI_ERR_UNSUPPORTED, "adjoint: FP64 operators only");
+    if (xbar && ff) return set_error(ctx, INTI_ERR_UNSUPPORTED, "adjoint: far-field directions carry no gradient");
+    const bool geo = ybar || nbar || wbar || xbar || kbar;
+    if (op->corr && geo)
+        return set_error(ctx, INTI_ERR_UNSUPPORTED,
+                         "adjoint: geometry/k gradients of a corrected operator are not available (the correction weights "
+                         "depend on the geometry); detach the correction or request sigma_bar only");
+    const int dim = op->dim;
+    const int cplx = op_is_complex(op) ? 1 : 0;
+    const int64_t m = op->trg->n, n = op->src->n;
+    if (m == 0 || n == 0) {
+        if (sigma_bar) std::fill(sigma_bar, sigma_bar + (cplx ? 2 : 1) * n, 0.0);
+        if (ybar) std::fill(ybar, ybar + dim * n, 0.0);
+        if (nbar) std::fill(nbar, nbar + dim * n, 0.0);
+        if (wbar) std::fill(wbar, wbar + n, 0.0);
+        if (xbar) std::fill(xbar, xbar + dim * m, 0.0);
+        if (kbar) *kbar = 0.0;
+        return INTI_OK;
+    }
+    SrcFn fs = nullptr, ft = nullptr;
+    if (op->pde == INTI_LAPLACE && dim == 2) pick<INTI_LAPLACE, 2>(kind, fs, ft);
+    if (op->pde == INTI_LAPLACE && dim == 3) pick<INTI_LAPLACE, 3>(kind, fs, ft);
+    if (op->pde == INTI_HELMHOLTZ && dim == 2) pick<INTI_HELMHOLTZ, 2>(kind, fs, ft);
+    if (op->pde == INTI_HELMHOLTZ && dim == 3) pick<INTI_HELMHOLTZ, 3>(kind, fs, ft);
+    if (!fs) return set_error(ctx, INTI_ERR_UNSUPPORTED, "adjoint: no kernel for this operator");
+
+    AdjParams p;
+    p.k = op->k;
+    p.kappa = op->k * kInv2Pi;
+    p.c0 = make_double2(op->coefs[0], op->coefs[1]);
+    p.c1 = make_double2(op->coefs[2], op->coefs[3]);
+    p.dc0 = dcoefs ? make_double2(dcoefs[0], dcoefs[1]) : make_double2(0, 0);
+    p.dc1 = dcoefs ? make_double2(dcoefs[2], dcoefs[3]) : make_double2(0, 0);
+    p.P = p.Pk = make_double2(0, 0);
+    if (ff) {
+        double pr, pi;
+        basic_post(op->pde, dim, INTI_FF_SL, op->k, pr, pi);
+        p.P = make_double2(pr, pi);
+        if (dim == 2) p.Pk = make_double2(-0.5 * pr / op->k, -0.5 * pi / op->k);
+    }
+
+    // host staging: c = conj(ubar), sigma as complex
+    std::vector<double2> hc(m), hs(n);
+    for (int64_t i = 0; i < m; ++i) hc[i] = cplx ? make_double2(ubar[2 * i], -ubar[2 * i + 1]) : make_double2(ubar[i], 0.0);
+    for (int64_t j = 0; j < n; ++j) hs[j] = cplx ? make_double2(sigma[2 * j], sigma[2 * j + 1]) : make_double2(sigma[j], 0.0);
+
+    hipStream_t s = ctx->stream;
+    int split_s, split_t = 0;
+    int64_t chunk_s, chunk_t = 0;
+    plan(n, m, ctx->num_cu, split_s, chunk_s);
+    if (xbar) plan(m, n, ctx->num_cu, split_t, chunk_t);
+    const int NVs = 2 + 2 * dim;
+    const int64_t nsb = (n + 255) / 256;
+    Dev dc, dsg, dpart, dsb, dyb, dnb, dwb, dkp, dxb;
+    const size_t part_bytes = std::max(sizeof(double2) * (size_t)split_s * n * NVs,
+                                       xbar ? sizeof(double2) * (size_t)split_t * m * dim : 0);
+    if (!dc.alloc(s, sizeof(double2) * m) || !dsg.alloc(s, sizeof(double2) * n) || !dpart.alloc(s, part_bytes) ||
+        !dsb.alloc(s, sizeof(double) * (cplx ? 2 : 1) * n) || !dyb.alloc(s, sizeof(double) * dim * n) ||
+        !dnb.alloc(s, sizeof(double) * dim * n) || !dwb.alloc(s, sizeof(double) * n) ||
+        !dkp.alloc(s, sizeof(double) * nsb) || !dxb.alloc(s, sizeof(double) * dim * m))
+        return set_error(ctx, INTI_ERR_NOMEM, "adjoint: allocation failed");
+    INTI_HIP(ctx, hipMemcpyAsync(dc.p, hc.data(), sizeof(double2) * m, hipMemcpyHostToDevice, s));
+    INTI_HIP(ctx, hipMemcpyAsync(dsg.p, hs.data(), sizeof(double2) * n, hipMemcpyHostToDevice, s));
+
+    AdjArgs a;
+    a.m = m; a.n = n;
+    a.tx = op->trg->coords; a.sx = op->src->coords; a.sn = op->src->normals; a.sw = op->src->weights;
+    a.c = (const double2 *)dc.p; a.sig = (const double2 *)dsg.p;
+    a.part = (double2 *)dpart.p;
+    a.p = p;
+    AdjOut o;
+    o.m = m; o.n = n; o.dim = dim; o.cplx = cplx;
+    o.part = (const double2 *)dpart.p; o.c = (const double2 *)dc.p; o.sig = (const double2 *)dsg.p;
+    o.sw = op->src->weights;
+    o.sigma_bar = (double *)dsb.p; o.ybar = (double *)dyb.p; o.nbar = (double *)dnb.p;
+    o.wbar = (double *)dwb.p; o.kpart = (double *)dkp.p; o.xbar = (double *)dxb.p;
+
+    // source-major pass
+    a.split = split_s; a.chunk = chunk_s;
+    if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev0, s));
+    fs(a, dim3((unsigned)(((n + kAdjBS - 1) / kAdjBS) * split_s)), s);
+    INTI_HIP(ctx, hipGetLastError());
+    o.split = split_s;
+    if (dim == 2) hipLaunchKernelGGL(k_adj_src_reduce<2>, dim3((unsigned)nsb), dim3(256), 0, s, o);
+    else hipLaunchKernelGGL(k_adj_src_reduce<3>, dim3((unsigned)nsb), dim3(256), 0, s, o);
+    INTI_HIP(ctx, hipGetLastError());
+    // target-major pass (reuses the partial buffer)
+    if (xbar) {
+        a.split = split_t; a.chunk = chunk_t;
+        ft(a, dim3((unsigned)(((m + kAdjBS - 1) / kAdjBS) * split_t)), s);
+        INTI_HIP(ctx, hipGetLastError());
+        o.split = split_t;
+        const unsigned mb = (unsigned)((m + 255) / 256);
+        if (dim == 2) hipLaunchKernelGGL(k_adj_trg_reduce<2>, dim3(mb), dim3(256), 0, s, o);
+        else hipLaunchKernelGGL(k_adj_trg_reduce<3>, dim3(mb), dim3(256), 0, s, o);
+        INTI_HIP(ctx, hipGetLastError());
+    }
+    if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev1, s));
+    // corrected operator: sigma_bar also gets corr^H ubar and conj(identity) ubar
+    if (sigma_bar && op->corr) {
+        const Csr *cr = op->corr;
+        hipLaunchKernelGGL(k_adj_corr, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m, cr->rowptr, cr->colind,
+                           cr->val, cplx, (const double2 *)dc.p, (double *)dsb.p);
+        INTI_HIP(ctx, hipGetLastError());
+    }
+    std::vector<double> kp(nsb);
+    if (sigma_bar) INTI_HIP(ctx, hipMemcpyAsync(sigma_bar, dsb.p, sizeof(double) * (cplx ? 2 : 1) * n, hipMemcpyDeviceToHost, s));
+    if (ybar) INTI_HIP(ctx, hipMemcpyAsync(ybar, dyb.p, sizeof(double) * dim * n, hipMemcpyDeviceToHost, s));
+    if (nbar) INTI_HIP(ctx, hipMemcpyAsync(nbar, dnb.p, sizeof(double) * dim * n, hipMemcpyDeviceToHost, s));
+    if (wbar) INTI_HIP(ctx, hipMemcpyAsync(wbar, dwb.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    if (xbar) INTI_HIP(ctx, hipMemcpyAsync(xbar, dxb.p, sizeof(double) * dim * m, hipMemcpyDeviceToHost, s));
+    if (kbar) INTI_HIP(ctx, hipMemcpyAsync(kp.data(), dkp.p, sizeof(double) * nsb, hipMemcpyDeviceToHost, s));
+    INTI_HIP(ctx, hipStreamSynchronize(s));
+    if (sigma_bar && (op->cid[0] != 0.0 || op->cid[1] != 0.0) && m == n) {
+        // y += cid sigma  =>  sigma_bar += conj(cid) ubar
+        for (int64_t j = 0; j < n; ++j) {
+            if (cplx) {
+                const double ur = ubar[2 * j], ui = ubar[2 * j + 1];
+                sigma_bar[2 * j] += op->cid[0] * ur + op->cid[1] * ui;
+                sigma_bar[2 * j + 1] += op->cid[0] * ui - op->cid[1] * ur;
+            } else {
+                sigma_bar[j] += op->cid[0] * ubar[j];
+            }
+        }
+    }
+    if (kbar) {
+        double sum = 0.0;
+        for (double v : kp) sum += v;
+        *kbar = sum;
+    }
+    return INTI_OK;
+}
+
+}  // namespace inti

@@ -257,20 +257,23 @@ int inti_op_create(inti_ctx *ctx, int pde, int kind, int dim, double k, const do
     if (!ctx) return INTI_ERR_ARG;
     REQUIRE(ctx, out && trg && src, "op_create: null argument");
     REQUIRE(ctx, pde == INTI_LAPLACE || pde == INTI_HELMHOLTZ, "op_create: unknown pde");
-    REQUIRE(ctx, kind >= INTI_SL && kind <= INTI_COMB_AH, "op_create: unknown kernel kind");
+    REQUIRE(ctx, kind >= INTI_SL && kind <= INTI_FF_COMB, "op_create: unknown kernel kind");
+    const bool farfield = kind >= INTI_FF_SL;
+    if (farfield && pde != INTI_HELMHOLTZ)
+        return set_error(ctx, INTI_ERR_UNSUPPORTED, "op_create: far-field kinds are Helmholtz only");
     REQUIRE(ctx, dim == 2 || dim == 3, "op_create: dim must be 2 or 3");
     REQUIRE(ctx, precision == INTI_FP64 || precision == INTI_MIXED, "op_create: unknown precision");
     // IntegralOperator(k, X, Y) requires equal dimensions (src/nystrom.jl:79-85)
     if (trg->dim != dim || src->dim != dim)
         return set_error(ctx, INTI_ERR_SIZE, "op_create: quadrature dimension differs from the operator's");
-    const bool needs_src_n = kind != INTI_SL && kind != INTI_ADL;
+    const bool needs_src_n = kind != INTI_SL && kind != INTI_ADL && kind != INTI_FF_SL;
     const bool needs_trg_n = kind == INTI_ADL || kind == INTI_HS || kind == INTI_COMB_AH;
     if (needs_src_n && !src->normals)
         return set_error(ctx, INTI_ERR_ARG, "op_create: this kernel needs source normals");
     if (needs_trg_n && !trg->normals)
         return set_error(ctx, INTI_ERR_ARG, "op_create: this kernel needs target normals");
     if (pde == INTI_HELMHOLTZ) REQUIRE(ctx, k > 0.0 && std::isfinite(k), "op_create: Helmholtz needs a real k > 0");
-    const bool comb = kind == INTI_COMB_SD || kind == INTI_COMB_AH;
+    const bool comb = kind == INTI_COMB_SD || kind == INTI_COMB_AH || kind == INTI_FF_COMB;
     if (comb) {
         REQUIRE(ctx, coefs != nullptr, "op_create: combined kinds need coefs");
         if (pde == INTI_LAPLACE && (coefs[1] != 0.0 || coefs[3] != 0.0))
@@ -379,6 +382,17 @@ int inti_op_assemble_dense(inti_op *op, double *out) {
     }
     (void)hipFree(d);
     return st;
+}
+
+int inti_op_adjoint(inti_op *op, const double *ubar, const double *sigma, double *sigma_bar,
+                    double *src_xyz_bar, double *src_normal_bar, double *src_weight_bar,
+                    double *trg_xyz_bar, const double *dcoefs, double *k_bar) {
+    if (!op) return INTI_ERR_ARG;
+    Ctx *ctx = op->ctx;
+    REQUIRE(ctx, ubar != nullptr && sigma != nullptr, "adjoint: ubar and sigma are required");
+    LOCK(ctx);
+    return op_adjoint(op, ubar, sigma, sigma_bar, src_xyz_bar, src_normal_bar, src_weight_bar, trg_xyz_bar,
+                      dcoefs, k_bar);
 }
 
 // merged correction: c_new = c_old + coef * csr (same pattern), or a copy

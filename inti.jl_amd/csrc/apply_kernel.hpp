@@ -118,11 +118,8 @@ __global__ __launch_bounds__(kApplyBS) void k_apply(ApplyArgs a) {
                 ys[0] = v0.x; ys[1] = v0.y;
                 ns[0] = v1.x; ns[1] = v1.y;
             }
-            double rv[DIM];
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) rv[d] = xt[d] - ys[d];
             double kr, ki;
-            pair_raw<PDE, DIM, KIND, MIXED>(rv, nt, ns, a.p, kr, ki);
+            pair_raw_xy<PDE, DIM, KIND, MIXED>(xt, nt, ys, ns, a.p, kr, ki);
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 const double2 q = s_q[jj * NR + r];
@@ -164,6 +161,13 @@ static ApplyFn apply_kind(int kind) {
         case INTI_HS: return apply_launcher<PDE, DIM, INTI_HS, MIXED, NR>;
         case INTI_COMB_SD: return apply_launcher<PDE, DIM, INTI_COMB_SD, MIXED, NR>;
         case INTI_COMB_AH: return apply_launcher<PDE, DIM, INTI_COMB_AH, MIXED, NR>;
+    }
+    if constexpr (PDE == INTI_HELMHOLTZ) {
+        switch (kind) {
+            case INTI_FF_SL: return apply_launcher<PDE, DIM, INTI_FF_SL, MIXED, NR>;
+            case INTI_FF_DL: return apply_launcher<PDE, DIM, INTI_FF_DL, MIXED, NR>;
+            case INTI_FF_COMB: return apply_launcher<PDE, DIM, INTI_FF_COMB, MIXED, NR>;
+        }
     }
     return nullptr;
 }

@@ -20,15 +20,16 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs a) {
     for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * 256) {
         const int64_t i = e % a.m, j = e / a.m;
-        double r[DIM], nx[DIM], ny[DIM];
+        double x[DIM], y[DIM], nx[DIM], ny[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-            r[d] = a.tx[d * a.m + i] - a.sx[d * a.n + j];
+            x[d] = a.tx[d * a.m + i];
+            y[d] = a.sx[d * a.n + j];
             nx[d] = a.tn ? a.tn[d * a.m + i] : 0.0;
             ny[d] = a.sn ? a.sn[d * a.n + j] : 0.0;
         }
         double kr, ki;
-        pair_raw<PDE, DIM, KIND, false>(r, nx, ny, a.p, kr, ki);
+        pair_raw_xy<PDE, DIM, KIND, false>(x, nx, y, ny, a.p, kr, ki);
         {
             const double tr = a.p.postr * kr - a.p.posti * ki;
             ki = a.p.postr * ki + a.p.posti * kr;
@@ -58,6 +59,13 @@ static AsmFn asm_kind(int kind) {
         case INTI_HS: return asm_launcher<PDE, DIM, INTI_HS>;
         case INTI_COMB_SD: return asm_launcher<PDE, DIM, INTI_COMB_SD>;
         case INTI_COMB_AH: return asm_launcher<PDE, DIM, INTI_COMB_AH>;
+    }
+    if constexpr (PDE == INTI_HELMHOLTZ) {
+        switch (kind) {
+            case INTI_FF_SL: return asm_launcher<PDE, DIM, INTI_FF_SL>;
+            case INTI_FF_DL: return asm_launcher<PDE, DIM, INTI_FF_DL>;
+            case INTI_FF_COMB: return asm_launcher<PDE, DIM, INTI_FF_COMB>;
+        }
     }
     return nullptr;
 }

@@ -103,6 +103,8 @@ int refine_solve(Op *op_hi, Op *op_lo, const double *b, double *x, int restart, 
                  double inner_rtol, int max_outer, double rtol, int use_graph, int *outer,
                  int *inner_total, double *relres);
 int launch_assemble(const Op *op, double *out_dev);
+int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_bar, double *ybar,
+               double *nbar, double *wbar, double *xbar, const double *dcoefs, double *kbar);
 
 bool op_is_complex(const Op *op);
 

@@ -106,9 +106,9 @@ KParams op_params(const Op *op, int *eff_kind) {
     const cd c0(op->coefs[0], op->coefs[1]), c1(op->coefs[2], op->coefs[3]);
     cd post, ch(0.0, 0.0);
     double pr, pi;
-    if (kind == INTI_COMB_SD || kind == INTI_COMB_AH) {
-        const int ka = kind == INTI_COMB_SD ? INTI_SL : INTI_ADL;
-        const int kb = kind == INTI_COMB_SD ? INTI_DL : INTI_HS;
+    if (kind == INTI_COMB_SD || kind == INTI_COMB_AH || kind == INTI_FF_COMB) {
+        const int ka = kind == INTI_COMB_SD ? INTI_SL : kind == INTI_COMB_AH ? INTI_ADL : INTI_FF_SL;
+        const int kb = kind == INTI_COMB_SD ? INTI_DL : kind == INTI_COMB_AH ? INTI_HS : INTI_FF_DL;
         double ar, ai, br, bi;
         basic_post(op->pde, op->dim, ka, op->k, ar, ai);
         basic_post(op->pde, op->dim, kb, op->k, br, bi);

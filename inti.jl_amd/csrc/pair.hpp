@@ -161,10 +161,31 @@ __device__ __forceinline__ void helm2_raw(const double *r, double d2s, bool ok, 
     }
 }
 
+// ------------------------------------------------------ Helmholtz far field
+// raw = e^{-i k xh.y} B, B = 1 (SL), -i k xh.n_y (DL), ch + (-i k xh.n_y)
+// (FF_COMB, post c1 P, ch = c0/c1); xh is the target "coordinate", y the
+// source node. Smooth: nothing to mask.
+template <int DIM, int KIND, bool MIXED>
+__device__ __forceinline__ void helm_ff_raw(const double *xh, const double *y, const double *ny,
+                                            const KParams &p, double &kr, double &ki) {
+    double s, c;
+    sincos_rev<MIXED>(-p.kappa * dotv<DIM>(xh, y), s, c);
+    if constexpr (KIND == INTI_FF_SL) {
+        kr = c; ki = s;
+    } else {
+        const double bi = -p.k * dotv<DIM>(xh, ny);
+        const double br = KIND == INTI_FF_COMB ? p.chr : 0.0;
+        const double bim = KIND == INTI_FF_COMB ? bi + p.chi : bi;
+        kr = c * br - s * bim;
+        ki = c * bim + s * br;
+    }
+}
+
 // raw value of the pair (post applied by the caller)
 template <int PDE, int DIM, int KIND, bool MIXED>
 __device__ __forceinline__ void pair_raw(const double *r, const double *nx, const double *ny,
                                          const KParams &p, double &kr, double &ki) {
+    static_assert(KIND < INTI_FF_SL, "far-field kinds go through pair_raw_xy");
     double d2 = dotv<DIM>(r, r);
     bool ok = d2 > kSamePointTol2;
     double d2s = fmax(d2, kClampD2);
@@ -175,6 +196,23 @@ __device__ __forceinline__ void pair_raw(const double *r, const double *nx, cons
         helm3_raw<KIND, MIXED>(r, d2s, ok, nx, ny, p, kr, ki);
     } else {
         helm2_raw<KIND, MIXED>(r, d2s, ok, nx, ny, p, kr, ki);
+    }
+}
+
+// Pair value from the two points: the far-field kinds need the direction
+// and the source node themselves, the others only r = x - y.
+template <int PDE, int DIM, int KIND, bool MIXED>
+__device__ __forceinline__ void pair_raw_xy(const double *x, const double *nx, const double *y,
+                                            const double *ny, const KParams &p, double &kr,
+                                            double &ki) {
+    if constexpr (KIND >= INTI_FF_SL) {
+        static_assert(PDE == INTI_HELMHOLTZ, "far fields are Helmholtz only");
+        helm_ff_raw<DIM, KIND, MIXED>(x, y, ny, p, kr, ki);
+    } else {
+        double r[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) r[d] = x[d] - y[d];
+        pair_raw<PDE, DIM, KIND, MIXED>(r, nx, ny, p, kr, ki);
     }
 }
 
@@ -195,6 +233,11 @@ KParams op_params(const Op *op, int *eff_kind);
 __host__ __device__ inline void basic_post(int pde, int dim, int kind, double k, double &pr,
                                            double &pi) {
     pr = 0.0; pi = 0.0;
+    if (kind == INTI_FF_SL || kind == INTI_FF_DL) {  // Helmholtz only
+        if (dim == 3) pr = kInv4Pi;
+        else pr = pi = 0.25 / sqrt(kPi * k);  // e^{i pi/4} / sqrt(8 pi k)
+        return;
+    }
     if (pde == INTI_LAPLACE) {
         if (dim == 2) pr = kind == INTI_SL ? -0.5 * kInv2Pi : kind == INTI_ADL ? -kInv2Pi : kInv2Pi;
         else pr = kind == INTI_ADL ? -kInv4Pi : kInv4Pi;
@@ -209,7 +252,8 @@ __host__ __device__ inline void basic_post(int pde, int dim, int kind, double k,
 template <int PDE, int DIM, int KIND>
 __device__ __forceinline__ void pair_value(const double *r, const double *nx, const double *ny,
                                            const KParams &p, double &kr, double &ki) {
-    static_assert(KIND <= INTI_HS, "combined kinds carry their post factor in KParams");
+    static_assert(KIND <= INTI_HS || KIND == INTI_FF_SL || KIND == INTI_FF_DL,
+                  "combined kinds carry their post factor in KParams");
     double rr, ri, pr, pi;
     pair_raw<PDE, DIM, KIND, false>(r, nx, ny, p, rr, ri);
     basic_post(PDE, DIM, KIND, p.k, pr, pi);

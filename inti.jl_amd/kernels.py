@@ -83,15 +83,37 @@ class HyperSingularKernel(_Kernel):
     kind = _lib.HS
 
 
+class _FarField(_Kernel):
+    def __init__(self, op):
+        if not isinstance(op, Helmholtz):
+            raise ValueError("far-field kernels are defined for Helmholtz only")
+        super().__init__(op)
+
+
+class FarFieldSingleLayerKernel(_FarField):
+    """Far-field pattern of the single-layer potential: targets are unit
+    directions xh, K = P e^{-ik xh.y}, P = 1/(4 pi) in 3D and
+    e^{i pi/4}/sqrt(8 pi k) in 2D (u ~ e^{ikr} r^{-(dim-1)/2} u_inf)."""
+    kind = _lib.FF_SL
+
+
+class FarFieldDoubleLayerKernel(_FarField):
+    """Far-field pattern of the double-layer potential: P e^{-ik xh.y} (-ik xh.n_y)."""
+    kind = _lib.FF_DL
+
+
 class CombinedFieldKernel(_Kernel):
-    """c0*SL + c1*DL (pair='SD') or c0*ADL + c1*HS (pair='AH'), evaluated in
-    one pass (shares the distance and e^{ikd} / Hankel evaluation)."""
+    """c0*SL + c1*DL (pair='SD'), c0*ADL + c1*HS (pair='AH') or the far field
+    of c0*SL + c1*DL (pair='FF'), evaluated in one pass (shares the distance
+    and e^{ikd} / Hankel evaluation)."""
 
     def __init__(self, op, c0, c1, pair="SD"):
         super().__init__(op)
-        if pair not in ("SD", "AH"):
-            raise ValueError("pair must be 'SD' or 'AH'")
-        self.kind = _lib.COMB_SD if pair == "SD" else _lib.COMB_AH
+        if pair not in ("SD", "AH", "FF"):
+            raise ValueError("pair must be 'SD', 'AH' or 'FF'")
+        if pair == "FF" and not isinstance(op, Helmholtz):
+            raise ValueError("far-field kernels are defined for Helmholtz only")
+        self.kind = {"SD": _lib.COMB_SD, "AH": _lib.COMB_AH, "FF": _lib.FF_COMB}[pair]
         self.pair = pair
         self.c0 = complex(c0)
         self.c1 = complex(c1)

@@ -187,6 +187,51 @@ class IntegralOperator:
     def __matmul__(self, x):
         return self.apply(x)
 
+    def pullback(self, ubar, sigma, *, wrt=("sigma",), dcoefs=None):
+        """Reverse mode of y = A sigma on the GPU (kernel values recomputed):
+        the gradients of Re<ubar, y> requested in `wrt` among "sigma"
+        (A^H ubar), "source_nodes", "source_normals", "source_weights",
+        "target_nodes" and "k". dcoefs = (dc0/dk, dc1/dk) for combined kernels
+        with k-dependent coefficients. Returns a dict."""
+        m, n = self.shape
+        dim = self.source.dim
+        u = np.ascontiguousarray(ubar, dtype=self.dtype).reshape(m)
+        s = np.ascontiguousarray(sigma, dtype=self.dtype).reshape(n)
+        out = {}
+        bufs = {
+            "sigma": np.zeros(n, dtype=self.dtype),
+            "source_nodes": np.zeros((dim, n)),
+            "source_normals": np.zeros((dim, n)),
+            "source_weights": np.zeros(n),
+            "target_nodes": np.zeros((dim, m)),
+            "k": np.zeros(1),
+        }
+        for w in wrt:
+            if w not in bufs:
+                raise ValueError(f"unknown gradient {w!r}")
+        ptr = {w: (_lib.dptr(bufs[w]) if w in wrt else None) for w in bufs}
+        dc = None
+        if dcoefs is not None:
+            d0, d1 = complex(dcoefs[0]), complex(dcoefs[1])
+            dc = np.array([d0.real, d0.imag, d1.real, d1.imag])
+        self.ctx.check(self.ctx.lib.inti_op_adjoint(
+            self.handle, _lib.dptr(u), _lib.dptr(s), ptr["sigma"], ptr["source_nodes"],
+            ptr["source_normals"], ptr["source_weights"], ptr["target_nodes"], _lib.dptr(dc), ptr["k"]))
+        for w in wrt:
+            out[w] = bufs[w][0] if w == "k" else bufs[w]
+        return out
+
+
+def rrule(A, sigma):
+    """ChainRulesCore-style rrule of y = A * sigma: returns (y, pullback),
+    pullback(ybar, wrt=...) -> dict of gradients (IntegralOperator.pullback)."""
+    iop = A.iop if hasattr(A, "iop") else A
+    y = A.apply(sigma)
+
+    def pullback(ybar, wrt=("sigma",), dcoefs=None):
+        return iop.pullback(ybar, sigma, wrt=wrt, dcoefs=dcoefs)
+    return y, pullback
+
 
 def assemble_matrix(iop):
     """Dense matrix of an IntegralOperator (nystrom.jl:103-124); the entries

@@ -57,6 +57,9 @@ def _bind(path):
                                       _i64p, C.c_double, _i64p, _i64p]
     lib.ork_csr_matvec.argtypes = [C.c_int64, C.c_int64, _i64p, _i64p, _dp, C.c_int, _dp, _dp,
                                    _dp, _dp]
+    lib.ork_adjoint.restype = C.c_int
+    lib.ork_adjoint.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, C.c_int64, _dp,
+                                C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     return lib
 
 
@@ -196,3 +199,27 @@ def corrected_matvec(op_pde, dim, kind, k, quad, sigma, mu, which="D", coefs=Non
     y = matvec(op_pde, dim, kind, k, quad, quad, sigma, coefs=coefs, nthreads=nthreads)
     val = vD if which == "D" else vS
     return y + csr_matvec(len(quad.weights), len(quad.weights), rowptr, colind, val, sigma)
+
+
+def adjoint(pde, dim, kind, k, trg, src, ubar, sigma, coefs=None, dcoefs=None, want_xbar=True):
+    """Pullback of y = K sigma: dict with sigma_bar (n,), ybar/nbar (dim, n),
+    wbar (n,), xbar (dim, m) and kbar (see ork_adjoint). coefs = (c0, c1),
+    dcoefs = (dc0/dk, dc1/dk), complex."""
+    m, n = trg.coords.shape[1], src.coords.shape[1]
+    u = np.ascontiguousarray(np.asarray(ubar, dtype=np.complex128))
+    sg = np.ascontiguousarray(np.asarray(sigma, dtype=np.complex128))
+    sb = np.zeros(n, dtype=np.complex128)
+    yb = np.zeros((dim, n))
+    nb = np.zeros((dim, n))
+    wb = np.zeros(n)
+    xb = np.zeros((dim, m)) if want_xbar else None
+    kb = np.zeros(1)
+    c = _coefs(coefs)
+    dc = None if dcoefs is None else _coefs(dcoefs)
+    rc = _load().ork_adjoint(pde, dim, kind, k, _d(c), _d(dc), m, _d(_f64(trg.coords)), n,
+                             _d(_f64(src.coords)), _d(_f64(src.normals)), _d(_f64(src.weights)),
+                             _d(u.view(np.float64)), _d(sg.view(np.float64)), _d(sb.view(np.float64)),
+                             _d(yb), _d(nb), _d(wb), _d(xb), _d(kb))
+    if rc != 0:
+        raise ValueError("oracle adjoint: unsupported kind")
+    return {"sigma_bar": sb, "ybar": yb, "nbar": nb, "wbar": wb, "xbar": xb, "kbar": kb[0]}

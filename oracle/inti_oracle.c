@@ -102,9 +102,28 @@ static cplx kernel_pt(int pde, int dim, int kind, double k, const double *x,
     }
 }
 
+/* Helmholtz far-field pattern kernels (no reference counterpart): xh a unit
+ * direction; P e^{-ik xh.y} times 1 (SL) or -ik xh.n_y (DL);
+ * P = 1/(4 pi) (3D), e^{i pi/4}/sqrt(8 pi k) (2D). */
+static cplx ff_prefactor(int dim, double k) {
+    return dim == 3 ? 1.0 / (4 * PI) : cexp(I * PI / 4) / sqrt(8 * PI * k);
+}
+
+static cplx kernel_ff(int dim, int kind, double k, const double *xh, const double *y,
+                      const double *ny) {
+    cplx e = ff_prefactor(dim, k) * cexp(-I * k * dotn(dim, xh, y));
+    if (kind == ORK_FF_SL) return e;
+    return e * (-I * k * dotn(dim, xh, ny));
+}
+
 static cplx kernel_any(int pde, int dim, int kind, double k, const double coefs[4],
                        const double *x, const double *nx, const double *y,
                        const double *ny) {
+    if (kind == ORK_FF_SL || kind == ORK_FF_DL) return kernel_ff(dim, kind, k, x, y, ny);
+    if (kind == ORK_FF_COMB) {
+        cplx c0 = coefs[0] + I * coefs[1], c1 = coefs[2] + I * coefs[3];
+        return c0 * kernel_ff(dim, ORK_FF_SL, k, x, y, ny) + c1 * kernel_ff(dim, ORK_FF_DL, k, x, y, ny);
+    }
     if (kind == ORK_COMB_SD || kind == ORK_COMB_AH) {
         int ka = kind == ORK_COMB_SD ? ORK_SL : ORK_ADL;
         int kb = kind == ORK_COMB_SD ? ORK_DL : ORK_HS;
@@ -508,4 +527,161 @@ void ork_csr_matvec(int64_t m, int64_t n, const int64_t *rowptr,
             yv[0] = creal(res); yv[1] = cimag(res);
         }
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reverse mode of the direct sum (no reference counterpart). Derivatives of
+ * one kernel written from its closed form: value K, dK/dy (gy), dK/dn_y
+ * (gn), dK/dk (gk). dcoefs: dc0/dk, dc1/dk of the combined kinds. */
+static void kernel_grad(int pde, int dim, int kind, double k, const double coefs[4],
+                        const double dcoefs[4], const double *x, const double *y,
+                        const double *ny, cplx *K, cplx gy[3], cplx gn[3], cplx *gk) {
+    for (int d = 0; d < 3; ++d) gy[d] = gn[d] = 0.0;
+    *K = *gk = 0.0;
+    if (kind == ORK_COMB_SD || kind == ORK_FF_COMB) {
+        int ka = kind == ORK_COMB_SD ? ORK_SL : ORK_FF_SL;
+        int kb = kind == ORK_COMB_SD ? ORK_DL : ORK_FF_DL;
+        cplx c0 = coefs[0] + I * coefs[1], c1 = coefs[2] + I * coefs[3];
+        cplx d0 = dcoefs ? dcoefs[0] + I * dcoefs[1] : 0.0;
+        cplx d1 = dcoefs ? dcoefs[2] + I * dcoefs[3] : 0.0;
+        cplx Ka, Kb, gya[3], gyb[3], gna[3], gnb[3], gka, gkb;
+        kernel_grad(pde, dim, ka, k, coefs, dcoefs, x, y, ny, &Ka, gya, gna, &gka);
+        kernel_grad(pde, dim, kb, k, coefs, dcoefs, x, y, ny, &Kb, gyb, gnb, &gkb);
+        *K = c0 * Ka + c1 * Kb;
+        *gk = d0 * Ka + c0 * gka + d1 * Kb + c1 * gkb;
+        for (int d = 0; d < dim; ++d) {
+            gy[d] = c0 * gya[d] + c1 * gyb[d];
+            gn[d] = c0 * gna[d] + c1 * gnb[d];
+        }
+        return;
+    }
+    if (kind == ORK_FF_SL || kind == ORK_FF_DL) {
+        cplx P = ff_prefactor(dim, k);
+        cplx Pk = dim == 3 ? 0.0 : -P / (2 * k);
+        double xy = dotn(dim, x, y);
+        cplx e = cexp(-I * k * xy);
+        if (kind == ORK_FF_SL) {
+            *K = P * e;
+            *gk = Pk * e + P * (-I * xy) * e;
+            for (int d = 0; d < dim; ++d) gy[d] = -I * k * x[d] * (*K);
+        } else {
+            double xn = dotn(dim, x, ny);
+            *K = P * (-I * k * xn) * e;
+            *gk = Pk * (-I * k * xn) * e + P * (-I * xn) * e + (*K) * (-I * xy);
+            for (int d = 0; d < dim; ++d) {
+                gy[d] = -I * k * x[d] * (*K);
+                gn[d] = P * (-I * k * x[d]) * e;
+            }
+        }
+        return;
+    }
+    double r[3] = {0, 0, 0};
+    for (int d = 0; d < dim; ++d) r[d] = x[d] - y[d];
+    double d2 = dotn(dim, r, r);
+    if (sqrt(d2) <= SAME_POINT_TOLERANCE) return;
+    double dd = sqrt(d2);
+    double rn = kind == ORK_DL ? dotn(dim, r, ny) : 0.0;
+    /* K = F(d) (SL) or F(d) (r.n) (DL); dK/dy = -F'(d) r/d [* r.n] - F n (DL) */
+    cplx F, Fp, Fk;
+    if (pde == ORK_LAPLACE) {
+        Fk = 0.0;
+        if (dim == 2) {
+            if (kind == ORK_SL) { F = -log(dd) / (2 * PI); Fp = -1.0 / (2 * PI * dd); }
+            else { F = 1.0 / (2 * PI * d2); Fp = -1.0 / (PI * d2 * dd); }
+        } else {
+            if (kind == ORK_SL) { F = 1.0 / (4 * PI * dd); Fp = -1.0 / (4 * PI * d2); }
+            else { F = 1.0 / (4 * PI * d2 * dd); Fp = -3.0 / (4 * PI * d2 * d2); }
+        }
+    } else if (dim == 3) {
+        cplx e = cexp(I * k * dd) / (4 * PI);
+        if (kind == ORK_SL) {
+            F = e / dd;
+            Fp = e * (I * k * dd - 1.0) / d2;
+            Fk = I * e;
+        } else {
+            F = e * (1.0 - I * k * dd) / (d2 * dd);
+            Fp = e * (k * k * d2 - 3.0 + 3.0 * I * k * dd) / (d2 * d2);
+            Fk = e * k / dd;
+        }
+    } else {
+        cplx H0 = hankel(0, k * dd), H1 = hankel(1, k * dd);
+        if (kind == ORK_SL) {
+            F = I / 4 * H0;
+            Fp = -I * k / 4 * H1;
+            Fk = -I / 4 * dd * H1;
+        } else {
+            F = I * k / 4 * H1 / dd;
+            Fp = I * k / 4 * (k * H0 / dd - 2.0 * H1 / d2);
+            Fk = I * k / 4 * H0;
+        }
+    }
+    if (kind == ORK_SL) {
+        *K = F;
+        *gk = Fk;
+        for (int d = 0; d < dim; ++d) gy[d] = -Fp * r[d] / dd;
+    } else {
+        *K = F * rn;
+        *gk = Fk * rn;
+        for (int d = 0; d < dim; ++d) {
+            gy[d] = -Fp * rn * r[d] / dd - F * ny[d];
+            gn[d] = F * r[d];
+        }
+    }
+}
+
+int ork_adjoint(int pde, int dim, int kind, double k, const double coefs[4],
+                const double dcoefs[4], int64_t m, const double *tx, int64_t n,
+                const double *sx, const double *sn, const double *sw, const double *ubar,
+                const double *sigma, double *sigma_bar, double *ybar, double *nbar,
+                double *wbar, double *xbar, double *kbar) {
+    if (!(kind == ORK_SL || kind == ORK_DL || kind == ORK_COMB_SD || kind == ORK_FF_SL ||
+          kind == ORK_FF_DL || kind == ORK_FF_COMB))
+        return -1;
+    double ksum = 0.0;
+#pragma omp parallel for schedule(dynamic, 8) reduction(+ : ksum)
+    for (int64_t j = 0; j < n; ++j) {
+        double y[3] = {0, 0, 0}, ny[3] = {0, 0, 0}, x[3] = {0, 0, 0};
+        gather(dim, sx, n, j, y);
+        gather(dim, sn, n, j, ny);
+        cplx sig = sigma[2 * j] + I * sigma[2 * j + 1];
+        cplx s = sw[j] * sig;
+        cplx SK = 0.0, SY[3] = {0, 0, 0}, SN[3] = {0, 0, 0}, SKk = 0.0;
+        for (int64_t i = 0; i < m; ++i) {
+            gather(dim, tx, m, i, x);
+            cplx c = ubar[2 * i] - I * ubar[2 * i + 1];
+            cplx K, gy[3], gn[3], gk;
+            kernel_grad(pde, dim, kind, k, coefs, dcoefs, x, y, ny, &K, gy, gn, &gk);
+            SK += c * K;
+            SKk += c * gk;
+            for (int d = 0; d < dim; ++d) { SY[d] += c * gy[d]; SN[d] += c * gn[d]; }
+        }
+        cplx sb = sw[j] * conj(SK);
+        if (sigma_bar) { sigma_bar[2 * j] = creal(sb); sigma_bar[2 * j + 1] = cimag(sb); }
+        if (wbar) wbar[j] = creal(sig * SK);
+        for (int d = 0; d < dim; ++d) {
+            if (ybar) ybar[d * n + j] = creal(s * SY[d]);
+            if (nbar) nbar[d * n + j] = creal(s * SN[d]);
+        }
+        ksum += creal(s * SKk);
+    }
+    if (kbar) *kbar = ksum;
+    if (xbar) {
+#pragma omp parallel for schedule(dynamic, 8)
+        for (int64_t i = 0; i < m; ++i) {
+            double x[3] = {0, 0, 0}, y[3] = {0, 0, 0}, ny[3] = {0, 0, 0};
+            gather(dim, tx, m, i, x);
+            cplx c = ubar[2 * i] - I * ubar[2 * i + 1];
+            cplx T[3] = {0, 0, 0};
+            for (int64_t j = 0; j < n; ++j) {
+                gather(dim, sx, n, j, y);
+                gather(dim, sn, n, j, ny);
+                cplx s = sw[j] * (sigma[2 * j] + I * sigma[2 * j + 1]);
+                cplx K, gy[3], gn[3], gk;
+                kernel_grad(pde, dim, kind, k, coefs, dcoefs, x, y, ny, &K, gy, gn, &gk);
+                for (int d = 0; d < dim; ++d) T[d] -= s * gy[d];
+            }
+            for (int d = 0; d < dim; ++d) xbar[d * m + i] = creal(c * T[d]);
+        }
+    }
+    return 0;
 }

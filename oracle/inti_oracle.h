@@ -33,7 +33,11 @@ enum { ORK_LAPLACE = 0, ORK_HELMHOLTZ = 1 };
 /* kinds: 0 SL, 1 DL, 2 ADL, 3 HS,
  *        4 c0*SL + c1*DL   (combined field, e.g. DL - i*eta*SL),
  *        5 c0*ADL + c1*HS                                     */
-enum { ORK_SL = 0, ORK_DL = 1, ORK_ADL = 2, ORK_HS = 3, ORK_COMB_SD = 4, ORK_COMB_AH = 5 };
+enum { ORK_SL = 0, ORK_DL = 1, ORK_ADL = 2, ORK_HS = 3, ORK_COMB_SD = 4, ORK_COMB_AH = 5,
+       ORK_FF_SL = 6, ORK_FF_DL = 7, ORK_FF_COMB = 8 };
+/* 6-8: Helmholtz far-field patterns of SL, DL, c0*SL + c1*DL (targets are
+ * unit directions; P e^{-ik xh.y} (1 or -ik xh.n_y), P = 1/(4 pi) in 3D,
+ * e^{i pi/4}/sqrt(8 pi k) in 2D). */
 
 /* One kernel value K(x, y); out[0..1] = (re, im). nx/ny may be NULL when
  * the kernel does not use them. */
@@ -102,6 +106,18 @@ void ork_csr_matvec(int64_t m, int64_t n, const int64_t *rowptr,
                     const int64_t *colind, const double *val, int nrhs,
                     const double *x, const double alpha[2], const double beta[2],
                     double *y);
+
+/* Reverse mode of y = K sigma for a cotangent ubar (all complex
+ * interleaved; Laplace with zero imaginary parts): sigma_bar = A^H ubar,
+ * ybar/nbar (dim x n SoA), wbar (n): gradients of Re<ubar, y> in the source
+ * nodes, normals and weights; xbar (dim x m): in the target nodes; kbar: in
+ * k. Kinds SL, DL, COMB_SD and FF_*; dcoefs (may be NULL) = dc0/dk, dc1/dk.
+ * Any output may be NULL. Returns 0, or -1 for an unsupported kind. */
+int ork_adjoint(int pde, int dim, int kind, double k, const double coefs[4],
+                const double dcoefs[4], int64_t m, const double *tx, int64_t n,
+                const double *sx, const double *sn, const double *sw, const double *ubar,
+                const double *sigma, double *sigma_bar, double *ybar, double *nbar,
+                double *wbar, double *xbar, double *kbar);
 
 #ifdef __cplusplus
 }
