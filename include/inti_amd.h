@@ -31,6 +31,7 @@ typedef struct inti_ctx inti_ctx;
 typedef struct inti_quad inti_quad;
 typedef struct inti_op inti_op;
 typedef struct inti_csr inti_csr;
+typedef struct inti_batch inti_batch;
 
 /* status codes */
 #define INTI_OK 0
@@ -158,6 +159,51 @@ int inti_op_adjoint(inti_op *op, const double *ubar, const double *sigma,
 int inti_op_attach_correction(inti_op *op, const inti_csr *csr, double coef_re,
                               double coef_im);
 int inti_op_detach_correction(inti_op *op);
+
+/* ------------------------------------------------------------ batches */
+/* Many independent problems (boundaries of uneven size, one wavenumber
+ * each) as one block-diagonal operator: one persistent launch applies all
+ * of them, one batched GMRES solves all (problem, right-hand side) systems.
+ * The reference has no batch: this replaces a loop over problems of
+ * IntegralOperator(K, Q_p, Q_p) (src/nystrom.jl:79-89) and `gmres(L_p, b_p)`
+ * (docs/src/examples/helmholtz_scattering.jl:494-495).
+ *   quads[p]: the problem's quadrature (target = source); k[p]: wavenumber
+ *   (Helmholtz); coefs: 4 per problem for INTI_COMB_SD (NULL otherwise).
+ *   Kinds: SL, DL, COMB_SD. Vectors are block vectors: ntot x nrhs,
+ *   column-major, problem p in rows [offsets[p], offsets[p+1]). */
+int inti_batch_create(inti_ctx *ctx, int pde, int kind, int dim, int nprob,
+                      const inti_quad *const *quads, const double *k,
+                      const double *coefs, int precision, inti_batch **out);
+int inti_batch_destroy(inti_batch *batch);
+/* ntot and offsets[nprob + 1] (either may be NULL) */
+int inti_batch_info(const inti_batch *batch, int64_t *ntot, int64_t *offsets);
+/* identity coefficients, 2 doubles (re, im) per problem */
+int inti_batch_set_identity(inti_batch *batch, const double *cid);
+/* add coef * csr (problem-local indices, n_p x n_p) to problem p */
+int inti_batch_attach_correction(inti_batch *batch, int p, const inti_csr *csr,
+                                 double coef_re, double coef_im);
+/* source chunk length of a work item (0 = 2048) and persistent grid size
+ * (0 = 4 workgroups per CU) */
+int inti_batch_set_launch(inti_batch *batch, int64_t chunk_len, int grid);
+/* y = alpha A x + beta y; device (leading dimensions >= ntot) or host */
+int inti_batch_apply_device(inti_batch *batch, int nrhs, const double *x,
+                            int64_t ldx, double *y, int64_t ldy,
+                            const double *alpha, const double *beta);
+int inti_batch_apply(inti_batch *batch, int nrhs, const double *x, double *y,
+                     const double *alpha, const double *beta);
+/* Time the apply of nrhs columns over chunk lengths {1024, 2048, 4096, 8192}
+ * and grids {2, 4, 8} workgroups per CU (reps applies each) and keep the
+ * fastest; reports the choice and its time per apply (ms). */
+int inti_batch_autotune(inti_batch *batch, int nrhs, int reps, int64_t *chunk,
+                        int *grid, double *ms);
+/* Batched restarted GMRES on every system (p, r): stops each at
+ * ||b - A x|| <= max(rtol ||b||, atol) of that system. b, x: block vectors
+ * (leading dimension ntot), x in/out, host or device (on_device). nrhs
+ * divides 256. iters: Arnoldi steps taken (shared by all systems); relres:
+ * nprob * nrhs final relative residuals (system p * nrhs + r). */
+int inti_batch_gmres(inti_batch *batch, int on_device, int nrhs, const double *b,
+                     double *x, int restart, int maxiter, double rtol, double atol,
+                     int use_graph, int *iters, double *relres);
 
 /* ---------------------------------------------------------- DIM correction */
 /* bdim_correction (src/bdim.jl:50-196). opS/opD are the SL/DL operators

@@ -55,6 +55,19 @@ SIGNATURES = {
     "inti_op_apply_device": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64, _dp, _dp]),
     "inti_op_assemble_dense": (C.c_int, [_vp, _dp]),
     "inti_op_adjoint": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "inti_batch_create": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp), _dp, _dp,
+                                    C.c_int, C.POINTER(_vp)]),
+    "inti_batch_destroy": (C.c_int, [_vp]),
+    "inti_batch_info": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "inti_batch_set_identity": (C.c_int, [_vp, _dp]),
+    "inti_batch_attach_correction": (C.c_int, [_vp, C.c_int, _vp, C.c_double, C.c_double]),
+    "inti_batch_set_launch": (C.c_int, [_vp, C.c_int64, C.c_int]),
+    "inti_batch_apply_device": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64, _dp, _dp]),
+    "inti_batch_apply": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
+    "inti_batch_autotune": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_double)]),
+    "inti_batch_gmres": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_double,
+                                   C.c_double, C.c_int, C.POINTER(C.c_int), _dp]),
     "inti_op_attach_correction": (C.c_int, [_vp, _vp, C.c_double, C.c_double]),
     "inti_op_detach_correction": (C.c_int, [_vp]),
     "inti_bdim_build": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int64, _i64p, _i64p, _i64p,

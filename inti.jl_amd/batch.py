@@ -1,0 +1,128 @@
+"""Batches of independent problems on the GPU: one block-diagonal operator
+over P quadratures (each with its own wavenumber), applied in one persistent
+launch, with a batched restarted GMRES over every (problem, right-hand side)
+system whose Arnoldi step is captured in a hipGraph.
+
+The reference has no batch; this replaces a loop over problems of
+IntegralOperator(K, Q_p, Q_p) (src/nystrom.jl:79-89) and
+gmres(L_p, b_p) (docs/src/examples/helmholtz_scattering.jl:494-495).
+
+Block vectors are (ntot, nrhs) arrays: problem p owns rows
+offsets[p]:offsets[p+1].
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .kernels import CombinedFieldKernel, Helmholtz
+
+
+class BatchOperator:
+    """BatchOperator(kernels, quads, precision="fp64"): kernels[p] over
+    quads[p] (target = source). All kernels share their kind and PDE
+    dimension; each carries its own wavenumber (and combined-field
+    coefficients)."""
+
+    def __init__(self, kernels, quads, precision="fp64", ctx=None):
+        if len(kernels) != len(quads) or not kernels:
+            raise ValueError("one kernel per quadrature, at least one problem")
+        k0 = kernels[0]
+        kind, pde, dim = k0.kind, k0.op.pde, k0.op.dim
+        for K in kernels:
+            if K.kind != kind or K.op.pde != pde or K.op.dim != dim:
+                raise ValueError("all kernels of a batch share kind, PDE and dimension")
+        self.kernels = list(kernels)
+        self.quads = list(quads)
+        self.ctx = ctx or _lib.default_context()
+        self.dtype = np.complex128 if isinstance(k0.op, Helmholtz) else np.float64
+        self.nprob = len(quads)
+        handles = (C.c_void_p * self.nprob)(*[q.device(self.ctx) for q in quads])
+        ks = np.array([K.op.k for K in kernels], dtype=np.float64)
+        coefs = None
+        if isinstance(k0, CombinedFieldKernel):
+            coefs = np.array([[K.c0.real, K.c0.imag, K.c1.real, K.c1.imag] for K in kernels]).reshape(-1)
+        prec = _lib.MIXED if precision == "mixed" else _lib.FP64
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.inti_batch_create(
+            self.ctx.handle, pde, kind, dim, self.nprob, C.cast(handles, C.POINTER(C.c_void_p)),
+            _lib.dptr(ks), _lib.dptr(coefs), prec, C.byref(h)))
+        self.handle = h
+        self._corr = []
+        nt = C.c_int64()
+        off = np.zeros(self.nprob + 1, dtype=np.int64)
+        self.ctx.check(self.ctx.lib.inti_batch_info(h, C.byref(nt), off.ctypes.data_as(C.POINTER(C.c_int64))))
+        self.ntot = nt.value
+        self.offsets = off
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and h.value and not _lib._exiting:
+            self.ctx.lib.inti_batch_destroy(h)
+            self.handle = None
+
+    def split(self, X):
+        """Per-problem row blocks of a block vector."""
+        return [X[self.offsets[p]:self.offsets[p + 1]] for p in range(self.nprob)]
+
+    def join(self, blocks):
+        return np.concatenate([np.asarray(b) for b in blocks], axis=0)
+
+    def set_identity(self, c):
+        """c * I added to every problem (scalar) or c[p] to problem p."""
+        c = np.broadcast_to(np.asarray(c, dtype=np.complex128), (self.nprob,))
+        cid = np.ascontiguousarray(np.stack([c.real, c.imag], axis=1).reshape(-1))
+        self.ctx.check(self.ctx.lib.inti_batch_set_identity(self.handle, _lib.dptr(cid)))
+
+    def attach(self, p, corr, coef=1.0):
+        """Add coef * corr (a SparseCorrection of problem p) to problem p."""
+        coef = complex(coef)
+        self.ctx.check(self.ctx.lib.inti_batch_attach_correction(self.handle, p, corr.handle, coef.real,
+                                                                 coef.imag))
+        self._corr.append(corr)
+
+    def set_launch(self, chunk_len=0, grid=0):
+        self.ctx.check(self.ctx.lib.inti_batch_set_launch(self.handle, int(chunk_len), int(grid)))
+
+    def apply(self, X, Y=None, alpha=1.0, beta=0.0):
+        """alpha A X + beta Y for a block vector X (ntot,) or (ntot, nrhs)."""
+        X = np.asarray(X)
+        vec = X.ndim == 1
+        Xc = np.ascontiguousarray(X.reshape(self.ntot, -1).T, dtype=self.dtype)
+        nrhs = Xc.shape[0]
+        Yc = (np.zeros_like(Xc) if Y is None
+              else np.ascontiguousarray(np.asarray(Y).reshape(self.ntot, -1).T, dtype=self.dtype))
+        a = np.array([complex(alpha).real, complex(alpha).imag])
+        b = np.array([complex(beta).real, complex(beta).imag])
+        self.ctx.check(self.ctx.lib.inti_batch_apply(self.handle, nrhs, _lib.dptr(Xc), _lib.dptr(Yc),
+                                                     _lib.dptr(a), _lib.dptr(b)))
+        out = Yc.T
+        return out.reshape(-1) if vec else np.ascontiguousarray(out)
+
+    def __matmul__(self, X):
+        return self.apply(X)
+
+    def autotune(self, nrhs=16, reps=3):
+        """Pick the work-item chunk length and persistent grid by timing."""
+        c, g, ms = C.c_int64(), C.c_int(), C.c_double()
+        self.ctx.check(self.ctx.lib.inti_batch_autotune(self.handle, nrhs, reps, C.byref(c), C.byref(g),
+                                                        C.byref(ms)))
+        return {"chunk_len": c.value, "grid": g.value, "ms": ms.value}
+
+    def gmres(self, B, X0=None, *, restart=30, maxiter=100, rtol=1e-6, atol=0.0, use_graph=True):
+        """Solve every system (p, r) of A X = B; returns (X, info) with
+        info["relres"] of shape (nprob, nrhs)."""
+        B = np.asarray(B)
+        vec = B.ndim == 1
+        Bc = np.ascontiguousarray(B.reshape(self.ntot, -1).T, dtype=self.dtype)
+        nrhs = Bc.shape[0]
+        Xc = (np.zeros_like(Bc) if X0 is None
+              else np.ascontiguousarray(np.asarray(X0).reshape(self.ntot, -1).T, dtype=self.dtype))
+        it = C.c_int()
+        rel = np.zeros(self.nprob * nrhs)
+        self.ctx.check(self.ctx.lib.inti_batch_gmres(
+            self.handle, 0, nrhs, _lib.dptr(Bc), _lib.dptr(Xc), restart, maxiter, rtol, atol,
+            1 if use_graph else 0, C.byref(it), _lib.dptr(rel)))
+        X = Xc.T
+        X = X.reshape(-1) if vec else np.ascontiguousarray(X)
+        return X, {"iterations": it.value, "relres": rel.reshape(self.nprob, nrhs)}

@@ -3,4 +3,5 @@
 
 namespace inti {
 ApplyFn apply_pick_helm2(int kind, int mixed, int nr) { return apply_pick_pd<INTI_HELMHOLTZ, 2>(kind, mixed, nr); }
+BatchFn batch_pick_helm2(int kind, int mixed, int nr) { return batch_pick_pd<INTI_HELMHOLTZ, 2>(kind, mixed, nr); }
 }  // namespace inti

@@ -17,6 +17,7 @@ namespace inti {
 
 struct ApplyArgs {
     int64_t m, n;
+    int64_t tstride, sstride;  // SoA strides of the target / source arrays
     const double *tx, *tn;
     const double *sx, *sn, *sw;
     const double *x;  // n x nr, leading dimension ldx (elements)
@@ -36,27 +37,27 @@ struct ApplyTile {
     static constexpr int TT = NR == 1 ? 256 : NR <= 4 ? 128 : 64;
 };
 
+// The work of one block: targets [tile*256, tile*256 + 256) against the
+// sources of chunk `chunk`; partial sums to a.part. s_geo / s_q are the
+// block's LDS staging buffers.
 template <int PDE, int DIM, int KIND, bool MIXED, int NR>
-__global__ __launch_bounds__(kApplyBS) void k_apply(ApplyArgs a) {
+__device__ __forceinline__ void apply_block(const ApplyArgs &a, int64_t chunk, int64_t tile,
+                                            double2 *s_geo, double2 *s_q) {
     constexpr bool CPLX = PDE == INTI_HELMHOLTZ;
     constexpr int BS = kApplyBS;
     constexpr int TT = ApplyTile<NR>::TT;
     constexpr int NG = DIM == 3 ? 3 : 2;  // double2 slots of a geometry record
-    __shared__ double2 s_geo[TT * NG];
-    __shared__ double2 s_q[TT * NR];
     const int tid = threadIdx.x;
-    const int64_t b = blockIdx.x;
-    const int64_t chunk = b % a.split;
-    const int64_t tile = b / a.split;
     const int64_t m = a.m, n = a.n;
+    const int64_t ts = a.tstride, ss = a.sstride;
 
     const int64_t i = tile * BS + tid;
     const int64_t ic = i < m ? i : m - 1;
     double xt[DIM], nt[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        xt[d] = a.tx[d * m + ic];
-        nt[d] = a.tn ? a.tn[d * m + ic] : 0.0;
+        xt[d] = a.tx[d * ts + ic];
+        nt[d] = a.tn ? a.tn[d * ts + ic] : 0.0;
     }
     double accr[NR], acci[NR];
 #pragma unroll
@@ -70,8 +71,8 @@ __global__ __launch_bounds__(kApplyBS) void k_apply(ApplyArgs a) {
             if (j < j1) {
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
-                    y[d] = a.sx[d * n + j];
-                    ny[d] = a.sn ? a.sn[d * n + j] : 0.0;
+                    y[d] = a.sx[d * ss + j];
+                    ny[d] = a.sn ? a.sn[d * ss + j] : 0.0;
                 }
             }
             double2 *rec = s_geo + tid * NG;
@@ -145,6 +146,65 @@ __global__ __launch_bounds__(kApplyBS) void k_apply(ApplyArgs a) {
     }
 }
 
+template <int PDE, int DIM, int KIND, bool MIXED, int NR>
+__global__ __launch_bounds__(kApplyBS) void k_apply(ApplyArgs a) {
+    constexpr int TT = ApplyTile<NR>::TT;
+    constexpr int NG = DIM == 3 ? 3 : 2;
+    __shared__ double2 s_geo[TT * NG];
+    __shared__ double2 s_q[TT * NR];
+    const int64_t b = blockIdx.x;
+    apply_block<PDE, DIM, KIND, MIXED, NR>(a, b % a.split, b / a.split, s_geo, s_q);
+}
+
+// ------------------------------------------------------------ batched apply
+// Many independent problems (block-diagonal operator) in one persistent
+// launch: the work list holds (problem, target tile, source chunk) items,
+// sorted by decreasing size; block b takes items b, b + G, b + 2G, ... so
+// that the large items are spread first and every block ends with small
+// ones. Problem p's nodes are rows [off, off + n) of the concatenated SoA
+// arrays (stride ntot); its partial sums start at part + poff.
+struct BatchItem {
+    int32_t p, tile, chunk, pad;
+};
+struct BatchProb {
+    int64_t off, n, poff, chunk;
+    int32_t split, pad;
+    KParams kp;
+};
+struct BatchArgs {
+    const BatchItem *items;
+    int64_t nitems;
+    const BatchProb *probs;
+    int64_t ntot;
+    const double *coords, *normals, *weights;
+    const double *x;  // ntot x nr (leading dimension ldx)
+    int64_t ldx;
+    double *part;
+};
+
+template <int PDE, int DIM, int KIND, bool MIXED, int NR>
+__global__ __launch_bounds__(kApplyBS) void k_apply_batch(BatchArgs b) {
+    constexpr int TT = ApplyTile<NR>::TT;
+    constexpr int NG = DIM == 3 ? 3 : 2;
+    constexpr int CW = PDE == INTI_HELMHOLTZ ? 2 : 1;
+    __shared__ double2 s_geo[TT * NG];
+    __shared__ double2 s_q[TT * NR];
+    for (int64_t it = blockIdx.x; it < b.nitems; it += gridDim.x) {
+        const BatchItem w = b.items[it];
+        const BatchProb P = b.probs[w.p];
+        ApplyArgs a;
+        a.m = P.n; a.n = P.n;
+        a.tstride = b.ntot; a.sstride = b.ntot;
+        a.tx = b.coords + P.off; a.tn = b.normals ? b.normals + P.off : nullptr;
+        a.sx = a.tx; a.sn = a.tn; a.sw = b.weights + P.off;
+        a.x = b.x + CW * P.off; a.ldx = b.ldx;
+        a.part = b.part + CW * P.poff;
+        a.split = P.split; a.chunk = P.chunk;
+        a.p = P.kp;
+        apply_block<PDE, DIM, KIND, MIXED, NR>(a, w.chunk, w.tile, s_geo, s_q);
+    }
+}
+
 using ApplyFn = void (*)(const ApplyArgs &, dim3, hipStream_t);
 
 template <int PDE, int DIM, int KIND, bool MIXED, int NR>
@@ -186,6 +246,43 @@ static ApplyFn apply_pick_pd(int kind, int mixed, int nr) {
     }
     return nullptr;
 }
+
+using BatchFn = void (*)(const BatchArgs &, dim3, hipStream_t);
+
+template <int PDE, int DIM, int KIND, bool MIXED, int NR>
+static void batch_launcher(const BatchArgs &a, dim3 grid, hipStream_t s) {
+    hipLaunchKernelGGL((k_apply_batch<PDE, DIM, KIND, MIXED, NR>), grid, dim3(kApplyBS), 0, s, a);
+}
+
+// batched kinds: SL, DL and c0 SL + c1 DL (the second-kind equations)
+template <int PDE, int DIM, bool MIXED, int NR>
+static BatchFn batch_kind(int kind) {
+    switch (kind) {
+        case INTI_SL: return batch_launcher<PDE, DIM, INTI_SL, MIXED, NR>;
+        case INTI_DL: return batch_launcher<PDE, DIM, INTI_DL, MIXED, NR>;
+        case INTI_COMB_SD: return batch_launcher<PDE, DIM, INTI_COMB_SD, MIXED, NR>;
+    }
+    return nullptr;
+}
+
+template <int PDE, int DIM>
+static BatchFn batch_pick_pd(int kind, int mixed, int nr) {
+    if (mixed) {
+        if (nr == 1) return batch_kind<PDE, DIM, true, 1>(kind);
+        if (nr == 4) return batch_kind<PDE, DIM, true, 4>(kind);
+        if (nr == 16) return batch_kind<PDE, DIM, true, 16>(kind);
+    } else {
+        if (nr == 1) return batch_kind<PDE, DIM, false, 1>(kind);
+        if (nr == 4) return batch_kind<PDE, DIM, false, 4>(kind);
+        if (nr == 16) return batch_kind<PDE, DIM, false, 16>(kind);
+    }
+    return nullptr;
+}
+
+BatchFn batch_pick_lap2(int kind, int mixed, int nr);
+BatchFn batch_pick_lap3(int kind, int mixed, int nr);
+BatchFn batch_pick_helm2(int kind, int mixed, int nr);
+BatchFn batch_pick_helm3(int kind, int mixed, int nr);
 
 ApplyFn apply_pick_lap2(int kind, int mixed, int nr);
 ApplyFn apply_pick_lap3(int kind, int mixed, int nr);

@@ -88,13 +88,54 @@ __device__ __forceinline__ T horner(const T (&c)[N], T u) {
     return s;
 }
 
+// FP64 coefficient tables of hankel01_f64, kept in memory: as inline
+// constants the ~150 coefficients would be materialised in registers ahead
+// of the direct-sum loop (more than half the register file); read through a
+// pointer the compiler cannot prove loop-invariant, they become scalar loads
+// at the point of use.
+struct HkF64Table {
+    double a0[18], a1[18], b0[18], b1[18];
+    double p0n[7], p0d[7], q0n[7], q0d[7], p1n[7], p1d[7], q1n[7], q1d[7];
+};
+
+constexpr HkF64Table make_hk_f64_table() {
+    HkF64Table t{};
+    for (int i = 0; i < 18; ++i) {
+        t.a0[i] = HK_F64_A0[i]; t.a1[i] = HK_F64_A1[i];
+        t.b0[i] = HK_F64_B0[i]; t.b1[i] = HK_F64_B1[i];
+    }
+    for (int i = 0; i < 7; ++i) {
+        t.p0n[i] = HK_F64_P0_NUM[i]; t.p0d[i] = HK_F64_P0_DEN[i];
+        t.q0n[i] = HK_F64_Q0_NUM[i]; t.q0d[i] = HK_F64_Q0_DEN[i];
+        t.p1n[i] = HK_F64_P1_NUM[i]; t.p1d[i] = HK_F64_P1_DEN[i];
+        t.q1n[i] = HK_F64_Q1_NUM[i]; t.q1d[i] = HK_F64_Q1_DEN[i];
+    }
+    return t;
+}
+
+static __constant__ HkF64Table kHkF64 = make_hk_f64_table();
+
+// the table in the constant address space: uniform reads are scalar loads
+typedef const __attribute__((address_space(4))) HkF64Table *HkF64Ptr;
+typedef const __attribute__((address_space(4))) double *ConstF64Ptr;
+
+template <int N>
+__device__ __forceinline__ double horner_p(ConstF64Ptr c, double u) {
+    double s = c[N - 1];
+#pragma unroll
+    for (int i = N - 2; i >= 0; --i) s = __builtin_fma(s, u, c[i]);
+    return s;
+}
+
 // H0^(1)(x) and H1^(1)(x), x > 0, FP64.
 __device__ __forceinline__ void hankel01_f64(double x, double &h0r, double &h0i,
                                              double &h1r, double &h1i) {
+    HkF64Ptr tb = (HkF64Ptr)&kHkF64;
+    asm volatile("" : "+s"(tb));  // opaque per call: no hoisting of the table
     if (x < HK_F64_XS) {
         double u = 0.25 * x * x;
-        double a0 = horner(HK_F64_A0, u), a1 = horner(HK_F64_A1, u);
-        double b0 = horner(HK_F64_B0, u), b1 = horner(HK_F64_B1, u);
+        double a0 = horner_p<18>(tb->a0, u), a1 = horner_p<18>(tb->a1, u);
+        double b0 = horner_p<18>(tb->b0, u), b1 = horner_p<18>(tb->b1, u);
         double L = log(0.5 * x);
         double ix = rcp64(x);
         double J0 = a0, J1 = 0.5 * x * a1;
@@ -105,13 +146,14 @@ __device__ __forceinline__ void hankel01_f64(double x, double &h0r, double &h0i,
     } else {
         double ix = rcp64(x);
         double t = (HK_F64_XS * HK_F64_XS) * ix * ix;
-        double P0 = horner(HK_F64_P0_NUM, t) * rcp64(horner(HK_F64_P0_DEN, t));
-        double Q0 = horner(HK_F64_Q0_NUM, t) * rcp64(horner(HK_F64_Q0_DEN, t)) * ix;
-        double P1 = horner(HK_F64_P1_NUM, t) * rcp64(horner(HK_F64_P1_DEN, t));
-        double Q1 = horner(HK_F64_Q1_NUM, t) * rcp64(horner(HK_F64_Q1_DEN, t)) * ix;
+        double P0 = horner_p<7>(tb->p0n, t) * rcp64(horner_p<7>(tb->p0d, t));
+        double Q0 = horner_p<7>(tb->q0n, t) * rcp64(horner_p<7>(tb->q0d, t)) * ix;
+        double P1 = horner_p<7>(tb->p1n, t) * rcp64(horner_p<7>(tb->p1d, t));
+        double Q1 = horner_p<7>(tb->q1n, t) * rcp64(horner_p<7>(tb->q1d, t)) * ix;
         double amp = kSqrt2OverPi * rsq64(x);
+        // phase x - pi/4 in revolutions: sin/cos(2 pi (x/(2 pi) - 1/8))
         double s, c;
-        sincos(x - kQuarterPi, &s, &c);
+        sincospi(2.0 * __builtin_fma(x, kInv2Pi, -0.125), &s, &c);
         // H0 = amp (P0 + i Q0)(c + i s); H1 = amp (P1 + i Q1)(s - i c)
         h0r = amp * (P0 * c - Q0 * s);
         h0i = amp * (P0 * s + Q0 * c);

@@ -195,6 +195,7 @@ int launch_apply(const DirectLaunch &L) {
             if (!fn) return set_error(ctx, INTI_ERR_UNSUPPORTED, "no direct-sum kernel for this operator");
             ApplyArgs a;
             a.m = m; a.n = n;
+            a.tstride = m; a.sstride = n;
             a.tx = op->trg->coords; a.tn = op->trg->normals;
             a.sx = op->src->coords; a.sn = op->src->normals; a.sw = op->src->weights;
             a.x = xr; a.ldx = L.ldx; a.part = part; a.split = split; a.chunk = chunk;
