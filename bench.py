@@ -96,11 +96,17 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--freq", type=int, default=23)
+    ap.add_argument("--workload", choices=["cfie", "batch", "bumpy", "shape", "circle"], default="cfie",
+                    help="cfie (default): the headline corrected combined-field matvec; batch: 256 2D "
+                         "Helmholtz problems x 16 right-hand sides; bumpy: 3D Laplace single layer on a "
+                         "~491k-node bumpy sphere with refined GMRES; shape: far-field objective and its "
+                         "adjoint on a perturbed sphere; circle: 2D Laplace DL on 640 nodes")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
                     help="profiling runs: time one leg only, no checks, no JSON")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="profiler calibration: run a plain device copy of this many bytes 5 times")
+    ap.add_argument("--nprob", type=int, default=256, help="batch workload: number of problems")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -115,6 +121,14 @@ def main():
         calibrate_copy(ctx, lib, args.copy_bytes)
         return
     ctx.set_profiling(True)
+    run = {"cfie": run_cfie, "batch": run_batch, "bumpy": run_bumpy, "shape": run_shape,
+           "circle": run_circle}[args.workload]
+    out = run(inti, lib, ctx, args)
+    if out is not None:
+        print(json.dumps(out), flush=True)
+
+
+def run_cfie(inti, lib, ctx, args):
 
     k = 2 * np.pi
     Q, x = make_workload(inti, args.freq)
@@ -215,7 +229,455 @@ def main():
     }
     ctx.check(lib.inti_device_free(ctx.handle, dx))
     ctx.check(lib.inti_device_free(ctx.handle, dy))
-    print(json.dumps(out), flush=True)
+    return out
+
+
+
+
+def _dev_buffers(ctx, lib, nbytes, host=None):
+    dx, dy = C.c_void_p(), C.c_void_p()
+    ctx.check(lib.inti_device_alloc(ctx.handle, nbytes, C.byref(dx)))
+    ctx.check(lib.inti_device_alloc(ctx.handle, nbytes, C.byref(dy)))
+    if host is not None:
+        ctx.check(lib.inti_memcpy_h2d(ctx.handle, dx, host.ctypes.data_as(C.c_void_p), nbytes))
+    return dx, dy
+
+
+def _fetch(ctx, lib, dptr, shape, dtype):
+    out = np.zeros(shape, dtype=dtype)
+    ctx.check(lib.inti_memcpy_d2h(ctx.handle, out.ctypes.data_as(C.c_void_p), dptr, out.nbytes))
+    return out
+
+
+def run_batch(inti, lib, ctx, args):
+    """256 2D Helmholtz double-layer problems on ellipses (a ~ U[1,3],
+    b ~ U[0.3,1], k ~ U[1,40], U{50..2000} P2 elements x 6 Gauss nodes),
+    16 complex Gaussian right-hand sides each, I/2 + D + dD per problem:
+    one persistent batched apply (autotuned work items), mixed vs FP64, and
+    the batched hipGraph GMRES to 1e-10."""
+    rng = np.random.default_rng(20261202)
+    P = args.nprob
+    a = rng.uniform(1, 3, P)
+    b = rng.uniform(0.3, 1, P)
+    ks = rng.uniform(1, 40, P)
+    ne = rng.integers(50, 2001, P)
+    qs = [inti.ellipse_quadrature(a[p], b[p], int(ne[p]), qorder=11) for p in range(P)]
+    ops = [inti.Helmholtz(k=ks[p], dim=2) for p in range(P)]
+    t0 = time.perf_counter()
+    corr = []
+    for p in range(P):
+        Sop = inti.IntegralOperator(inti.SingleLayerKernel(ops[p]), qs[p])
+        Dop = inti.IntegralOperator(inti.DoubleLayerKernel(ops[p]), qs[p])
+        corr.append(inti.bdim_correction(ops[p], qs[p], qs[p], Sop, Dop,
+                                         green_multiplier=np.full(len(qs[p]), -0.5))[1])
+    ctx.synchronize()
+    t_dim = time.perf_counter() - t0
+    Ks = [inti.DoubleLayerKernel(o) for o in ops]
+    B64 = inti.BatchOperator(Ks, qs)
+    B32 = inti.BatchOperator(Ks, qs, precision="mixed")
+    for Bt in (B64, B32):
+        Bt.set_identity(0.5)
+        for p in range(P):
+            Bt.attach(p, corr[p])
+    ntot = B64.ntot
+    sizes = np.diff(B64.offsets)
+    R = 16
+    pairs = float(np.sum(sizes.astype(np.float64) ** 2)) * R
+    log(f"bench batch: {P} problems, ntot={ntot}, sum N_p^2 x 16 = {pairs:.3e}, DIM {t_dim:.2f} s")
+    tune64 = B64.autotune(nrhs=R, reps=2)
+    tune32 = B32.autotune(nrhs=R, reps=2)
+    log(f"bench batch: autotune fp64 {tune64}, mixed {tune32}")
+    X = (rng.normal(size=(R, ntot)) + 1j * rng.normal(size=(R, ntot))) * np.sqrt(0.5)
+    X = np.ascontiguousarray(X)  # column-major ntot x R
+    nbytes = X.nbytes
+    dx, dy = _dev_buffers(ctx, lib, nbytes, X)
+    one = (C.c_double * 2)(1.0, 0.0)
+    zero = (C.c_double * 2)(0.0, 0.0)
+
+    def timed(Bt, steps, warmup):
+        for _ in range(warmup):
+            ctx.check(lib.inti_batch_apply_device(Bt.handle, R, dx, ntot, dy, ntot, one, zero))
+        ctx.synchronize()
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.check(lib.inti_batch_apply_device(Bt.handle, R, dx, ntot, dy, ntot, one, zero))
+            kms.append(ctx.last_kernel_ms())
+        ctx.synchronize()
+        return (time.perf_counter() - t0) / steps, float(np.mean(kms))
+
+    steps = max(3, args.steps // 4)
+    t64, k64 = timed(B64, steps, 1)
+    y64 = _fetch(ctx, lib, dy, (R, ntot), np.complex128)
+    tmx, kmx = timed(B32, steps, 1)
+    ymx = _fetch(ctx, lib, dy, (R, ntot), np.complex128)
+    err = float(np.linalg.norm(ymx - y64) / np.linalg.norm(y64))
+    log(f"bench batch: fp64 {t64 * 1e3:.2f} ms, mixed {tmx * 1e3:.2f} ms per apply, rel err {err:.2e}")
+    # FP64 batched apply vs the single-operator path on three problems
+    Xh = X.T
+    chk = []
+    for p in (0, P // 2, P - 1):
+        A = inti.IntegralOperator(Ks[p], qs[p])
+        A.set_identity(0.5)
+        A.attach(corr[p])
+        ref = A.apply(Xh[B64.offsets[p]:B64.offsets[p + 1]])
+        got = y64.T[B64.offsets[p]:B64.offsets[p + 1]]
+        chk.append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+    # batched GMRES (FP64 operator), hipGraph-captured Arnoldi step
+    t0 = time.perf_counter()
+    Xs, info = B64.gmres(Xh, restart=100, maxiter=300, rtol=1e-10)
+    t_gm = time.perf_counter() - t0
+    rel = info["relres"]
+    conv = rel.max(axis=1) <= 1e-10
+    perim = np.array([q.weights.sum() for q in qs])
+    ppw = sizes / (perim * ks / (2 * np.pi))  # nodes per wavelength
+    log(f"bench batch: gmres {info['iterations']} steps in {t_gm:.2f} s, {int(conv.sum())}/{P} problems "
+        f"converged to 1e-10; median relres {np.median(rel):.2e}; converged median ppw "
+        f"{np.median(ppw[conv]) if conv.any() else 0:.1f}, unconverged median ppw "
+        f"{np.median(ppw[~conv]) if (~conv).any() else 0:.1f}")
+    ctx.check(lib.inti_device_free(ctx.handle, dx))
+    ctx.check(lib.inti_device_free(ctx.handle, dy))
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        p = int(np.argmax(sizes))
+        Q = qs[p]
+        rows = 64
+        t1 = time.perf_counter()
+        oracle.matvec(1, 2, 1, ks[p], Q, Q, Xh[B64.offsets[p]:B64.offsets[p + 1], 0], rows=(0, rows),
+                      nthreads=threads)
+        tc = time.perf_counter() - t1
+        cpu = {"value": rows * len(Q) / tc / 1e9, "unit": "Gpair-evals/s", "cores": threads, "kind": "port",
+               "sample": f"oracle FP64 2D Helmholtz DL direct sum, rows 0..{rows} of problem {p} "
+                         f"(N={len(Q)}), one right-hand side ({tc:.2f} s)"}
+    return {
+        "metric": "Gpair-rhs-evals/s of the batched corrected DL apply (256 2D Helmholtz problems x 16 RHS), mixed",
+        "value": round(pairs / tmx / 1e9, 3), "unit": "Gpair-rhs-evals/s", "n_gpus": 1, "steps": steps,
+        "warmup": 1, "ms_per_step": round(tmx * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 arithmetic + f32 transcendentals (mixed)", "data": "synthetic",
+        "config": {"workload": f"{P} 2D Helmholtz DL problems on ellipses a~U[1,3], b~U[0.3,1], k~U[1,40], "
+                               f"U{{50..2000}} P2 elements x 6 Gauss nodes, 16 complex Gaussian RHS, "
+                               f"I/2 + D + dD (seed 20261202)",
+                   "problems": P, "ntot": int(ntot), "pair_rhs_per_step": pairs},
+        "fp64_value": round(pairs / t64 / 1e9, 3), "mixed_over_fp64": round(t64 / tmx, 3),
+        "rel_err_mixed_vs_fp64": err, "rel_err_batch_vs_single_fp64": max(chk),
+        "kernel_ms": {"fp64": round(k64, 3), "mixed": round(kmx, 3)},
+        "autotune": {"fp64": tune64, "mixed": tune32}, "dim_build_s": round(t_dim, 2),
+        "gmres": {"steps": info["iterations"], "seconds": round(t_gm, 3), "rtol": 1e-10, "restart": 100,
+                  "graph": True, "ms_per_step": round(t_gm / max(info["iterations"], 1) * 1e3, 2),
+                  "problems_converged": int(conv.sum()), "median_relres": float(np.median(rel)),
+                  "max_relres": float(rel.max()),
+                  "nodes_per_wavelength": {"min": float(ppw.min()), "median": float(np.median(ppw)),
+                                           "max": float(ppw.max())}},
+        "cpu_baseline": cpu,
+    }
+
+
+def _real_harmonics(theta, phi, lmax, rng):
+    """sum over l <= lmax, |m| <= l of c_lm Y_lm (real form), c ~ N(0, 1)."""
+    from scipy import special
+    out = np.zeros_like(theta)
+    for l in range(lmax + 1):
+        for m in range(0, l + 1):
+            Y = special.sph_harm_y(l, m, theta, phi)
+            out += rng.normal() * Y.real
+            if m > 0:
+                out += rng.normal() * Y.imag
+    return out
+
+
+def _sl_setup(inti, ctx, Q):
+    op = inti.Laplace(dim=3)
+    n = len(Q)
+    t0 = time.perf_counter()
+    S64 = inti.IntegralOperator(inti.SingleLayerKernel(op), Q)
+    D64 = inti.IntegralOperator(inti.DoubleLayerKernel(op), Q)
+    dS, _ = inti.bdim_correction(op, Q, Q, S64, D64, green_multiplier=np.full(n, -0.5))
+    ctx.synchronize()
+    t_dim = time.perf_counter() - t0
+    S64.attach(dS)
+    S32 = inti.IntegralOperator(inti.SingleLayerKernel(op), Q, precision="mixed")
+    S32.attach(dS)
+    return S64, S32, t_dim
+
+
+def _refined(inti, S64, S32, rhs, outer_max, inner_max, tag):
+    """Refinement (FP64 residuals, mixed GMRES corrections) to 1e-12, one outer
+    step per call (x carried over) so that progress is logged; then the
+    all-FP64 GMRES to 1e-13 in slices. Returns (x_ref, info_ref, t_ref, x64,
+    info64, t64)."""
+    n = len(rhs)
+    t1 = time.perf_counter()
+    xr = np.zeros(n)
+    ir = {"outer": 0, "inner_iterations": 0, "relres": 1.0}
+    for _ in range(outer_max):
+        xr, step = inti.refine_solve(S64, S32, rhs, x0=xr, restart=300, maxiter_inner=inner_max,
+                                     inner_rtol=1e-6, max_outer=1, rtol=1e-12)
+        ir = {"outer": ir["outer"] + step["outer"],
+              "inner_iterations": ir["inner_iterations"] + step["inner_iterations"], "relres": step["relres"]}
+        log(f"bench {tag}: refinement {ir} at {time.perf_counter() - t1:.1f} s")
+        if step["relres"] <= 1e-12 or step["outer"] == 0:
+            break
+    t_ref = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    x64 = np.zeros(n)
+    i64 = {"iterations": 0, "relres": 1.0}
+    for _ in range(max(1, outer_max)):
+        # the reference solve goes to 1e-13 so that the comparison of solutions
+        # measures the refined solve, not the reference's own stopping error
+        x64, step = inti.gmres(S64, rhs, x0=x64, restart=300, maxiter=inner_max, rtol=1e-13)
+        i64 = {"iterations": i64["iterations"] + step["iterations"], "relres": step["relres"]}
+        log(f"bench {tag}: fp64 gmres {i64} at {time.perf_counter() - t1:.1f} s")
+        if step["relres"] <= 1e-13 or step["iterations"] == 0:
+            break
+    return xr, ir, t_ref, x64, i64, time.perf_counter() - t1
+
+
+def run_bumpy(inti, lib, ctx, args):
+    """3D Laplace single layer, DIM-corrected, icosphere frequency 64 (81,920
+    P2 triangles x VR4N6 = 491,520 nodes): on the bumpy sphere r = 1 + 0.15
+    sin(5 theta) cos(4 phi) the mixed vs FP64 apply (the metric) and a
+    bounded refined solve; on the unit sphere of the same size the refined
+    GMRES to 1e-10 against the all-FP64 GMRES solution. The right-hand side
+    is sum_(l<=8) c_lm Y_lm, c ~ N(0, 1)."""
+    freq = 64
+    Q = inti.sphere_quadrature(freq, radial=lambda th, ph: 1.0 + 0.15 * np.sin(5 * th) * np.cos(4 * ph))
+    n = len(Q)
+    S64, S32, t_dim = _sl_setup(inti, ctx, Q)
+    rng = np.random.default_rng(20261202)
+    th = np.arccos(np.clip(Q.coords[2] / np.linalg.norm(Q.coords, axis=0), -1, 1))
+    ph = np.arctan2(Q.coords[1], Q.coords[0])
+    rhs = _real_harmonics(th, ph, 8, rng)
+    log(f"bench bumpy: nodes={n}, DIM {t_dim:.2f} s")
+    nbytes = 8 * n
+    dx, dy = _dev_buffers(ctx, lib, nbytes, np.ascontiguousarray(rhs))
+    one = (C.c_double * 2)(1.0, 0.0)
+    zero = (C.c_double * 2)(0.0, 0.0)
+    res = {}
+    for name, A in (("fp64", S64), ("mixed", S32)):
+        ctx.check(lib.inti_op_apply_device(A.handle, 1, dx, n, dy, n, one, zero))
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        steps = 5
+        kms = []
+        for _ in range(steps):
+            ctx.check(lib.inti_op_apply_device(A.handle, 1, dx, n, dy, n, one, zero))
+            kms.append(ctx.last_kernel_ms())
+        ctx.synchronize()
+        res[name] = ((time.perf_counter() - t1) / steps, float(np.mean(kms)), _fetch(ctx, lib, dy, (n,), np.float64))
+    ctx.check(lib.inti_device_free(ctx.handle, dx))
+    ctx.check(lib.inti_device_free(ctx.handle, dy))
+    pairs = float(n) * n
+    err = float(np.linalg.norm(res["mixed"][2] - res["fp64"][2]) / np.linalg.norm(res["fp64"][2]))
+    log(f"bench bumpy: fp64 {res['fp64'][0] * 1e3:.1f} ms, mixed {res['mixed'][0] * 1e3:.1f} ms, err {err:.2e}")
+    # bounded refined solve on the bumpy sphere (one outer step of 300 inner iterations)
+    xb, ib, tb, _, _, _ = _refined(inti, S64, S32, rhs, 1, 300, "bumpy")
+    del S64, S32
+    # the full refinement on the unit sphere of the same size
+    Qs = inti.sphere_quadrature(freq)
+    T64, T32, t_dim_s = _sl_setup(inti, ctx, Qs)
+    ths = np.arccos(np.clip(Qs.coords[2], -1, 1))
+    phs = np.arctan2(Qs.coords[1], Qs.coords[0])
+    rhs_s = _real_harmonics(ths, phs, 8, np.random.default_rng(20261202))
+    xr, ir, t_ref, x64, i64, t_64 = _refined(inti, T64, T32, rhs_s, 6, 600, "sphere")
+    sol_err = float(np.linalg.norm(xr - x64) / np.linalg.norm(x64))
+    log(f"bench sphere: refined vs fp64 solution {sol_err:.2e}")
+    return {
+        "metric": "Gpair-evals/s of the corrected 3D Laplace SL matvec (491,520-node bumpy sphere), mixed",
+        "value": round(pairs / res["mixed"][0] / 1e9, 3), "unit": "Gpair-evals/s", "n_gpus": 1, "steps": 5,
+        "warmup": 1, "ms_per_step": round(res["mixed"][0] * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 arithmetic + f32 rsqrt (mixed)",
+        "data": "synthetic",
+        "config": {"workload": f"3D Laplace SL, bumpy sphere r = 1 + 0.15 sin(5 theta) cos(4 phi), icosphere "
+                               f"freq {freq}: {Q.ne} P2 triangles x VR4N6 = {n} nodes, RHS sum_(l<=8) c_lm Y_lm "
+                               f"(seed 20261202), DIM-corrected", "nodes": n},
+        "fp64_value": round(pairs / res["fp64"][0] / 1e9, 3),
+        "mixed_over_fp64": round(res["fp64"][0] / res["mixed"][0], 3),
+        "rel_err_mixed_vs_fp64": err, "kernel_ms": {k: round(v[1], 3) for k, v in res.items()},
+        "dim_build_s": round(t_dim, 2),
+        "bumpy_bounded_refinement": {"seconds": round(tb, 2), **ib},
+        "unit_sphere_491520": {"dim_build_s": round(t_dim_s, 2),
+                               "refined_solve": {"seconds": round(t_ref, 2), **ir, "rtol": 1e-12},
+                               "fp64_gmres": {"seconds": round(t_64, 2), **i64},
+                               "rel_err_refined_vs_fp64_solution": sol_err},
+    }
+
+
+def run_shape(inti, lib, ctx, args):
+    """Shape-gradient objective: sound-soft scattering of e^{ikz} by the
+    sphere r = 1 + 0.1 sin(3 theta) cos(2 phi), k = 10 (icosphere frequency
+    29: 100,920 nodes), sigma from the corrected CFIE by GMRES;
+    J = sum over 64 Fibonacci directions of |u_inf|^2. The adjoint gives
+    dJ/d(sigma, nodes, normals, weights, k); checked by central differences
+    (h = 1e-6) along 20 seeded random directions; the adjoint of the full
+    N x N combined-field direct sum is timed against its forward."""
+    k = 10.0
+    Q = inti.sphere_quadrature(29, radial=lambda th, ph: 1.0 + 0.1 * np.sin(3 * th) * np.cos(2 * ph))
+    n = len(Q)
+    op = inti.Helmholtz(k=k, dim=3)
+    L, _ = inti.combined_field(op=op, quad=Q, c_single=-1j * k, c_double=1.0, c_identity=0.5)
+    b = -np.exp(1j * k * Q.coords[2])
+    t0 = time.perf_counter()
+    sig, info = inti.gmres(L, b, restart=100, maxiter=300, rtol=1e-8)
+    t_solve = time.perf_counter() - t0
+    nd = 64
+    i = np.arange(nd) + 0.5
+    pol = np.arccos(1 - 2 * i / nd)
+    az = np.pi * (1 + 5 ** 0.5) * i
+    Xh = np.stack([np.cos(az) * np.sin(pol), np.sin(az) * np.sin(pol), np.cos(pol)])
+    P = inti.points(Xh)
+
+    def quad(y, nv, w):
+        return inti.Quadrature(y, nv, w, np.arange(n, dtype=np.int64).reshape(1, -1))
+
+    def J(y, nv, w, kk, s):
+        o = inti.Helmholtz(k=kk, dim=3)
+        F = inti.IntegralOperator(inti.CombinedFieldKernel(o, -1j * kk, 1.0, "FF"), P, quad(y, nv, w))
+        u = F.apply(s)
+        return float(np.sum(np.abs(u) ** 2)), u
+
+    base = (Q.coords, Q.normals, Q.weights, k, sig)
+    J0, u0 = J(*base)
+    F = inti.IntegralOperator(inti.CombinedFieldKernel(op, -1j * k, 1.0, "FF"), P, quad(*base[:3]))
+    wrt = ("sigma", "source_nodes", "source_normals", "source_weights", "k")
+    F.pullback(2 * u0, sig, wrt=wrt, dcoefs=(-1j, 0.0))
+    ctx.synchronize()
+    reps = 10
+    ka, kf = [], []
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g = F.pullback(2 * u0, sig, wrt=wrt, dcoefs=(-1j, 0.0))
+        ka.append(ctx.last_kernel_ms())  # source pass + its reduction
+    t_adj_ff = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        F.apply(sig)
+        kf.append(ctx.last_kernel_ms())  # direct sum kernel
+    t_fwd_ff = (time.perf_counter() - t0) / reps
+    k_adj_ff, k_fwd_ff = float(np.mean(ka)), float(np.mean(kf))
+    rng = np.random.default_rng(20261202)
+    worst = 0.0
+    checks = []
+    for t in range(20):
+        which = t % 5
+        v = rng.normal(size=np.shape(base[which])) if which != 3 else 1.0
+        if which == 4:
+            v = v + 1j * rng.normal(size=n)
+            pred = float(np.real(np.vdot(g["sigma"], v)))
+        elif which == 3:
+            pred = float(g["k"])
+        else:
+            pred = float(np.sum(g[("source_nodes", "source_normals", "source_weights")[which]] * v))
+        h = 1e-6
+        ap, am = list(base), list(base)
+        ap[which] = base[which] + h * v
+        am[which] = base[which] - h * v
+        fd = (J(*ap)[0] - J(*am)[0]) / (2 * h)
+        rel = abs(fd - pred) / max(abs(pred), 1e-12)
+        worst = max(worst, rel)
+        checks.append(rel)
+    log(f"bench shape: nodes {n}, gmres {info}, J {J0:.6e}, FD worst rel {worst:.2e}")
+    # adjoint of the N x N direct sum (plain combined field) vs its forward
+    A = inti.IntegralOperator(inti.CombinedFieldKernel(op, -1j * k, 1.0), Q)
+    ubar = np.conj(u0.mean()) * np.ones(n) + 0.0j
+    A.apply(sig)
+    A.pullback(ubar, sig, wrt=("sigma",))
+    ctx.synchronize()
+    A.apply(sig)
+    t_fwd = ctx.last_kernel_ms() * 1e-3
+    A.pullback(ubar, sig, wrt=("sigma", "source_nodes", "source_normals", "source_weights", "k"))
+    t_adj_src = ctx.last_kernel_ms() * 1e-3
+    A.pullback(ubar, sig, wrt=("sigma", "source_nodes", "source_normals", "source_weights", "target_nodes", "k"))
+    t_adj_all = ctx.last_kernel_ms() * 1e-3
+    log(f"bench shape: N x N forward {t_fwd * 1e3:.1f} ms, adjoint (source side) {t_adj_src * 1e3:.1f} ms, "
+        f"(both sides) {t_adj_all * 1e3:.1f} ms")
+    return {
+        "metric": "adjoint / forward time of the far-field objective's pullback (shape gradient, k=10, ~1e5 nodes)",
+        "value": round(k_adj_ff / k_fwd_ff, 3), "unit": "ratio", "n_gpus": 1, "steps": reps, "warmup": 1,
+        "ms_per_step": round(k_adj_ff, 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"sound-soft 3D Helmholtz k=10, sphere r = 1 + 0.1 sin(3 theta) cos(2 phi), "
+                               f"icosphere freq 29: {Q.ne} P2 triangles x VR4N6 = {n} nodes, plane wave +z, "
+                               f"J = sum over 64 Fibonacci directions |u_inf|^2", "nodes": n},
+        "solve": {"seconds": round(t_solve, 2), **info},
+        "objective": J0, "fd_checks": {"directions": 20, "h": 1e-6, "worst_rel_err": worst},
+        "farfield_kernel_ms": {"forward": round(k_fwd_ff, 4), "adjoint": round(k_adj_ff, 4)},
+        "farfield_call_ms": {"forward": round(t_fwd_ff * 1e3, 3), "adjoint": round(t_adj_ff * 1e3, 3)},
+        "direct_sum_NxN_kernel_ms": {"forward": round(t_fwd * 1e3, 2), "adjoint_source_side": round(t_adj_src * 1e3, 2),
+                              "adjoint_both_sides": round(t_adj_all * 1e3, 2),
+                              "ratio_source_side": round(t_adj_src / t_fwd, 3),
+                              "ratio_both_sides": round(t_adj_all / t_fwd, 3)},
+    }
+
+
+def run_circle(inti, lib, ctx, args):
+    """2D Laplace double layer on the unit circle, 128 P2 elements x 5 Gauss
+    nodes (N = 640), sigma = cos(3 theta) + 0.1 N(0, 1): bdim_correction +
+    assemble_matrix + one corrected matvec on the GPU, and the same three
+    steps on the CPU oracle (the reference CPU path, restated)."""
+    import oracle
+    Q = inti.circle_quadrature(128, qorder=9)
+    n = len(Q)
+    rng = np.random.default_rng(20261202)
+    th = np.arctan2(Q.coords[1], Q.coords[0])
+    sig = np.cos(3 * th) + 0.1 * rng.normal(size=n)
+    op = inti.Laplace(dim=2)
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        S, D = inti.single_double_layer(op=op, target=Q, source=Q, compression={"method": "none"},
+                                        correction={"method": "dim"})
+        y = D @ sig
+        times.append(time.perf_counter() - t0)
+    t_gpu = float(np.median(times))
+    Dm = inti.IntegralOperator(inti.DoubleLayerKernel(op), Q)
+    Dm.attach(inti.bdim_correction(op, Q, Q, inti.IntegralOperator(inti.SingleLayerKernel(op), Q), Dm,
+                                   green_multiplier=np.full(n, -0.5))[1])
+    dx, dy = _dev_buffers(ctx, lib, 8 * n, np.ascontiguousarray(sig))
+    one = (C.c_double * 2)(1.0, 0.0)
+    zero = (C.c_double * 2)(0.0, 0.0)
+    for _ in range(10):
+        ctx.check(lib.inti_op_apply_device(Dm.handle, 1, dx, n, dy, n, one, zero))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    reps = 200
+    for _ in range(reps):
+        ctx.check(lib.inti_op_apply_device(Dm.handle, 1, dx, n, dy, n, one, zero))
+    ctx.synchronize()
+    t_mv = (time.perf_counter() - t0) / reps
+    ymv = _fetch(ctx, lib, dy, (n,), np.float64)
+    ctx.check(lib.inti_device_free(ctx.handle, dx))
+    ctx.check(lib.inti_device_free(ctx.handle, dy))
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        rp, ci, vS, vD, diag = oracle.bdim(0, 2, 0.0, Q, Q, np.full(n, -0.5))
+        Am = oracle.assemble(0, 2, 1, 0.0, Q, Q).real
+        yo = Am @ sig + oracle.csr_matvec(n, n, rp, ci, vD, sig).real
+        times.append(time.perf_counter() - t0)
+    t_cpu = float(np.median(times))
+    # the oracle's own rounding spread: the same algorithm built with FMA
+    # contraction (the DIM least squares here have R-diagonal ratios ~5e8)
+    rpf, cif, _, vDf, _ = oracle.bdim(0, 2, 0.0, Q, Q, np.full(n, -0.5), variant="fma")
+    yf = Am @ sig + oracle.csr_matvec(n, n, rpf, cif, vDf, sig).real
+    spread = float(np.linalg.norm(yf - yo) / np.linalg.norm(yo))
+    err = float(np.linalg.norm(ymv - yo) / np.linalg.norm(yo))
+    err_dense = float(np.linalg.norm(y - yo) / np.linalg.norm(yo))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "metric": "Gpair-evals/s of corrected DL matvec, 2D Laplace unit circle N=640, FP64",
+        "value": round(float(n) * n / t_mv / 1e9, 3), "unit": "Gpair-evals/s", "n_gpus": 1, "steps": reps,
+        "warmup": 10, "ms_per_step": round(t_mv * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "2D Laplace DL, unit circle, 128 P2 elements x 5 Gauss nodes = 640 nodes, "
+                               "sigma = cos(3 theta) + 0.1 N(0,1) (seed 20261202)", "nodes": n},
+        "pipeline_s": {"gpu_bdim_assemble_matvec": round(t_gpu, 4), "cpu_oracle_bdim_assemble_matvec": round(t_cpu, 4)},
+        "rel_err_gpu_vs_oracle": err, "rel_err_gpu_dense_vs_oracle": err_dense,
+        "oracle_rounding_spread": spread, "dim_rdiag_ratio": float(diag[0]),
+        "cpu_baseline": {"value": round(t_cpu, 4), "unit": "s per bdim + assemble + matvec", "cores": threads,
+                         "kind": "port", "sample": "the whole 640-node pipeline, median of 5"},
+    }
 
 
 if __name__ == "__main__":

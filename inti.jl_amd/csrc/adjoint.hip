@@ -111,11 +111,16 @@ __device__ __forceinline__ void radial(double d2, const AdjParams &p, double2 &g
     }
 }
 
-// Value and derivatives of one pair; x the target (or far-field direction).
+// Value and derivatives of one pair, x the target (or far-field direction),
+// in the factored form
+//   dK/dy = a_r v + a_n n_y,   dK/dn_y = b_r v,
+// with v = r = x - y for the translation-invariant kernels and v = xh for
+// the far fields (where a_n = 0); gk = dK/dk. The sums then carry a_r v,
+// a_n and b_r v, and the per-source normal is applied once at the end.
 template <int PDE, int DIM, int KIND>
 __device__ __forceinline__ void pair_grad(const double *x, const double *y, const double *ny,
-                                          const AdjParams &p, double2 &K, double2 (&gy)[DIM],
-                                          double2 (&gn)[DIM], double2 &gk) {
+                                          const AdjParams &p, double (&v)[DIM], double2 &K,
+                                          double2 &a_r, double2 &a_n, double2 &b_r, double2 &gk) {
     if constexpr (KIND >= INTI_FF_SL) {
         const double k = p.k;
         const double xy = dotv<DIM>(x, y);
@@ -124,65 +129,52 @@ __device__ __forceinline__ void pair_grad(const double *x, const double *y, cons
         const double2 E = c2(c, s);  // e^{-ik xh.y}
         const double2 PE = cm(p.P, E);
         const double2 mixy = c2(0.0, -xy);  // -i xh.y
-        double2 Ks = PE, Kd = c2(0, 0), gkS = ca(cm(p.Pk, E), cm(PE, mixy)), gkD = c2(0, 0);
-        double xn = 0.0;
+        const double2 Ks = PE, gkS = ca(cm(p.Pk, E), cm(PE, mixy));
+        double2 Kd = c2(0, 0), gkD = c2(0, 0);
         if constexpr (KIND != INTI_FF_SL) {
-            xn = dotv<DIM>(x, ny);
+            const double xn = dotv<DIM>(x, ny);
             const double2 B = c2(0.0, -k * xn);  // -ik xh.n
             Kd = cm(PE, B);
             gkD = ca(ca(cm(cm(p.Pk, E), B), cm(PE, c2(0.0, -xn))), cm(Kd, mixy));
         }
-        double2 kn;  // coefficient of xh in dK/dn_y
         if constexpr (KIND == INTI_FF_SL) {
-            K = Ks; gk = gkS; kn = c2(0, 0);
+            K = Ks; gk = gkS; b_r = c2(0, 0);
         } else if constexpr (KIND == INTI_FF_DL) {
-            K = Kd; gk = gkD; kn = cm(PE, c2(0.0, -k));
+            K = Kd; gk = gkD; b_r = cm(PE, c2(0.0, -k));
         } else {
             K = ca(cm(p.c0, Ks), cm(p.c1, Kd));
             gk = ca(ca(cm(p.dc0, Ks), cm(p.c0, gkS)), ca(cm(p.dc1, Kd), cm(p.c1, gkD)));
-            kn = cm(p.c1, cm(PE, c2(0.0, -k)));
+            b_r = cm(p.c1, cm(PE, c2(0.0, -k)));
         }
-        const double2 mik_K = cm(c2(0.0, -k), K);  // dK/dy = -ik xh K
+        a_r = cm(c2(0.0, -k), K);  // dK/dy = -ik xh K
+        a_n = c2(0, 0);
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            gy[d] = cs(mik_K, x[d]);
-            gn[d] = cs(kn, x[d]);
-        }
+        for (int d = 0; d < DIM; ++d) v[d] = x[d];
     } else {
-        double r[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) r[d] = x[d] - y[d];
-        const double d2 = dotv<DIM>(r, r);
+        for (int d = 0; d < DIM; ++d) v[d] = x[d] - y[d];
+        const double d2 = dotv<DIM>(v, v);
         const bool ok = d2 > kSamePointTol2;
         const double d2s = fmax(d2, kClampD2);
         constexpr bool WG = KIND != INTI_DL, WH = KIND != INTI_SL;
         double2 g = c2(0, 0), gpd = c2(0, 0), gkg = c2(0, 0), h = c2(0, 0), hpd = c2(0, 0), hkk = c2(0, 0);
         radial<PDE, DIM, WG, WH>(d2s, p, g, gpd, gkg, h, hpd, hkk);
-        const double rn = WH ? dotv<DIM>(r, ny) : 0.0;
-        // SL part: K = g, dK/dy = -gpd r; DL part: K = h rn,
-        // dK/dy = -(hpd rn r + h n), dK/dn = h r
-        double2 a_r, a_n, Kv;  // dK/dy = a_r r + a_n n;  dK/dn = b_r r
-        double2 b_r = c2(0, 0);
+        const double rn = WH ? dotv<DIM>(v, ny) : 0.0;
+        // SL: K = g, dK/dy = -gpd r;  DL: K = h rn, dK/dy = -(hpd rn r + h n), dK/dn = h r
         if constexpr (KIND == INTI_SL) {
-            Kv = g; gk = gkg; a_r = cs(gpd, -1.0); a_n = c2(0, 0);
+            K = g; gk = gkg; a_r = cs(gpd, -1.0); a_n = c2(0, 0); b_r = c2(0, 0);
         } else if constexpr (KIND == INTI_DL) {
-            Kv = cs(h, rn); gk = cs(hkk, rn); a_r = cs(hpd, -rn); a_n = cs(h, -1.0); b_r = h;
+            K = cs(h, rn); gk = cs(hkk, rn); a_r = cs(hpd, -rn); a_n = cs(h, -1.0); b_r = h;
         } else {  // c0 SL + c1 DL
-            const double2 KS = g, KD = cs(h, rn);
-            Kv = ca(cm(p.c0, KS), cm(p.c1, KD));
-            gk = ca(ca(cm(p.dc0, KS), cm(p.c0, gkg)), ca(cm(p.dc1, KD), cm(p.c1, cs(hkk, rn))));
+            const double2 KD = cs(h, rn);
+            K = ca(cm(p.c0, g), cm(p.c1, KD));
+            gk = ca(ca(cm(p.dc0, g), cm(p.c0, gkg)), ca(cm(p.dc1, KD), cm(p.c1, cs(hkk, rn))));
             a_r = ca(cm(p.c0, cs(gpd, -1.0)), cm(p.c1, cs(hpd, -rn)));
-            a_n = cm(p.c1, cs(h, -1.0));
             b_r = cm(p.c1, h);
+            a_n = cs(b_r, -1.0);
         }
         const double mk = ok ? 1.0 : 0.0;
-        K = cs(Kv, mk);
-        gk = cs(gk, mk);
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            gy[d] = cs(ca(cs(a_r, r[d]), cs(a_n, ny ? ny[d] : 0.0)), mk);
-            gn[d] = cs(b_r, r[d] * mk);
-        }
+        K = cs(K, mk); gk = cs(gk, mk); a_r = cs(a_r, mk); a_n = cs(a_n, mk); b_r = cs(b_r, mk);
     }
 }
 
@@ -199,12 +191,12 @@ struct AdjArgs {
     AdjParams p;
 };
 
-// Source-major pass: NV = 2 + 2 DIM complex sums per source:
-// [sum c K, sum c dK/dy (DIM), sum c dK/dn (DIM), sum c dK/dk]
+// Source-major pass: NV = 3 + 2 DIM complex sums per source,
+// [sum c K, sum c a_r v (DIM), sum c b_r v (DIM), sum c a_n, sum c gk]
 template <int PDE, int DIM, int KIND>
 __global__ __launch_bounds__(kAdjBS) void k_adj_src(AdjArgs a) {
-    constexpr int NV = 2 + 2 * DIM;
-    __shared__ double s_x[kAdjBS * DIM];
+    constexpr int NV = 3 + 2 * DIM;
+    __shared__ double s_x[DIM][kAdjBS];
     __shared__ double2 s_c[kAdjBS];
     const int tid = threadIdx.x;
     const int64_t chunk = blockIdx.x % a.split, tile = blockIdx.x / a.split;
@@ -218,46 +210,51 @@ __global__ __launch_bounds__(kAdjBS) void k_adj_src(AdjArgs a) {
     }
     double2 acc[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] = c2(0, 0);
+    for (int q = 0; q < NV; ++q) acc[q] = c2(0, 0);
     const int64_t i0 = chunk * a.chunk;
     const int64_t i1 = i0 + a.chunk < a.m ? i0 + a.chunk : a.m;
     for (int64_t t0 = i0; t0 < i1; t0 += kAdjBS) {
         const int64_t i = t0 + tid;
         const bool live = i < i1;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) s_x[tid * DIM + d] = live ? a.tx[d * a.m + i] : 0.0;  // unused: the loop stops at cnt
+        for (int d = 0; d < DIM; ++d) s_x[d][tid] = live ? a.tx[d * a.m + i] : 0.0;  // unused: the loop stops at cnt
         s_c[tid] = live ? a.c[i] : c2(0, 0);
         __syncthreads();
         const int cnt = (int)(i1 - t0 < kAdjBS ? i1 - t0 : kAdjBS);
         for (int q = 0; q < cnt; ++q) {
-            double x[DIM];
+            double x[DIM], v[DIM];
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) x[d] = s_x[q * DIM + d];
-            double2 K, gy[DIM], gn[DIM], gk;
-            pair_grad<PDE, DIM, KIND>(x, y, ny, a.p, K, gy, gn, gk);
+            for (int d = 0; d < DIM; ++d) x[d] = s_x[d][q];
+            double2 K, a_r, a_n, b_r, gk;
+            pair_grad<PDE, DIM, KIND>(x, y, ny, a.p, v, K, a_r, a_n, b_r, gk);
             const double2 cc = s_c[q];
             cfma(acc[0], cc, K);
+            const double2 car = cm(cc, a_r), cbr = cm(cc, b_r);
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                cfma(acc[1 + d], cc, gy[d]);
-                cfma(acc[1 + DIM + d], cc, gn[d]);
+                acc[1 + d].x = __builtin_fma(car.x, v[d], acc[1 + d].x);
+                acc[1 + d].y = __builtin_fma(car.y, v[d], acc[1 + d].y);
+                acc[1 + DIM + d].x = __builtin_fma(cbr.x, v[d], acc[1 + DIM + d].x);
+                acc[1 + DIM + d].y = __builtin_fma(cbr.y, v[d], acc[1 + DIM + d].y);
             }
-            cfma(acc[NV - 1], cc, gk);
+            cfma(acc[1 + 2 * DIM], cc, a_n);
+            cfma(acc[2 + 2 * DIM], cc, gk);
         }
         __syncthreads();
     }
     if (j < a.n) {
         double2 *out = a.part + (chunk * a.n + j) * NV;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) out[v] = acc[v];
+        for (int q = 0; q < NV; ++q) out[q] = acc[q];
     }
 }
 
-// Target-major pass: DIM complex sums per target, sum_j s_j dK/dx_i.
+// Target-major pass: DIM complex sums per target, sum_j s_j dK/dx_i
+// (= -sum_j s_j (a_r r + a_n n_j)).
 template <int PDE, int DIM, int KIND>
 __global__ __launch_bounds__(kAdjBS) void k_adj_trg(AdjArgs a) {
-    __shared__ double s_y[kAdjBS * DIM];
-    __shared__ double s_n[kAdjBS * DIM];
+    __shared__ double s_y[DIM][kAdjBS];
+    __shared__ double s_n[DIM][kAdjBS];
     __shared__ double2 s_s[kAdjBS];
     const int tid = threadIdx.x;
     const int64_t chunk = blockIdx.x % a.split, tile = blockIdx.x / a.split;
@@ -276,24 +273,28 @@ __global__ __launch_bounds__(kAdjBS) void k_adj_trg(AdjArgs a) {
         const bool live = j < j1;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-            s_y[tid * DIM + d] = live ? a.sx[d * a.n + j] : 0.0;  // unused: the loop stops at cnt
-            s_n[tid * DIM + d] = live && a.sn ? a.sn[d * a.n + j] : 0.0;
+            s_y[d][tid] = live ? a.sx[d * a.n + j] : 0.0;  // unused: the loop stops at cnt
+            s_n[d][tid] = live && a.sn ? a.sn[d * a.n + j] : 0.0;
         }
         s_s[tid] = live ? cs(a.sig[j], a.sw[j]) : c2(0, 0);
         __syncthreads();
         const int cnt = (int)(j1 - t0 < kAdjBS ? j1 - t0 : kAdjBS);
         for (int q = 0; q < cnt; ++q) {
-            double y[DIM], ny[DIM];
+            double y[DIM], ny[DIM], v[DIM];
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                y[d] = s_y[q * DIM + d];
-                ny[d] = s_n[q * DIM + d];
+                y[d] = s_y[d][q];
+                ny[d] = s_n[d][q];
             }
-            double2 K, gy[DIM], gn[DIM], gk;
-            pair_grad<PDE, DIM, KIND>(x, y, ny, a.p, K, gy, gn, gk);
+            double2 K, a_r, a_n, b_r, gk;
+            pair_grad<PDE, DIM, KIND>(x, y, ny, a.p, v, K, a_r, a_n, b_r, gk);
             const double2 ss = s_s[q];
+            const double2 sar = cm(ss, a_r), san = cm(ss, a_n);
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) cfma(acc[d], ss, cs(gy[d], -1.0));  // dK/dx = -dK/dy
+            for (int d = 0; d < DIM; ++d) {  // dK/dx = -dK/dy
+                acc[d].x -= __builtin_fma(sar.x, v[d], san.x * ny[d]);
+                acc[d].y -= __builtin_fma(sar.y, v[d], san.y * ny[d]);
+            }
         }
         __syncthreads();
     }
@@ -308,7 +309,7 @@ struct AdjOut {
     int64_t m, n;
     int split, dim, cplx;
     const double2 *part, *c, *sig;
-    const double *sw;
+    const double *sw, *sn;
     double *sigma_bar, *ybar, *nbar, *wbar, *kpart, *xbar;
 };
 
@@ -316,18 +317,18 @@ struct AdjOut {
 // per-block partial sums of kbar
 template <int DIM>
 __global__ __launch_bounds__(256) void k_adj_src_reduce(AdjOut o) {
-    constexpr int NV = 2 + 2 * DIM;
+    constexpr int NV = 3 + 2 * DIM;
     __shared__ double s_k[256];
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     double kb = 0.0;
     if (j < o.n) {
         double2 S[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) S[v] = c2(0, 0);
+        for (int q = 0; q < NV; ++q) S[q] = c2(0, 0);
         for (int c = 0; c < o.split; ++c) {
             const double2 *p = o.part + ((int64_t)c * o.n + j) * NV;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) S[v] = ca(S[v], p[v]);
+            for (int q = 0; q < NV; ++q) S[q] = ca(S[q], p[q]);
         }
         const double w = o.sw[j];
         const double2 sg = o.sig[j];
@@ -341,9 +342,12 @@ __global__ __launch_bounds__(256) void k_adj_src_reduce(AdjOut o) {
             }
         }
         if (o.wbar) o.wbar[j] = sg.x * S[0].x - sg.y * S[0].y;
+        const double2 AN = S[1 + 2 * DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-            if (o.ybar) o.ybar[d * o.n + j] = s.x * S[1 + d].x - s.y * S[1 + d].y;
+            const double nd = o.sn ? o.sn[d * o.n + j] : 0.0;
+            const double2 gy = make_double2(S[1 + d].x + AN.x * nd, S[1 + d].y + AN.y * nd);
+            if (o.ybar) o.ybar[d * o.n + j] = s.x * gy.x - s.y * gy.y;
             if (o.nbar) o.nbar[d * o.n + j] = s.x * S[1 + DIM + d].x - s.y * S[1 + DIM + d].y;
         }
         kb = s.x * S[NV - 1].x - s.y * S[NV - 1].y;
@@ -513,7 +517,7 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
     int64_t chunk_s, chunk_t = 0;
     plan(n, m, ctx->num_cu, split_s, chunk_s);
     if (xbar) plan(m, n, ctx->num_cu, split_t, chunk_t);
-    const int NVs = 2 + 2 * dim;
+    const int NVs = 3 + 2 * dim;
     const int64_t nsb = (n + 255) / 256;
     Dev dc, dsg, dpart, dsb, dyb, dnb, dwb, dkp, dxb;
     const size_t part_bytes = std::max(sizeof(double2) * (size_t)split_s * n * NVs,
@@ -536,6 +540,7 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
     o.m = m; o.n = n; o.dim = dim; o.cplx = cplx;
     o.part = (const double2 *)dpart.p; o.c = (const double2 *)dc.p; o.sig = (const double2 *)dsg.p;
     o.sw = op->src->weights;
+    o.sn = op->src->normals;
     o.sigma_bar = (double *)dsb.p; o.ybar = (double *)dyb.p; o.nbar = (double *)dnb.p;
     o.wbar = (double *)dwb.p; o.kpart = (double *)dkp.p; o.xbar = (double *)dxb.p;
 
