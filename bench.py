@@ -307,6 +307,12 @@ def run_batch(inti, lib, ctx, args):
         return (time.perf_counter() - t0) / steps, float(np.mean(kms))
 
     steps = max(3, args.steps // 4)
+    if args.only:  # profiling runs: one apply leg, no solver, no JSON
+        t, kms = timed(B32 if args.only == "mixed" else B64, steps, 1)
+        log(f"bench batch: {args.only} only: {t * 1e3:.2f} ms/apply, kernel {kms:.2f} ms")
+        ctx.check(lib.inti_device_free(ctx.handle, dx))
+        ctx.check(lib.inti_device_free(ctx.handle, dy))
+        return None
     t64, k64 = timed(B64, steps, 1)
     y64 = _fetch(ctx, lib, dy, (R, ntot), np.complex128)
     tmx, kmx = timed(B32, steps, 1)
@@ -516,6 +522,17 @@ def run_shape(inti, lib, ctx, args):
     Q = inti.sphere_quadrature(29, radial=lambda th, ph: 1.0 + 0.1 * np.sin(3 * th) * np.cos(2 * ph))
     n = len(Q)
     op = inti.Helmholtz(k=k, dim=3)
+    if args.only:  # profiling runs: a random density instead of the solve, kernels only, no JSON
+        rng = np.random.default_rng(1)
+        sig = rng.normal(size=n) + 1j * rng.normal(size=n)
+        A = inti.IntegralOperator(inti.CombinedFieldKernel(op, -1j * k, 1.0), Q)
+        for _ in range(3):
+            A.apply(sig)
+            A.pullback(sig, sig, wrt=("sigma", "source_nodes", "source_normals", "source_weights", "k"))
+            A.pullback(sig, sig, wrt=("sigma", "source_nodes", "target_nodes"))
+        ctx.synchronize()
+        log("bench shape: profiling legs done")
+        return None
     L, _ = inti.combined_field(op=op, quad=Q, c_single=-1j * k, c_double=1.0, c_identity=0.5)
     b = -np.exp(1j * k * Q.coords[2])
     t0 = time.perf_counter()
