@@ -153,6 +153,22 @@ int inti_op_adjoint(inti_op *op, const double *ubar, const double *sigma,
                     double *sigma_bar, double *src_xyz_bar,
                     double *src_normal_bar, double *src_weight_bar,
                     double *trg_xyz_bar, const double *dcoefs, double *k_bar);
+/* Dense matrix of the operator written to device memory (column-major,
+ * ldout = number of targets): assemble_matrix (src/nystrom.jl:103-124) kept
+ * in HBM for the dense products below. */
+int inti_op_assemble_dense_device(inti_op *op, double *out, int64_t ldout);
+/* A += coef * csr on a dense device matrix (the axpy!(true, dS, Smat) of
+ * src/api.jl:171-173). */
+int inti_dense_add_csr(inti_ctx *ctx, const inti_csr *csr, double coef_re,
+                       double coef_im, double *A, int64_t lda);
+/* Y = alpha A X + beta Y for an assembled dense m x n matrix A and nrhs
+ * columns, all device pointers, column-major, complex interleaved when
+ * is_complex: the dense `S * sigma` of src/api.jl:171-173 on many densities
+ * at once, a real GEMM on the FP64 matrix cores (complex A is multiplied in
+ * its real form). */
+int inti_dense_gemm(inti_ctx *ctx, int is_complex, int64_t m, int64_t n, int nrhs,
+                    const double *A, int64_t lda, const double *X, int64_t ldx,
+                    double *Y, int64_t ldy, const double *alpha, const double *beta);
 /* Fuse coef * csr into every following apply of op (the sparse correction
  * of src/api.jl:171-185). Several attachments with the same pattern are
  * merged into one CSR. */

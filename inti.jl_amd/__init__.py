@@ -11,6 +11,7 @@ from . import _lib
 from .api import (CorrectedOperator, adj_double_layer_hypersingular, combined_field,
                   single_double_layer, single_double_layer_potential)
 from .batch import BatchOperator
+from .dense import DenseOperator
 from .bdim import DimParameters, bdim_correction, etype_to_nearest_points
 from .kernels import (AdjointDoubleLayerKernel, CombinedFieldKernel, DoubleLayerKernel,
                       FarFieldDoubleLayerKernel, FarFieldSingleLayerKernel, Helmholtz,
@@ -40,7 +41,7 @@ __all__ = [
     "polynomials", "vdim_auxiliary_quantities", "vdim_correction",
     "volume_potential", "IntegralPotential", "SingleLayerPotential", "DoubleLayerPotential",
     "single_double_layer_potential", "FarFieldSingleLayerKernel", "FarFieldDoubleLayerKernel",
-    "rrule", "BatchOperator",
+    "rrule", "BatchOperator", "DenseOperator",
     "default_context", "default_density_eltype", "default_kernel_eltype",
     "ellipse_quadrature", "etype_to_nearest_points", "gauss_legendre", "gauss_legendre_order", "gmres",
     "refine_solve",

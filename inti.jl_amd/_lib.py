@@ -55,6 +55,10 @@ SIGNATURES = {
     "inti_op_apply_device": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64, _dp, _dp]),
     "inti_op_assemble_dense": (C.c_int, [_vp, _dp]),
     "inti_op_adjoint": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "inti_op_assemble_dense_device": (C.c_int, [_vp, _vp, C.c_int64]),
+    "inti_dense_add_csr": (C.c_int, [_vp, _vp, C.c_double, C.c_double, _vp, C.c_int64]),
+    "inti_dense_gemm": (C.c_int, [_vp, C.c_int, C.c_int64, C.c_int64, C.c_int, _vp, C.c_int64, _vp,
+                                  C.c_int64, _vp, C.c_int64, _dp, _dp]),
     "inti_batch_create": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp), _dp, _dp,
                                     C.c_int, C.POINTER(_vp)]),
     "inti_batch_destroy": (C.c_int, [_vp]),

@@ -384,6 +384,55 @@ int inti_op_assemble_dense(inti_op *op, double *out) {
     return st;
 }
 
+int inti_op_assemble_dense_device(inti_op *op, double *out, int64_t ldout) {
+    if (!op) return INTI_ERR_ARG;
+    Ctx *ctx = op->ctx;
+    REQUIRE(ctx, out != nullptr, "assemble_device: out is null");
+    REQUIRE(ctx, ldout == op->trg->n, "assemble_device: the leading dimension must equal the number of targets");
+    LOCK(ctx);
+    return launch_assemble(op, out);
+}
+
+// A[i + lda j] += coef * csr(i, j): the axpy!(true, dS, Smat) of src/api.jl:171-173
+__global__ void k_dense_add_csr(int64_t m, const int32_t *rowptr, const int32_t *colind, const double *val,
+                                int cplx, double cr, double ci, double *A, int64_t lda) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+        const int64_t o = i + lda * (int64_t)colind[q];
+        if (cplx) {
+            const double vr = val[2 * q], vi = val[2 * q + 1];
+            A[2 * o] += cr * vr - ci * vi;
+            A[2 * o + 1] += cr * vi + ci * vr;
+        } else {
+            A[o] += cr * val[q];
+        }
+    }
+}
+
+int inti_dense_add_csr(inti_ctx *ctx, const inti_csr *csr, double coef_re, double coef_im, double *A,
+                       int64_t lda) {
+    if (!ctx) return INTI_ERR_ARG;
+    REQUIRE(ctx, csr && A, "dense_add_csr: null argument");
+    REQUIRE(ctx, lda >= csr->m, "dense_add_csr: leading dimension below the row count");
+    LOCK(ctx);
+    if (csr->m > 0)
+        hipLaunchKernelGGL(k_dense_add_csr, dim3((unsigned)((csr->m + 255) / 256)), dim3(256), 0, ctx->stream, csr->m,
+                           csr->rowptr, csr->colind, csr->val, csr->is_complex, coef_re, coef_im, A, lda);
+    INTI_HIP(ctx, hipGetLastError());
+    return INTI_OK;
+}
+
+int inti_dense_gemm(inti_ctx *ctx, int is_complex, int64_t m, int64_t n, int nrhs, const double *A, int64_t lda,
+                    const double *X, int64_t ldx, double *Y, int64_t ldy, const double *alpha, const double *beta) {
+    if (!ctx) return INTI_ERR_ARG;
+    REQUIRE(ctx, A && X && Y && alpha && beta, "dense_gemm: null argument");
+    REQUIRE(ctx, m >= 0 && n >= 0 && nrhs >= 0, "dense_gemm: negative size");
+    REQUIRE(ctx, lda >= m && ldx >= n && ldy >= m, "dense_gemm: leading dimension too small");
+    LOCK(ctx);
+    return dense_gemm(ctx, is_complex ? 1 : 0, m, n, nrhs, A, lda, X, ldx, Y, ldy, alpha, beta);
+}
+
 int inti_op_adjoint(inti_op *op, const double *ubar, const double *sigma, double *sigma_bar,
                     double *src_xyz_bar, double *src_normal_bar, double *src_weight_bar,
                     double *trg_xyz_bar, const double *dcoefs, double *k_bar) {
