@@ -24,7 +24,7 @@ from .quadrature import (Quadrature, boundary_quadrature, circle_quadrature, cur
                          disk_quadratures, ellipse_disk_quadratures, ellipse_quadrature,
                          gauss_legendre, gauss_legendre_order, points, sphere_quadrature,
                          vioreanu_rokhlin_triangle)
-from . import polynomials
+from . import gmsh, polynomials
 from .vdim import vdim_auxiliary_quantities, vdim_correction, volume_potential
 
 IntiError = _lib.IntiError
@@ -38,7 +38,7 @@ __all__ = [
     "SparseCorrection", "_green_multiplier", "adj_double_layer_hypersingular",
     "assemble_matrix", "bdim_correction", "circle_quadrature", "combined_field",
     "disk_quadratures", "ellipse_disk_quadratures", "boundary_quadrature", "curve_quadrature",
-    "polynomials", "vdim_auxiliary_quantities", "vdim_correction",
+    "polynomials", "gmsh", "vdim_auxiliary_quantities", "vdim_correction",
     "volume_potential", "IntegralPotential", "SingleLayerPotential", "DoubleLayerPotential",
     "single_double_layer_potential", "FarFieldSingleLayerKernel", "FarFieldDoubleLayerKernel",
     "rrule", "BatchOperator", "DenseOperator",

@@ -1,0 +1,112 @@
+"""Gmsh MSH 4.1 ingest (ext/IntiGmshExt.jl:49-221): the binary fixture
+docs/assets/piece.msh of the reference (a copy in tests/golden/) and ASCII
+round trips of P2 meshes written here. Pins: node and element counts, the
+divergence theorem on every volume's boundary against the volume of its
+tetrahedra (orientation), closure of each boundary, and exact agreement of a
+round-tripped P2 disk with the quadratures built from the same elements
+(node permutation of tri6 / line3)."""
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def piece(inti):
+    return inti.gmsh.read_msh(os.path.join(GOLDEN, "piece.msh"))
+
+
+def test_piece_contents(inti, piece):
+    assert piece.nodes.shape == (4191, 3)
+    assert piece.element_types() == ["line2", "point", "tet4", "tri3"]
+    assert piece.entity_tags(3) == [1, 2, 3, 4]
+    ntet = sum(len(n) for _, n in piece.elements(3))
+    assert ntet == 459 + 12661 + 458 + 457
+
+
+@pytest.mark.parametrize("vols", [[1], [2], [3], [4], None])
+def test_piece_boundary_orientation(inti, piece, vols):
+    """(1/3) sum w x.n = |volume| (sum of the tetrahedra) and sum w n = 0."""
+    G = inti.gmsh
+    Q = G.boundary_quadrature(piece, volumes=vols, qorder=4)
+    vol = G.tetra_volume(piece, volumes=vols)
+    flux = float((Q.weights * (Q.coords * Q.normals).sum(axis=0)).sum()) / 3
+    assert abs(flux - vol) <= 1e-12 * vol
+    assert np.abs((Q.weights * Q.normals).sum(axis=1)).max() <= 1e-13 * Q.weights.sum()
+
+
+def test_ascii_round_trip_p2_disk(inti, tmp_path):
+    """A P2 disk (curved boundary edges) written as tri6 + line3 blocks and
+    read back gives the same volume and boundary quadratures."""
+    G = inti.gmsh
+    V, B = inti.disk_quadratures(3, a=1.5, b=0.8, qorder=5, boundary_qorder=10)
+    tri = V.elements                       # (ne, 6, 2) reference order
+    lin = B.elements                       # (nb, 3, 2)
+    pts = np.concatenate([tri.reshape(-1, 2), lin.reshape(-1, 2)])
+    nodes = np.concatenate([pts, np.zeros((len(pts), 1))], axis=1)
+    nt = tri.shape[0] * 6
+    tri_idx = np.arange(nt).reshape(-1, 6)
+    lin_idx = nt + np.arange(lin.shape[0] * 3).reshape(-1, 3)
+    path = str(tmp_path / "disk.msh")
+    G.write_msh(path, nodes, [(2, 1, 9, tri_idx), (1, 1, 8, lin_idx)],
+                entities={(2, 1): {"physical": [7], "bounding": [1]}, (1, 1): {"physical": [], "bounding": []}},
+                physical_names={(2, 7): "disk interior"})
+    m = G.read_msh(path)
+    assert m.physical_names == {(2, 7): "disk interior"}
+    V2 = G.entity_quadrature(m, 2, qorder=5)
+    B2 = G.entity_quadrature(m, 1, qorder=10)
+    np.testing.assert_allclose(V2.coords, V.coords, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(V2.weights, V.weights, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(B2.coords, B.coords, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(B2.normals, B.normals, rtol=0, atol=1e-15)
+
+
+def test_ascii_round_trip_sphere_surface(inti, tmp_path):
+    """tri6 surface elements in 3D: the surface quadrature of the read mesh
+    equals the generator's (normals included)."""
+    G = inti.gmsh
+    Q = inti.sphere_quadrature(3)
+    vals = Q.elements
+    nodes = vals.reshape(-1, 3)
+    idx = np.arange(len(nodes)).reshape(-1, 6)
+    path = str(tmp_path / "sphere.msh")
+    G.write_msh(path, nodes, [(2, 1, 9, idx)])
+    m = G.read_msh(path)
+    Q2 = G.entity_quadrature(m, 2, qorder=4)
+    np.testing.assert_allclose(Q2.coords, Q.coords, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(Q2.weights, Q.weights, rtol=1e-14, atol=0)
+    np.testing.assert_allclose(Q2.normals, Q.normals, rtol=0, atol=1e-14)
+
+
+def test_unsupported_inputs(inti, tmp_path):
+    p = tmp_path / "v2.msh"
+    p.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n")
+    with pytest.raises(ValueError):
+        inti.gmsh.read_msh(str(p))
+
+
+@pytest.mark.gpu
+def test_piece_double_layer_and_green_identity(inti, piece):
+    """DIM-corrected Laplace layers on the boundary of the piece (41,352
+    nodes, flat triangles, sharp edges): D maps 1 to -1/2 on the surface
+    (docs/src/tutorials/layer_potentials.md:60, test/dim_test.jl:79-80), and
+    Green's identity S[du/dn] - D[u] = u/2 holds for u = 1/|x - x0|, x0
+    outside (test/dim_test.jl:54-62)."""
+    print("piece double layer", flush=True)
+    Q = inti.gmsh.boundary_quadrature(piece, qorder=4)
+    op = inti.Laplace(dim=3)
+    S, D = inti.single_double_layer(op=op, target=Q, source=Q, compression={"method": "gpu"},
+                                    correction={"method": "dim"})
+    err = np.abs(D @ np.ones(len(Q)) + 0.5)
+    x0 = np.array([0.0, 0.0, 3.0])
+    r = Q.coords - x0[:, None]
+    d = np.linalg.norm(r, axis=0)
+    u = 1 / (4 * np.pi * d)
+    dudn = -(r * Q.normals).sum(axis=0) / (4 * np.pi * d**3)
+    gerr = np.abs(S @ dudn - D @ u - 0.5 * u) / np.abs(u).max()
+    print(f"piece: nodes {len(Q)}, |D1 + 1/2| median {np.median(err):.2e} max {err.max():.2e}; "
+          f"green median {np.median(gerr):.2e} max {gerr.max():.2e}", flush=True)
+    assert np.median(err) < 1e-3 and np.median(gerr) < 1e-3
+    assert err.max() < 5e-2 and gerr.max() < 5e-2
