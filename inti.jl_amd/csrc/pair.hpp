@@ -20,6 +20,7 @@ namespace inti {
 struct KParams {
     double k;          // wavenumber
     double kappa;      // k / (2 pi): phase in revolutions per unit length
+    double kappa_half; // kappa / 2
     double chr, chi;   // normalised combined-field coefficient (see post_factor)
     double postr, posti;
 };
@@ -118,6 +119,46 @@ __device__ __forceinline__ void helm3_raw(const double *r, double d2s, bool ok, 
     ki = cf * bi + sf * br;
 }
 
+// MIXED 3D Helmholtz SL, DL, ADL and DL + ch*SL with the fewest VALU
+// operations: the FP32 estimate y of 1/d is used unrefined for amplitudes
+// (relative error ~1e-7); only the phase kappa*d gets one Newton step,
+// kappa*d = kappa*e*(1 + h/2) with e = d2*y, h = 1 - d2*y^2 (error ~h^2);
+// the phase is reduced with fract (any whole number of revolutions drops
+// out), and coincident points are masked on the FP32 amplitude factor. d2 is
+// the unclamped squared distance; FP32 clamping keeps every intermediate
+// finite for d2 = 0.
+template <int KIND>
+__device__ __forceinline__ void helm3_raw_mixed(const double *r, double d2, const double *nx,
+                                                const double *ny, const KParams &p, double &kr,
+                                                double &ki) {
+    const bool ok = d2 > kSamePointTol2;
+    const float y32 = __builtin_amdgcn_rsqf(fmaxf((float)d2, 1.0e-30f));
+    const double y = (double)y32;
+    const double e = d2 * y;
+    const double h = __builtin_fma(-e, y, 1.0);
+    const double t = __builtin_fma(p.kappa_half * e, h, p.kappa * e);
+    const float u = (float)__builtin_amdgcn_fract(t);
+    const float ym = ok ? y32 : 0.0f;
+    const double cf = (double)(__builtin_amdgcn_cosf(u) * ym);
+    const double sf = (double)(__builtin_amdgcn_sinf(u) * ym);
+    if constexpr (KIND == INTI_SL) {
+        kr = cf; ki = sf;
+    } else {
+        double br, bi;
+        if constexpr (KIND == INTI_COMB_SD) {
+            const double a = dotv<3>(r, ny) * y;
+            br = __builtin_fma(a, y, p.chr);
+            bi = __builtin_fma(-p.k, a, p.chi);
+        } else {  // DL / ADL: e*id*[(a id, -k a)] with a = r.n id
+            const double a = dotv<3>(r, KIND == INTI_DL ? ny : nx) * y;
+            br = a * y;
+            bi = -p.k * a;
+        }
+        kr = cf * br - sf * bi;
+        ki = cf * bi + sf * br;
+    }
+}
+
 // ---------------------------------------------------------------- Helmholtz 2D
 // K = (i/4) Z; raw chosen so that post = i/4 (SL), i k/4 (DL, HS),
 // -i k/4 (ADL), i c1 k/4 (COMB_SD with ch = c0/(c1 k), COMB_AH with
@@ -192,6 +233,10 @@ __device__ __forceinline__ void pair_raw(const double *r, const double *nx, cons
     if constexpr (PDE == INTI_LAPLACE) {
         kr = laplace_raw<DIM, KIND, MIXED>(r, d2s, ok, nx, ny, p);
         ki = 0.0;
+    } else if constexpr (DIM == 3 && MIXED &&
+                         (KIND == INTI_SL || KIND == INTI_DL || KIND == INTI_ADL ||
+                          KIND == INTI_COMB_SD)) {
+        helm3_raw_mixed<KIND>(r, d2, nx, ny, p, kr, ki);
     } else if constexpr (DIM == 3) {
         helm3_raw<KIND, MIXED>(r, d2s, ok, nx, ny, p, kr, ki);
     } else {
@@ -220,6 +265,7 @@ inline KParams basic_params(double k) {
     KParams p;
     p.k = k;
     p.kappa = k * kInv2Pi;
+    p.kappa_half = 0.5 * p.kappa;
     p.chr = p.chi = 0.0;
     p.postr = 1.0; p.posti = 0.0;
     return p;
