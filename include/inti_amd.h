@@ -120,6 +120,15 @@ int inti_op_set_identity(inti_op *op, double re, double im);
 /* Launch shape: threads per block, targets per thread, source splits
  * (0 = automatic). Used by the tuner; defaults are sensible. */
 int inti_op_set_launch(inti_op *op, int block, int tpt, int split);
+/* Symmetric evaluation of a square operator whose target and source are the
+ * same quadrature handle (single right-hand side): each unordered pair's
+ * shared part (distance, e^{ikd}, Hankel values) is computed once and applied
+ * in both directions. mode 0: off; 1 (default): Helmholtz operators; 2: every
+ * operator it applies to. group: tiles of 256 nodes per work group (1, 2 or
+ * 4; 0 = the smallest whose partial sums fit 4 GiB). Results are
+ * deterministic in every mode. No counterpart in Inti.jl (its matrices are
+ * assembled entry by entry). */
+int inti_op_set_symmetric(inti_op *op, int mode, int group);
 
 /* y = alpha*(A + dA)*x + beta*y for nrhs columns; host pointers
  * (mul!(y, A, x, alpha, beta) of LinearMaps, src/api.jl:182-185). */

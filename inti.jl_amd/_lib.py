@@ -51,6 +51,7 @@ SIGNATURES = {
     "inti_op_destroy": (C.c_int, [_vp]),
     "inti_op_set_identity": (C.c_int, [_vp, C.c_double, C.c_double]),
     "inti_op_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "inti_op_set_symmetric": (C.c_int, [_vp, C.c_int, C.c_int]),
     "inti_op_apply": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
     "inti_op_apply_device": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64, _dp, _dp]),
     "inti_op_assemble_dense": (C.c_int, [_vp, _dp]),

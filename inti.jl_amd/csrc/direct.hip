@@ -5,6 +5,7 @@
 #include <complex>
 
 #include "apply_kernel.hpp"
+#include "apply_sym.hpp"
 
 namespace inti {
 
@@ -97,7 +98,40 @@ static ApplyFn pick_apply(int pde, int dim, int kind, int mixed, int nr) {
     return nullptr;
 }
 
+static SymFn pick_sym(int pde, int dim, int kind, int mixed, int G) {
+    if (pde == INTI_LAPLACE && dim == 2) return sym_pick_lap2(kind, mixed, G);
+    if (pde == INTI_LAPLACE && dim == 3) return sym_pick_lap3(kind, mixed, G);
+    if (pde == INTI_HELMHOLTZ && dim == 2) return sym_pick_helm2(kind, mixed, G);
+    if (pde == INTI_HELMHOLTZ && dim == 3) return sym_pick_helm3(kind, mixed, G);
+    return nullptr;
+}
+
 bool op_is_complex(const Op *op) { return op->pde == INTI_HELMHOLTZ; }
+
+// partial sums of the symmetric evaluation are capped: beyond this the
+// plain direct sum runs
+constexpr size_t kSymMaxBytes = size_t(4) << 30;
+
+// Symmetric plan for one right-hand side of op: tiles of 256 in groups of G
+// (the smallest of 1, 2, 4 whose S x n partials fit kSymMaxBytes). False if
+// the operator does not qualify.
+static bool plan_sym(const Op *op, int kind, int nr, int &G, int64_t &T, int64_t &S) {
+    if (op->sym == 0 || nr != 1 || op->trg != op->src || kind >= INTI_FF_SL) return false;
+    if (op->sym == 1 && op->pde != INTI_HELMHOLTZ) return false;
+    const int64_t n = op->src->n;
+    T = (n + kSymBS - 1) / kSymBS;
+    if (T < 2) return false;
+    const size_t cw = op_is_complex(op) ? 2 : 1;
+    for (int g : {1, 2, 4}) {
+        if (op->sym_group && g != op->sym_group) continue;
+        S = (T + g - 1) / g;
+        if ((size_t)S * (size_t)n * cw * sizeof(double) <= kSymMaxBytes) {
+            G = g;
+            return true;
+        }
+    }
+    return false;
+}
 
 KParams op_params(const Op *op, int *eff_kind) {
     using cd = std::complex<double>;
@@ -163,8 +197,14 @@ int apply_reserve(const Op *op, int nrhs) {
     int64_t chunk;
     plan_split(op, m, n, split, chunk);
     const int nrmax = nrhs >= 16 ? 16 : nrhs >= 4 ? 4 : 1;
+    size_t bytes = sizeof(double) * cw * (size_t)m * split * nrmax;
+    int kind = op->kind, G;
+    int64_t T, S;
+    (void)op_params(op, &kind);
+    if (nrhs >= 1 && plan_sym(op, kind, 1, G, T, S))
+        bytes = std::max(bytes, sizeof(double) * cw * (size_t)S * (size_t)n);
     void *scr = nullptr;
-    return ctx_scratch(op->ctx, kScrPartial, sizeof(double) * cw * (size_t)m * split * nrmax, &scr);
+    return ctx_scratch(op->ctx, kScrPartial, bytes, &scr);
 }
 
 int launch_apply(const DirectLaunch &L) {
@@ -185,12 +225,36 @@ int launch_apply(const DirectLaunch &L) {
     if (st) return st;
     double *part = (double *)scr;
     const int64_t ntiles = (m + kApplyBS - 1) / kApplyBS;
+    int symG = 0;
+    int64_t symT = 0, symS = 0;
+    const bool sym = plan_sym(op, kind, 1, symG, symT, symS);
+    if (sym) {
+        const size_t bytes = sizeof(double) * cw * (size_t)symS * (size_t)n;
+        st = ctx_scratch(ctx, kScrPartial, bytes, &scr);
+        if (st) return st;
+        part = (double *)scr;
+    }
     for (int r0 = 0; r0 < L.nrhs;) {
         const int left = L.nrhs - r0;
         const int nr = left >= 16 ? 16 : left >= 4 ? 4 : 1;
         const double *xr = L.x + (int64_t)r0 * L.ldx * cw;
         double *yr = L.y + (int64_t)r0 * L.ldy * cw;
-        if (n > 0) {
+        int nsplit = split;
+        if (n > 0 && nr == 1 && sym) {
+            SymFn fn = pick_sym(op->pde, op->dim, kind, op->precision == INTI_MIXED, symG);
+            if (!fn) return set_error(ctx, INTI_ERR_UNSUPPORTED, "no symmetric kernel for this operator");
+            SymArgs a;
+            a.n = n;
+            a.xy = op->src->coords; a.nrm = op->src->normals; a.w = op->src->weights;
+            a.x = xr; a.part = part;
+            a.ntiles = symT; a.ngroups = symS;
+            a.p = kp;
+            if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+            fn(a, symS * (symS + 1) / 2, ctx->stream);
+            INTI_HIP(ctx, hipGetLastError());
+            if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+            nsplit = (int)symS;
+        } else if (n > 0) {
             ApplyFn fn = pick_apply(op->pde, op->dim, kind, op->precision == INTI_MIXED, nr);
             if (!fn) return set_error(ctx, INTI_ERR_UNSUPPORTED, "no direct-sum kernel for this operator");
             ApplyArgs a;
@@ -207,7 +271,7 @@ int launch_apply(const DirectLaunch &L) {
             if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         }
         ReduceArgs ra;
-        ra.m = m; ra.split = n > 0 ? split : 0; ra.part = part;
+        ra.m = m; ra.split = n > 0 ? nsplit : 0; ra.part = part;
         ra.ldx = L.ldx; ra.ldy = L.ldy;
         ra.rowptr = op->corr ? op->corr->rowptr : nullptr;
         ra.colind = op->corr ? op->corr->colind : nullptr;

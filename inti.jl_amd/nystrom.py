@@ -155,6 +155,12 @@ class IntegralOperator:
         self.ctx.check(self.ctx.lib.inti_op_set_identity(self.handle, c.real, c.imag))
         self.identity = c
 
+    def set_symmetric(self, mode, group=0):
+        """Symmetric evaluation when target is source (single right-hand
+        side): mode 0 off, 1 Helmholtz operators (default), 2 every
+        operator; group = tiles of 256 nodes per work group (0 automatic)."""
+        self.ctx.check(self.ctx.lib.inti_op_set_symmetric(self.handle, int(mode), int(group)))
+
     def attach(self, corr, coef=1.0):
         """Fuse coef * corr (a SparseCorrection) into every apply."""
         coef = complex(coef)
