@@ -108,6 +108,8 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="profiler calibration: run a plain device copy of this many bytes 5 times")
     ap.add_argument("--nprob", type=int, default=256, help="batch workload: number of problems")
+    ap.add_argument("--sym", type=int, choices=[0, 1], default=None,
+                    help="symmetric evaluation of operators on one quadrature: 0 off, 1 on (default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -116,6 +118,8 @@ def main():
 
     from __graft_entry__ import load_package
     inti = load_package()
+    if args.sym is not None:
+        inti.set_default_symmetric(args.sym)
     lib = inti._lib.load()
     ctx = inti.default_context()
     if args.copy_bytes:
@@ -171,20 +175,21 @@ def run_cfie(inti, lib, ctx, args):
     log(f"bench: fp64 {t64 * 1e3:.3f} ms/apply (kernel {k64:.3f} ms), "
         f"mixed {tmx * 1e3:.3f} ms/apply (kernel {kmx:.3f} ms), rel err {err_mx:.2e}")
 
-    # FP64 GPU vs the oracle on a row sample (parity of the measured result)
-    import oracle
-    rows = (0, 48)
-    ref = oracle.matvec(1, 3, 4, k, Q, Q, x, coefs=(-1j * k, 1.0), rows=rows)[:rows[1]]
-    rp, ci, vals = corr[0].export()
-    _, _, vd = corr[1].export()
-    cv = vd - 1j * k * vals
-    for i in range(rows[1]):
-        ref[i] += np.dot(cv[rp[i]:rp[i + 1]], x[ci[rp[i]:rp[i + 1]]])
-    err64 = float(np.linalg.norm(y64[:rows[1]] - ref) / np.linalg.norm(ref))
-    log(f"bench: fp64 vs oracle (rows 0..{rows[1]}): {err64:.2e}")
-
     cpu = None
+    err64 = None
     if not args.no_cpu:
+        # the CPU baseline leg: the oracle times a row sample of the same
+        # operator and checks the FP64 GPU result on its first rows
+        import oracle
+        rows = (0, 48)
+        ref = oracle.matvec(1, 3, 4, k, Q, Q, x, coefs=(-1j * k, 1.0), rows=rows)[:rows[1]]
+        rp, ci, vals = corr[0].export()
+        _, _, vd = corr[1].export()
+        cv = vd - 1j * k * vals
+        for i in range(rows[1]):
+            ref[i] += np.dot(cv[rp[i]:rp[i + 1]], x[ci[rp[i]:rp[i + 1]]])
+        err64 = float(np.linalg.norm(y64[:rows[1]] - ref) / np.linalg.norm(ref))
+        log(f"bench: fp64 vs oracle (rows 0..{rows[1]}): {err64:.2e}")
         cpu = cpu_baseline(oracle, Q, x, k)
         log(f"bench: cpu baseline {cpu['value']:.4f} Gpair/s on {cpu['cores']} threads")
 
@@ -224,8 +229,10 @@ def run_cfie(inti, lib, ctx, args):
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_VALU_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_VALU_TFLOPS, 4),
                      "traffic": traffic,
-                     "kernel": "k_apply<HELMHOLTZ,3,COMB_SD,MIXED>", "kernel_ms": round(kmx, 4),
-                     "flops_per_pair": FLOPS_PER_PAIR},
+                     "kernel": ("k_apply<HELMHOLTZ,3,COMB_SD,MIXED>" if args.sym == 0 else
+                                "k_apply_sym<HELMHOLTZ,3,COMB_SD,MIXED,G=1>"),
+                     "kernel_ms": round(kmx, 4), "flops_per_pair": FLOPS_PER_PAIR,
+                     "symmetric": args.sym != 0},
         "cpu_baseline": cpu,
     }
     ctx.check(lib.inti_device_free(ctx.handle, dx))
@@ -478,6 +485,8 @@ def run_bumpy(inti, lib, ctx, args):
     pairs = float(n) * n
     err = float(np.linalg.norm(res["mixed"][2] - res["fp64"][2]) / np.linalg.norm(res["fp64"][2]))
     log(f"bench bumpy: fp64 {res['fp64'][0] * 1e3:.1f} ms, mixed {res['mixed'][0] * 1e3:.1f} ms, err {err:.2e}")
+    if args.only:  # apply legs only (profiling and launch comparisons), no solves, no JSON
+        return None
     # bounded refined solve on the bumpy sphere (one outer step of 300 inner iterations)
     xb, ib, tb, _, _, _ = _refined(inti, S64, S32, rhs, 1, 300, "bumpy")
     del S64, S32
@@ -633,8 +642,8 @@ def run_circle(inti, lib, ctx, args):
     """2D Laplace double layer on the unit circle, 128 P2 elements x 5 Gauss
     nodes (N = 640), sigma = cos(3 theta) + 0.1 N(0, 1): bdim_correction +
     assemble_matrix + one corrected matvec on the GPU, and the same three
-    steps on the CPU oracle (the reference CPU path, restated)."""
-    import oracle
+    steps on the CPU oracle (the reference CPU path, restated) as the CPU
+    baseline leg."""
     Q = inti.circle_quadrature(128, qorder=9)
     n = len(Q)
     rng = np.random.default_rng(20261202)
@@ -667,22 +676,35 @@ def run_circle(inti, lib, ctx, args):
     ymv = _fetch(ctx, lib, dy, (n,), np.float64)
     ctx.check(lib.inti_device_free(ctx.handle, dx))
     ctx.check(lib.inti_device_free(ctx.handle, dy))
-    times = []
-    for _ in range(5):
-        t0 = time.perf_counter()
-        rp, ci, vS, vD, diag = oracle.bdim(0, 2, 0.0, Q, Q, np.full(n, -0.5))
-        Am = oracle.assemble(0, 2, 1, 0.0, Q, Q).real
-        yo = Am @ sig + oracle.csr_matvec(n, n, rp, ci, vD, sig).real
-        times.append(time.perf_counter() - t0)
-    t_cpu = float(np.median(times))
-    # the oracle's own rounding spread: the same algorithm built with FMA
-    # contraction (the DIM least squares here have R-diagonal ratios ~5e8)
-    rpf, cif, _, vDf, _ = oracle.bdim(0, 2, 0.0, Q, Q, np.full(n, -0.5), variant="fma")
-    yf = Am @ sig + oracle.csr_matvec(n, n, rpf, cif, vDf, sig).real
-    spread = float(np.linalg.norm(yf - yo) / np.linalg.norm(yo))
-    err = float(np.linalg.norm(ymv - yo) / np.linalg.norm(yo))
-    err_dense = float(np.linalg.norm(y - yo) / np.linalg.norm(yo))
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cpu_leg = {}
+    if not args.no_cpu:
+        # the CPU baseline leg: the oracle runs the same pipeline and checks
+        # the GPU results
+        import oracle
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            rp, ci, vS, vD, diag = oracle.bdim(0, 2, 0.0, Q, Q, np.full(n, -0.5))
+            Am = oracle.assemble(0, 2, 1, 0.0, Q, Q).real
+            yo = Am @ sig + oracle.csr_matvec(n, n, rp, ci, vD, sig).real
+            times.append(time.perf_counter() - t0)
+        t_cpu = float(np.median(times))
+        # the oracle's own rounding spread: the same algorithm built with FMA
+        # contraction (the DIM least squares here have R-diagonal ratios ~5e8)
+        rpf, cif, _, vDf, _ = oracle.bdim(0, 2, 0.0, Q, Q, np.full(n, -0.5), variant="fma")
+        yf = Am @ sig + oracle.csr_matvec(n, n, rpf, cif, vDf, sig).real
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu_leg = {
+            "pipeline_s": {"gpu_bdim_assemble_matvec": round(t_gpu, 4),
+                           "cpu_oracle_bdim_assemble_matvec": round(t_cpu, 4)},
+            "rel_err_gpu_vs_oracle": float(np.linalg.norm(ymv - yo) / np.linalg.norm(yo)),
+            "rel_err_gpu_dense_vs_oracle": float(np.linalg.norm(y - yo) / np.linalg.norm(yo)),
+            "oracle_rounding_spread": float(np.linalg.norm(yf - yo) / np.linalg.norm(yo)),
+            "dim_rdiag_ratio": float(diag[0]),
+            "cpu_baseline": {"value": round(t_cpu, 4), "unit": "s per bdim + assemble + matvec",
+                             "cores": threads, "kind": "port",
+                             "sample": "the whole 640-node pipeline, median of 5"},
+        }
     return {
         "metric": "Gpair-evals/s of corrected DL matvec, 2D Laplace unit circle N=640, FP64",
         "value": round(float(n) * n / t_mv / 1e9, 3), "unit": "Gpair-evals/s", "n_gpus": 1, "steps": reps,
@@ -690,11 +712,8 @@ def run_circle(inti, lib, ctx, args):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "2D Laplace DL, unit circle, 128 P2 elements x 5 Gauss nodes = 640 nodes, "
                                "sigma = cos(3 theta) + 0.1 N(0,1) (seed 20261202)", "nodes": n},
-        "pipeline_s": {"gpu_bdim_assemble_matvec": round(t_gpu, 4), "cpu_oracle_bdim_assemble_matvec": round(t_cpu, 4)},
-        "rel_err_gpu_vs_oracle": err, "rel_err_gpu_dense_vs_oracle": err_dense,
-        "oracle_rounding_spread": spread, "dim_rdiag_ratio": float(diag[0]),
-        "cpu_baseline": {"value": round(t_cpu, 4), "unit": "s per bdim + assemble + matvec", "cores": threads,
-                         "kind": "port", "sample": "the whole 640-node pipeline, median of 5"},
+        "gpu_pipeline_s": round(t_gpu, 4),
+        "cpu_baseline": None, **cpu_leg,
     }
 
 

@@ -19,7 +19,7 @@ from .kernels import (AdjointDoubleLayerKernel, CombinedFieldKernel, DoubleLayer
                       default_kernel_eltype)
 from .nystrom import (DoubleLayerPotential, IntegralOperator, IntegralPotential,
                       SingleLayerPotential, SparseCorrection, _green_multiplier, assemble_matrix,
-                      gmres, refine_solve, rrule)
+                      gmres, refine_solve, rrule, set_default_symmetric)
 from .quadrature import (Quadrature, boundary_quadrature, circle_quadrature, curve_quadrature,
                          disk_quadratures, ellipse_disk_quadratures, ellipse_quadrature,
                          gauss_legendre, gauss_legendre_order, points, sphere_quadrature,
@@ -34,7 +34,7 @@ Context = _lib.Context
 __all__ = [
     "AdjointDoubleLayerKernel", "CombinedFieldKernel", "Context", "CorrectedOperator",
     "DimParameters", "DoubleLayerKernel", "Helmholtz", "HyperSingularKernel",
-    "IntegralOperator", "IntiError", "Laplace", "Quadrature", "SingleLayerKernel",
+    "IntegralOperator", "IntiError", "set_default_symmetric", "Laplace", "Quadrature", "SingleLayerKernel",
     "SparseCorrection", "_green_multiplier", "adj_double_layer_hypersingular",
     "assemble_matrix", "bdim_correction", "circle_quadrature", "combined_field",
     "disk_quadratures", "ellipse_disk_quadratures", "boundary_quadrature", "curve_quadrature",

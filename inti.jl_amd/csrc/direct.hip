@@ -117,7 +117,6 @@ constexpr size_t kSymMaxBytes = size_t(4) << 30;
 // the operator does not qualify.
 static bool plan_sym(const Op *op, int kind, int nr, int &G, int64_t &T, int64_t &S) {
     if (op->sym == 0 || nr != 1 || op->trg != op->src || kind >= INTI_FF_SL) return false;
-    if (op->sym == 1 && op->pde != INTI_HELMHOLTZ) return false;
     const int64_t n = op->src->n;
     T = (n + kSymBS - 1) / kSymBS;
     if (T < 2) return false;

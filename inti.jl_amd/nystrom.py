@@ -98,6 +98,17 @@ def DoubleLayerPotential(op, source):
     return IntegralPotential(DoubleLayerKernel(op), source)
 
 
+_SYMMETRIC_DEFAULT = None
+
+
+def set_default_symmetric(mode):
+    """Symmetric-evaluation mode given to every IntegralOperator created
+    afterwards (see IntegralOperator.set_symmetric); None keeps the
+    library's default (1)."""
+    global _SYMMETRIC_DEFAULT
+    _SYMMETRIC_DEFAULT = None if mode is None else int(mode)
+
+
 class IntegralOperator:
     """IntegralOperator(kernel, target, source=target) (nystrom.jl:67-89).
 
@@ -141,6 +152,8 @@ class IntegralOperator:
                 _lib.dptr(coefs), self.target.device(self.ctx), self.source.device(self.ctx),
                 prec, C.byref(h)))
             self._h = h
+            if _SYMMETRIC_DEFAULT is not None:
+                self.ctx.check(self.ctx.lib.inti_op_set_symmetric(h, _SYMMETRIC_DEFAULT, 0))
         return self._h
 
     def __del__(self):
@@ -157,8 +170,8 @@ class IntegralOperator:
 
     def set_symmetric(self, mode, group=0):
         """Symmetric evaluation when target is source (single right-hand
-        side): mode 0 off, 1 Helmholtz operators (default), 2 every
-        operator; group = tiles of 256 nodes per work group (0 automatic)."""
+        side): mode 0 off, 1 on (default); group = tiles of 256 nodes per
+        work group (0 automatic)."""
         self.ctx.check(self.ctx.lib.inti_op_set_symmetric(self.handle, int(mode), int(group)))
 
     def attach(self, corr, coef=1.0):
