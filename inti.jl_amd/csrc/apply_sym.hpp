@@ -8,14 +8,14 @@
 // Work decomposition. The n nodes are cut into T tiles of 256 and the tiles
 // into S groups of G. Block (a, b), a <= b, takes the target tiles of group a
 // against the source tiles of group b (within a diagonal block, the tile
-// pairs I <= J only). Wave w of a block holds 64 targets. For each 64-source
-// sub-tile, lane l at step s pairs its target with source (l + s) mod 64, so
-// the 64 lanes touch 64 different sources, read from LDS without bank
-// conflicts. The transposed sum for source (l + s) mod 64 travels with the
-// step: after each step lane l takes lane l+1's running sum (ds_bpermute, a
-// crossbar move with no LDS storage), and after 64 steps lane l holds the sum
-// for source l over the wave's 64 targets. The four waves' sums are added in
-// LDS in a fixed order.
+// pairs I <= J only). A wave holds 64 x TPL targets (TPL = 1 or 2 per lane).
+// For each 64-source sub-tile, lane l at step s pairs its targets with
+// source (l + s) mod 64, so the 64 lanes touch 64 different sources, read
+// from LDS without bank conflicts. The transposed sum for source (l + s) mod
+// 64 travels with the step: after each step lane l takes lane l+1's running
+// sum (ds_bpermute, a crossbar move with no LDS storage), and after 64 steps
+// lane l holds the sum for source l over the wave's targets. The waves' sums
+// are added in LDS in a fixed order.
 //
 // Partial sums. A row of group c receives exactly one partial per group:
 // from block (c, b), b >= c (direct part) in slot b, and from block (a, c),
@@ -84,28 +84,68 @@ __device__ __forceinline__ double lane_next(double v, int addr) {
     return __hiloint2double(hi, lo);
 }
 
-template <int DIM>
-__device__ __forceinline__ void sym_read(const double2 (*s)[kSymBS], int j, double *y, double *nv,
-                                         double &qr, double &qi) {
-    if constexpr (DIM == 3) {
-        const double2 v0 = s[0][j], v1 = s[1][j], v2 = s[2][j], v3 = s[3][j];
-        y[0] = v0.x; y[1] = v0.y; y[2] = v1.x;
-        nv[0] = v1.y; nv[1] = v2.x; nv[2] = v2.y;
-        qr = v3.x; qi = v3.y;
-    } else {
-        const double2 v0 = s[0][j], v1 = s[1][j], v2 = s[2][j];
-        y[0] = v0.x; y[1] = v0.y;
-        nv[0] = v1.x; nv[1] = v1.y;
-        qr = v2.x; qi = v2.y;
-    }
-}
+// LDS record of a source node: its coordinates, its normal when the kernel
+// reads one (every kind but SL), and q (1 or 2 doubles), packed in double2
+// slots; each slot is an array over the tile, so lanes reading consecutive
+// nodes hit consecutive banks.
+template <int PDE, int DIM, int KIND>
+struct SymRec {
+    static constexpr bool CPLX = PDE == INTI_HELMHOLTZ;
+    static constexpr bool NRM = KIND != INTI_SL;
+    static constexpr int NF = DIM + (NRM ? DIM : 0) + (CPLX ? 2 : 1);
+    static constexpr int NS = (NF + 1) / 2;
 
-template <int PDE, int DIM, int KIND, bool MIXED, int G>
-__global__ __launch_bounds__(kSymBS) void k_apply_sym(SymArgs A) {
-    constexpr bool CPLX = PDE == INTI_HELMHOLTZ;
-    constexpr int NS = DIM == 3 ? 4 : 3;
-    __shared__ double2 s_src[NS][kSymBS];
-    __shared__ double2 s_red[4][kSymBS];
+    __device__ static __forceinline__ void put(double2 (*s)[kSymBS], int j, const double *y,
+                                               const double *nv, double qr, double qi) {
+        double f[2 * NS];
+        int c = 0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) f[c++] = y[d];
+        if constexpr (NRM) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) f[c++] = nv[d];
+        }
+        f[c++] = qr;
+        if constexpr (CPLX) f[c++] = qi;
+        if (c < 2 * NS) f[c] = 0.0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) s[k][j] = make_double2(f[2 * k], f[2 * k + 1]);
+    }
+    __device__ static __forceinline__ void get(const double2 (*s)[kSymBS], int j, double *y,
+                                               double *nv, double &qr, double &qi) {
+        double f[2 * NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const double2 v = s[k][j];
+            f[2 * k] = v.x;
+            f[2 * k + 1] = v.y;
+        }
+        int c = 0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) y[d] = f[c++];
+        if constexpr (NRM) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) nv[d] = f[c++];
+        } else {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) nv[d] = 0.0;
+        }
+        qr = f[c++];
+        qi = CPLX ? f[c] : 0.0;
+    }
+};
+
+// TPL targets per lane (one LDS read of a source then serves TPL pairs): a
+// block of 256 / TPL threads still covers a tile of 256 targets; thread t
+// holds targets t, t + 256 / TPL, ...
+template <int PDE, int DIM, int KIND, bool MIXED, int G, int TPL>
+__global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
+    using R = SymRec<PDE, DIM, KIND>;
+    constexpr bool CPLX = R::CPLX;
+    constexpr int BS = kSymBS / TPL;  // threads
+    constexpr int NW = BS / 64;       // waves
+    __shared__ double2 s_src[R::NS][kSymBS];
+    __shared__ double2 s_red[NW][kSymBS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nxt = ((lane + 1) & 63) << 2;  // ds_bpermute address of lane l+1
     int64_t a, b;
@@ -113,26 +153,20 @@ __global__ __launch_bounds__(kSymBS) void k_apply_sym(SymArgs A) {
     const bool diag = a == b;
     const int64_t n = A.n, T = A.ntiles;
     const KParams &p = A.p;
-    double adr[G], adi[G];
+    double adr[G][TPL], adi[G][TPL];
 #pragma unroll
-    for (int g = 0; g < G; ++g) { adr[g] = 0.0; adi[g] = 0.0; }
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) { adr[g][t] = 0.0; adi[g][t] = 0.0; }
 
     for (int gj = 0; gj < G; ++gj) {
         const int64_t J = b * G + gj;
         if (J >= T) break;
-        {
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) {
             double y[3], nv[3], qr, qi;
-            sym_node<DIM, CPLX>(A, J * kSymBS + tid, y, nv, qr, qi);
-            if constexpr (DIM == 3) {
-                s_src[0][tid] = make_double2(y[0], y[1]);
-                s_src[1][tid] = make_double2(y[2], nv[0]);
-                s_src[2][tid] = make_double2(nv[1], nv[2]);
-                s_src[3][tid] = make_double2(qr, qi);
-            } else {
-                s_src[0][tid] = make_double2(y[0], y[1]);
-                s_src[1][tid] = make_double2(nv[0], nv[1]);
-                s_src[2][tid] = make_double2(qr, qi);
-            }
+            sym_node<DIM, CPLX>(A, J * kSymBS + tid + t * BS, y, nv, qr, qi);
+            R::put(s_src, tid + t * BS, y, nv, qr, qi);
         }
         __syncthreads();
         const bool trans = !diag || gj > 0;  // some tile I < J feeds tile J
@@ -142,23 +176,28 @@ __global__ __launch_bounds__(kSymBS) void k_apply_sym(SymArgs A) {
             for (int gi = 0; gi < G; ++gi) {
                 const int64_t I = a * G + gi;
                 if (I >= T || (diag && gi > gj)) continue;
-                double xi[3], ni[3], qir, qii;
-                sym_node<DIM, CPLX>(A, I * kSymBS + tid, xi, ni, qir, qii);
+                double xi[TPL][3], ni[TPL][3], qir[TPL], qii[TPL];
+#pragma unroll
+                for (int t = 0; t < TPL; ++t)
+                    sym_node<DIM, CPLX>(A, I * kSymBS + tid + t * BS, xi[t], ni[t], qir[t], qii[t]);
                 if (diag && gi == gj) {
                     // the tile against itself: every ordered pair once
                     for (int s = 0; s < 64; ++s) {
                         const int jl = u * 64 + ((lane + s) & 63);
                         double yj[3], nj[3], qjr, qji;
-                        sym_read<DIM>(s_src, jl, yj, nj, qjr, qji);
-                        double kr, ki;
-                        pair_raw_xy<PDE, DIM, KIND, MIXED>(xi, ni, yj, nj, p, kr, ki);
-                        if constexpr (CPLX) {
-                            adr[gi] = __builtin_fma(kr, qjr, adr[gi]);
-                            adr[gi] = __builtin_fma(-ki, qji, adr[gi]);
-                            adi[gi] = __builtin_fma(kr, qji, adi[gi]);
-                            adi[gi] = __builtin_fma(ki, qjr, adi[gi]);
-                        } else {
-                            adr[gi] = __builtin_fma(kr, qjr, adr[gi]);
+                        R::get(s_src, jl, yj, nj, qjr, qji);
+#pragma unroll
+                        for (int t = 0; t < TPL; ++t) {
+                            double kr, ki;
+                            pair_raw_xy<PDE, DIM, KIND, MIXED>(xi[t], ni[t], yj, nj, p, kr, ki);
+                            if constexpr (CPLX) {
+                                adr[gi][t] = __builtin_fma(kr, qjr, adr[gi][t]);
+                                adr[gi][t] = __builtin_fma(-ki, qji, adr[gi][t]);
+                                adi[gi][t] = __builtin_fma(kr, qji, adi[gi][t]);
+                                adi[gi][t] = __builtin_fma(ki, qjr, adi[gi][t]);
+                            } else {
+                                adr[gi][t] = __builtin_fma(kr, qjr, adr[gi][t]);
+                            }
                         }
                     }
                 } else {
@@ -166,31 +205,30 @@ __global__ __launch_bounds__(kSymBS) void k_apply_sym(SymArgs A) {
                     for (int s = 0; s < 64; ++s) {
                         const int jl = u * 64 + ((lane + s) & 63);
                         double yj[3], nj[3], qjr, qji;
-                        sym_read<DIM>(s_src, jl, yj, nj, qjr, qji);
-                        double r[DIM], rn[DIM];
+                        R::get(s_src, jl, yj, nj, qjr, qji);
 #pragma unroll
-                        for (int d = 0; d < DIM; ++d) {
-                            r[d] = xi[d] - yj[d];
-                            rn[d] = -r[d];
-                        }
-                        double k1r, k1i, k2r, k2i;
-                        pair_raw<PDE, DIM, KIND, MIXED>(r, ni, nj, p, k1r, k1i);   // K(x_i, x_j)
-                        pair_raw<PDE, DIM, KIND, MIXED>(rn, nj, ni, p, k2r, k2i);  // K(x_j, x_i)
-                        if constexpr (CPLX) {
-                            adr[gi] = __builtin_fma(k1r, qjr, adr[gi]);
-                            adr[gi] = __builtin_fma(-k1i, qji, adr[gi]);
-                            adi[gi] = __builtin_fma(k1r, qji, adi[gi]);
-                            adi[gi] = __builtin_fma(k1i, qjr, adi[gi]);
-                            ajr = __builtin_fma(k2r, qir, ajr);
-                            ajr = __builtin_fma(-k2i, qii, ajr);
-                            aji = __builtin_fma(k2r, qii, aji);
-                            aji = __builtin_fma(k2i, qir, aji);
-                            aji = lane_next(aji, nxt);
-                        } else {
-                            adr[gi] = __builtin_fma(k1r, qjr, adr[gi]);
-                            ajr = __builtin_fma(k2r, qir, ajr);
+                        for (int t = 0; t < TPL; ++t) {
+                            double r[DIM];
+#pragma unroll
+                            for (int d = 0; d < DIM; ++d) r[d] = xi[t][d] - yj[d];
+                            double k1r, k1i, k2r, k2i;  // K(x_i, x_j), K(x_j, x_i)
+                            pair_raw_sym<PDE, DIM, KIND, MIXED>(r, ni[t], nj, p, k1r, k1i, k2r, k2i);
+                            if constexpr (CPLX) {
+                                adr[gi][t] = __builtin_fma(k1r, qjr, adr[gi][t]);
+                                adr[gi][t] = __builtin_fma(-k1i, qji, adr[gi][t]);
+                                adi[gi][t] = __builtin_fma(k1r, qji, adi[gi][t]);
+                                adi[gi][t] = __builtin_fma(k1i, qjr, adi[gi][t]);
+                                ajr = __builtin_fma(k2r, qir[t], ajr);
+                                ajr = __builtin_fma(-k2i, qii[t], ajr);
+                                aji = __builtin_fma(k2r, qii[t], aji);
+                                aji = __builtin_fma(k2i, qir[t], aji);
+                            } else {
+                                adr[gi][t] = __builtin_fma(k1r, qjr, adr[gi][t]);
+                                ajr = __builtin_fma(k2r, qir[t], ajr);
+                            }
                         }
                         ajr = lane_next(ajr, nxt);
+                        if constexpr (CPLX) aji = lane_next(aji, nxt);
                     }
                 }
             }
@@ -198,17 +236,17 @@ __global__ __launch_bounds__(kSymBS) void k_apply_sym(SymArgs A) {
         }
         if (trans) {
             __syncthreads();
-            const int64_t j = J * kSymBS + tid;
-            if (j < n) {
-                const double2 v0 = s_red[0][tid], v1 = s_red[1][tid], v2 = s_red[2][tid],
-                              v3 = s_red[3][tid];
-                const double vr = ((v0.x + v1.x) + v2.x) + v3.x;
-                const int64_t off = a * n + j;
-                if constexpr (CPLX) {
-                    const double vi = ((v0.y + v1.y) + v2.y) + v3.y;
-                    reinterpret_cast<double2 *>(A.part)[off] = make_double2(vr, vi);
-                } else {
-                    A.part[off] = vr;
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) {
+                const int jt = tid + t * BS;
+                const int64_t j = J * kSymBS + jt;
+                if (j < n) {
+                    double vr = s_red[0][jt].x, vi = s_red[0][jt].y;
+#pragma unroll
+                    for (int w = 1; w < NW; ++w) { vr += s_red[w][jt].x; vi += s_red[w][jt].y; }
+                    const int64_t off = a * n + j;
+                    if constexpr (CPLX) reinterpret_cast<double2 *>(A.part)[off] = make_double2(vr, vi);
+                    else A.part[off] = vr;
                 }
             }
         }
@@ -217,31 +255,39 @@ __global__ __launch_bounds__(kSymBS) void k_apply_sym(SymArgs A) {
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
         const int64_t I = a * G + gi;
-        const int64_t i = I * kSymBS + tid;
-        if (I >= T || i >= n) continue;
-        const int64_t off = (diag ? a : b) * n + i;
-        // in a diagonal block, tile gi > 0 already holds its transposed sum,
-        // written above by this same thread
-        const bool add = diag && gi > 0;
-        if constexpr (CPLX) {
-            double2 *pp = reinterpret_cast<double2 *>(A.part) + off;
-            double2 v = make_double2(adr[gi], adi[gi]);
-            if (add) { const double2 o = *pp; v.x += o.x; v.y += o.y; }
-            *pp = v;
-        } else {
-            double v = adr[gi];
-            if (add) v += A.part[off];
-            A.part[off] = v;
+        if (I >= T) continue;
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) {
+            const int64_t i = I * kSymBS + tid + t * BS;
+            if (i >= n) continue;
+            const int64_t off = (diag ? a : b) * n + i;
+            // in a diagonal block, tile gi > 0 already holds its transposed
+            // sum, written above by this same thread
+            const bool add = diag && gi > 0;
+            if constexpr (CPLX) {
+                double2 *pp = reinterpret_cast<double2 *>(A.part) + off;
+                double2 v = make_double2(adr[gi][t], adi[gi][t]);
+                if (add) { const double2 o = *pp; v.x += o.x; v.y += o.y; }
+                *pp = v;
+            } else {
+                double v = adr[gi][t];
+                if (add) v += A.part[off];
+                A.part[off] = v;
+            }
         }
     }
 }
 
 using SymFn = void (*)(const SymArgs &, int64_t, hipStream_t);
 
+// two targets per lane (one LDS read of a source serves two pairs), except
+// for the 2D Helmholtz kernels, whose Hankel evaluation already fills the
+// register file
 template <int PDE, int DIM, int KIND, bool MIXED, int G>
 static void sym_launcher(const SymArgs &a, int64_t nblocks, hipStream_t s) {
-    hipLaunchKernelGGL((k_apply_sym<PDE, DIM, KIND, MIXED, G>), dim3((unsigned)nblocks),
-                       dim3(kSymBS), 0, s, a);
+    constexpr int TPL = (PDE == INTI_HELMHOLTZ && DIM == 2) ? 1 : 2;
+    hipLaunchKernelGGL((k_apply_sym<PDE, DIM, KIND, MIXED, G, TPL>), dim3((unsigned)nblocks),
+                       dim3(kSymBS / TPL), 0, s, a);
 }
 
 template <int PDE, int DIM, bool MIXED, int G>

@@ -163,22 +163,33 @@ __device__ __forceinline__ void helm3_raw_mixed(const double *r, double d2, cons
 // K = (i/4) Z; raw chosen so that post = i/4 (SL), i k/4 (DL, HS),
 // -i k/4 (ADL), i c1 k/4 (COMB_SD with ch = c0/(c1 k), COMB_AH with
 // ch = c0/c1).
+// The radial part: 1/d, x = k d and H0(x), H1(x); shared by K(x_i, x_j) and
+// K(x_j, x_i).
+struct Helm2Radial {
+    double id, x, h0r, h0i, h1r, h1i;
+};
+template <bool MIXED>
+__device__ __forceinline__ Helm2Radial helm2_radial(double d2s, const KParams &p) {
+    Helm2Radial R;
+    R.id = rsqp<MIXED>(d2s);
+    R.x = p.k * (d2s * R.id);
+    hankel01<MIXED>(R.x, R.h0r, R.h0i, R.h1r, R.h1i);
+    return R;
+}
+
 template <int KIND, bool MIXED>
-__device__ __forceinline__ void helm2_raw(const double *r, double d2s, bool ok, const double *nx,
-                                          const double *ny, const KParams &p, double &kr,
-                                          double &ki) {
-    const double k = p.k;
-    double id = rsqp<MIXED>(d2s);
-    double x = k * (d2s * id);
-    double h0r, h0i, h1r, h1i;
-    hankel01<MIXED>(x, h0r, h0i, h1r, h1i);
+__device__ __forceinline__ void helm2_combine(const double *r, bool ok, const double *nx,
+                                              const double *ny, const KParams &p,
+                                              const Helm2Radial &R, double &kr, double &ki) {
+    const double k = p.k, id = R.id;
+    const double h0r = R.h0r, h0i = R.h0i, h1r = R.h1r, h1i = R.h1i;
     if constexpr (KIND == INTI_SL) {
         kr = mask(ok, h0r); ki = mask(ok, h0i);
     } else if constexpr (KIND == INTI_DL || KIND == INTI_ADL) {
         double f = mask(ok, id * dotv<2>(r, KIND == INTI_DL ? ny : nx));
         kr = f * h1r; ki = f * h1i;
     } else if constexpr (KIND == INTI_HS) {
-        double ix = rcpv<MIXED>(x);
+        double ix = rcpv<MIXED>(R.x);
         double h2r = 2.0 * ix * h1r - h0r, h2i = 2.0 * ix * h1i - h0i;
         double fa = mask(ok, -k * id * id * dotv<2>(r, nx) * dotv<2>(r, ny));
         double fb = mask(ok, id * dotv<2>(nx, ny));
@@ -189,7 +200,7 @@ __device__ __forceinline__ void helm2_raw(const double *r, double d2s, bool ok, 
         double zi = f * h1i + p.chr * h0i + p.chi * h0r;
         kr = mask(ok, zr); ki = mask(ok, zi);
     } else {  // HS raw - ch * rdnx id H1
-        double ix = rcpv<MIXED>(x);
+        double ix = rcpv<MIXED>(R.x);
         double rdnx = dotv<2>(r, nx);
         double h2r = 2.0 * ix * h1r - h0r, h2i = 2.0 * ix * h1i - h0i;
         double fa = -k * id * id * rdnx * dotv<2>(r, ny);
@@ -200,6 +211,13 @@ __device__ __forceinline__ void helm2_raw(const double *r, double d2s, bool ok, 
         double zi = fa * h2i + fb * h1i + p.chr * ai + p.chi * ar;
         kr = mask(ok, zr); ki = mask(ok, zi);
     }
+}
+
+template <int KIND, bool MIXED>
+__device__ __forceinline__ void helm2_raw(const double *r, double d2s, bool ok, const double *nx,
+                                          const double *ny, const KParams &p, double &kr,
+                                          double &ki) {
+    helm2_combine<KIND, MIXED>(r, ok, nx, ny, p, helm2_radial<MIXED>(d2s, p), kr, ki);
 }
 
 // ------------------------------------------------------ Helmholtz far field
@@ -230,7 +248,27 @@ __device__ __forceinline__ void pair_raw(const double *r, const double *nx, cons
     double d2 = dotv<DIM>(r, r);
     bool ok = d2 > kSamePointTol2;
     double d2s = fmax(d2, kClampD2);
-    if constexpr (PDE == INTI_LAPLACE) {
+    if constexpr (PDE == INTI_LAPLACE && DIM == 3 && MIXED) {
+        // 1/d from the FP32 unit, masked in FP32: zero for coincident points,
+        // and every kind below vanishes with it
+        const float y32 = __builtin_amdgcn_rsqf(fmaxf((float)d2, 1.0e-30f));
+        const double y = (double)(ok ? y32 : 0.0f);
+        if constexpr (KIND == INTI_SL) {
+            kr = y;
+        } else if constexpr (KIND == INTI_DL || KIND == INTI_ADL) {
+            kr = dotv<3>(r, KIND == INTI_DL ? ny : nx) * (y * y * y);
+        } else if constexpr (KIND == INTI_HS) {
+            const double y2 = y * y;
+            kr = (y2 * y) * (dotv<3>(nx, ny) - 3.0 * dotv<3>(r, nx) * dotv<3>(r, ny) * y2);
+        } else if constexpr (KIND == INTI_COMB_SD) {  // DL + ch*SL
+            kr = __builtin_fma(dotv<3>(r, ny), y * y * y, p.chr * y);
+        } else {  // HS + ch*ADL
+            const double y2 = y * y;
+            const double rdnx = dotv<3>(r, nx);
+            kr = (y2 * y) * (dotv<3>(nx, ny) - 3.0 * rdnx * dotv<3>(r, ny) * y2 + p.chr * rdnx);
+        }
+        ki = 0.0;
+    } else if constexpr (PDE == INTI_LAPLACE) {
         kr = laplace_raw<DIM, KIND, MIXED>(r, d2s, ok, nx, ny, p);
         ki = 0.0;
     } else if constexpr (DIM == 3 && MIXED &&
@@ -258,6 +296,30 @@ __device__ __forceinline__ void pair_raw_xy(const double *x, const double *nx, c
 #pragma unroll
         for (int d = 0; d < DIM; ++d) r[d] = x[d] - y[d];
         pair_raw<PDE, DIM, KIND, MIXED>(r, nx, ny, p, kr, ki);
+    }
+}
+
+// Both directions of one unordered pair of a quadrature with itself:
+// K(x_i, x_j) with target normal ni and source normal nj, and K(x_j, x_i);
+// r = x_i - x_j. The shared radial parts are computed once: explicitly for
+// the 2D Helmholtz kernels (the Hankel evaluation branches, so two calls
+// would not be merged), as common subexpressions of two calls otherwise.
+template <int PDE, int DIM, int KIND, bool MIXED>
+__device__ __forceinline__ void pair_raw_sym(const double *r, const double *ni, const double *nj,
+                                             const KParams &p, double &k1r, double &k1i,
+                                             double &k2r, double &k2i) {
+    double rn[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) rn[d] = -r[d];
+    if constexpr (PDE == INTI_HELMHOLTZ && DIM == 2) {
+        const double d2 = dotv<2>(r, r);
+        const bool ok = d2 > kSamePointTol2;
+        const Helm2Radial R = helm2_radial<MIXED>(fmax(d2, kClampD2), p);
+        helm2_combine<KIND, MIXED>(r, ok, ni, nj, p, R, k1r, k1i);
+        helm2_combine<KIND, MIXED>(rn, ok, nj, ni, p, R, k2r, k2i);
+    } else {
+        pair_raw<PDE, DIM, KIND, MIXED>(r, ni, nj, p, k1r, k1i);
+        pair_raw<PDE, DIM, KIND, MIXED>(rn, nj, ni, p, k2r, k2i);
     }
 }
 
