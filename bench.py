@@ -608,17 +608,26 @@ def run_shape(inti, lib, ctx, args):
     # adjoint of the N x N direct sum (plain combined field) vs its forward
     A = inti.IntegralOperator(inti.CombinedFieldKernel(op, -1j * k, 1.0), Q)
     ubar = np.conj(u0.mean()) * np.ones(n) + 0.0j
-    A.apply(sig)
-    A.pullback(ubar, sig, wrt=("sigma",))
-    ctx.synchronize()
-    A.apply(sig)
-    t_fwd = ctx.last_kernel_ms() * 1e-3
-    A.pullback(ubar, sig, wrt=("sigma", "source_nodes", "source_normals", "source_weights", "k"))
-    t_adj_src = ctx.last_kernel_ms() * 1e-3
-    A.pullback(ubar, sig, wrt=("sigma", "source_nodes", "source_normals", "source_weights", "target_nodes", "k"))
-    t_adj_all = ctx.last_kernel_ms() * 1e-3
-    log(f"bench shape: N x N forward {t_fwd * 1e3:.1f} ms, adjoint (source side) {t_adj_src * 1e3:.1f} ms, "
-        f"(both sides) {t_adj_all * 1e3:.1f} ms")
+    nxn = {}
+    for label, mode in (("symmetric", 1), ("plain", 0)):
+        # symmetric: one pass over unordered pairs (forward and pullback);
+        # plain: one direction per pair, source-major + target-major passes
+        A.set_symmetric(mode)
+        A.apply(sig)
+        A.pullback(ubar, sig, wrt=("sigma",))
+        ctx.synchronize()
+        A.apply(sig)
+        t_fwd = ctx.last_kernel_ms() * 1e-3
+        A.pullback(ubar, sig, wrt=("sigma", "source_nodes", "source_normals", "source_weights", "k"))
+        t_adj_src = ctx.last_kernel_ms() * 1e-3
+        A.pullback(ubar, sig, wrt=("sigma", "source_nodes", "source_normals", "source_weights", "target_nodes", "k"))
+        t_adj_all = ctx.last_kernel_ms() * 1e-3
+        log(f"bench shape: N x N {label}: forward {t_fwd * 1e3:.1f} ms, adjoint (source side) "
+            f"{t_adj_src * 1e3:.1f} ms, (both sides) {t_adj_all * 1e3:.1f} ms")
+        nxn[label] = {"forward": round(t_fwd * 1e3, 2), "adjoint_source_side": round(t_adj_src * 1e3, 2),
+                      "adjoint_both_sides": round(t_adj_all * 1e3, 2),
+                      "ratio_source_side": round(t_adj_src / t_fwd, 3),
+                      "ratio_both_sides": round(t_adj_all / t_fwd, 3)}
     return {
         "metric": "adjoint / forward time of the far-field objective's pullback (shape gradient, k=10, ~1e5 nodes)",
         "value": round(k_adj_ff / k_fwd_ff, 3), "unit": "ratio", "n_gpus": 1, "steps": reps, "warmup": 1,
@@ -631,10 +640,7 @@ def run_shape(inti, lib, ctx, args):
         "objective": J0, "fd_checks": {"directions": 20, "h": 1e-6, "worst_rel_err": worst},
         "farfield_kernel_ms": {"forward": round(k_fwd_ff, 4), "adjoint": round(k_adj_ff, 4)},
         "farfield_call_ms": {"forward": round(t_fwd_ff * 1e3, 3), "adjoint": round(t_adj_ff * 1e3, 3)},
-        "direct_sum_NxN_kernel_ms": {"forward": round(t_fwd * 1e3, 2), "adjoint_source_side": round(t_adj_src * 1e3, 2),
-                              "adjoint_both_sides": round(t_adj_all * 1e3, 2),
-                              "ratio_source_side": round(t_adj_src / t_fwd, 3),
-                              "ratio_both_sides": round(t_adj_all / t_fwd, 3)},
+        "direct_sum_NxN_kernel_ms": nxn,
     }
 
 

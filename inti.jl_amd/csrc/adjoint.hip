@@ -29,6 +29,7 @@
 #include <complex>
 #include <vector>
 
+#include "apply_sym.hpp"
 #include "common.hpp"
 #include "pair.hpp"
 
@@ -111,6 +112,58 @@ __device__ __forceinline__ void radial(double d2, const AdjParams &p, double2 &g
     }
 }
 
+// The radial functions of one pair distance (shared by K(x_i, x_j) and
+// K(x_j, x_i)), with the coincident-point flag.
+// For c0 SL + c1 DL the coefficient products are formed here once per
+// distance: g <- c0 g, gpd <- c0 gpd, gk <- dc0 g + c0 gk, h <- c1 h,
+// hpd <- c1 hpd, hk <- dc1 h + c1 hk.
+struct AdjRadial {
+    double2 g, gpd, gk, h, hpd, hk;
+    bool ok;
+};
+template <int PDE, int DIM, int KIND>
+__device__ __forceinline__ AdjRadial adj_radial(double d2, const AdjParams &p) {
+    constexpr bool WG = KIND != INTI_DL, WH = KIND != INTI_SL;
+    AdjRadial R;
+    R.ok = d2 > kSamePointTol2;
+    R.g = R.gpd = R.gk = R.h = R.hpd = R.hk = c2(0, 0);
+    radial<PDE, DIM, WG, WH>(fmax(d2, kClampD2), p, R.g, R.gpd, R.gk, R.h, R.hpd, R.hk);
+    if constexpr (KIND == INTI_COMB_SD) {
+        const double2 g = R.g, h = R.h;
+        R.gk = ca(cm(p.dc0, g), cm(p.c0, R.gk));
+        R.g = cm(p.c0, g);
+        R.gpd = cm(p.c0, R.gpd);
+        R.hk = ca(cm(p.dc1, h), cm(p.c1, R.hk));
+        R.h = cm(p.c1, h);
+        R.hpd = cm(p.c1, R.hpd);
+    }
+    return R;
+}
+
+// K, dK/dk and the factored derivatives (below) of the translation-invariant
+// kinds for v = x - y and the source normal ny, from the radial functions.
+// SL: K = g, dK/dy = -gpd r;  DL: K = h rn, dK/dy = -(hpd rn r + h n),
+// dK/dn = h r (rn = r . n_y); coincident points give zeros.
+template <int PDE, int DIM, int KIND>
+__device__ __forceinline__ void adj_combine(const double *v, const double *ny, const AdjRadial &R,
+                                            const AdjParams &p, double2 &K, double2 &gk,
+                                            double2 &a_r, double2 &a_n, double2 &b_r) {
+    const double rn = KIND != INTI_SL ? dotv<DIM>(v, ny) : 0.0;
+    if constexpr (KIND == INTI_SL) {
+        K = R.g; gk = R.gk; a_r = cs(R.gpd, -1.0); a_n = c2(0, 0); b_r = c2(0, 0);
+    } else if constexpr (KIND == INTI_DL) {
+        K = cs(R.h, rn); gk = cs(R.hk, rn); a_r = cs(R.hpd, -rn); a_n = cs(R.h, -1.0); b_r = R.h;
+    } else {  // c0 SL + c1 DL, coefficients folded into R
+        K = make_double2(__builtin_fma(R.h.x, rn, R.g.x), __builtin_fma(R.h.y, rn, R.g.y));
+        gk = make_double2(__builtin_fma(R.hk.x, rn, R.gk.x), __builtin_fma(R.hk.y, rn, R.gk.y));
+        a_r = make_double2(-__builtin_fma(R.hpd.x, rn, R.gpd.x), -__builtin_fma(R.hpd.y, rn, R.gpd.y));
+        b_r = R.h;
+        a_n = cs(b_r, -1.0);
+    }
+    const double mk = R.ok ? 1.0 : 0.0;
+    K = cs(K, mk); gk = cs(gk, mk); a_r = cs(a_r, mk); a_n = cs(a_n, mk); b_r = cs(b_r, mk);
+}
+
 // Value and derivatives of one pair, x the target (or far-field direction),
 // in the factored form
 //   dK/dy = a_r v + a_n n_y,   dK/dn_y = b_r v,
@@ -153,28 +206,8 @@ __device__ __forceinline__ void pair_grad(const double *x, const double *y, cons
     } else {
 #pragma unroll
         for (int d = 0; d < DIM; ++d) v[d] = x[d] - y[d];
-        const double d2 = dotv<DIM>(v, v);
-        const bool ok = d2 > kSamePointTol2;
-        const double d2s = fmax(d2, kClampD2);
-        constexpr bool WG = KIND != INTI_DL, WH = KIND != INTI_SL;
-        double2 g = c2(0, 0), gpd = c2(0, 0), gkg = c2(0, 0), h = c2(0, 0), hpd = c2(0, 0), hkk = c2(0, 0);
-        radial<PDE, DIM, WG, WH>(d2s, p, g, gpd, gkg, h, hpd, hkk);
-        const double rn = WH ? dotv<DIM>(v, ny) : 0.0;
-        // SL: K = g, dK/dy = -gpd r;  DL: K = h rn, dK/dy = -(hpd rn r + h n), dK/dn = h r
-        if constexpr (KIND == INTI_SL) {
-            K = g; gk = gkg; a_r = cs(gpd, -1.0); a_n = c2(0, 0); b_r = c2(0, 0);
-        } else if constexpr (KIND == INTI_DL) {
-            K = cs(h, rn); gk = cs(hkk, rn); a_r = cs(hpd, -rn); a_n = cs(h, -1.0); b_r = h;
-        } else {  // c0 SL + c1 DL
-            const double2 KD = cs(h, rn);
-            K = ca(cm(p.c0, g), cm(p.c1, KD));
-            gk = ca(ca(cm(p.dc0, g), cm(p.c0, gkg)), ca(cm(p.dc1, KD), cm(p.c1, cs(hkk, rn))));
-            a_r = ca(cm(p.c0, cs(gpd, -1.0)), cm(p.c1, cs(hpd, -rn)));
-            b_r = cm(p.c1, h);
-            a_n = cs(b_r, -1.0);
-        }
-        const double mk = ok ? 1.0 : 0.0;
-        K = cs(K, mk); gk = cs(gk, mk); a_r = cs(a_r, mk); a_n = cs(a_n, mk); b_r = cs(b_r, mk);
+        const AdjRadial R = adj_radial<PDE, DIM, KIND>(dotv<DIM>(v, v), p);
+        adj_combine<PDE, DIM, KIND>(v, ny, R, p, K, gk, a_r, a_n, b_r);
     }
 }
 
@@ -309,6 +342,7 @@ struct AdjOut {
     int64_t m, n;
     int split, dim, cplx;
     const double2 *part, *c, *sig;
+    const double *partr;  // symmetric pass: S x NR x n real sums
     const double *sw, *sn;
     double *sigma_bar, *ybar, *nbar, *wbar, *kpart, *xbar;
 };
@@ -378,6 +412,194 @@ __global__ __launch_bounds__(256) void k_adj_trg_reduce(AdjOut o) {
     for (int d = 0; d < DIM; ++d) o.xbar[d * o.m + i] = cc.x * S[d].x - cc.y * S[d].y;
 }
 
+// ------------------------------------------------ symmetric pullback
+// For an operator whose targets are its sources (one quadrature), both
+// passes above in one: each unordered pair {i, j} is visited once, its
+// radial functions computed once, and the two ordered pairs (i target, j
+// source) and (j target, i source) are differentiated from them. Work and
+// lane rotation as in the symmetric forward sum (tiles of 256, block (a, b),
+// a <= b; lane l pairs its node with source (l + s) mod 64 of a sub-tile and
+// passes the source's running sums to lane l - 1). Per node, NR = 3 DIM + 2
+// real sums:
+//   [S0 (2): sum_t c_t K_t,node    Y (DIM): ybar    X (DIM): xbar    N (DIM): nbar]
+// where, for a pair (target t, source s) with v = x_t - x_s and
+// sc = s_s c_t: alpha = Re(sc a_r), beta = Re(sc a_n), gamma = Re(sc b_r),
+// g = alpha v + beta n_s; then Y_s += g, X_t -= g, N_s += gamma v, and
+// kbar += Re(sc gk). The waves' sums for a sub-tile are added in LDS in wave
+// order; one partial slot per (node, tile) as in the forward sum;
+// deterministic.
+struct AdjSymArgs {
+    int64_t n;
+    const double *xy, *nrm, *w;
+    const double2 *c, *sig;
+    double *part;   // S x NR x n
+    double *kpart;  // one per block
+    int64_t ntiles;
+    AdjParams p;
+};
+
+template <int DIM>
+__device__ __forceinline__ void adj_sym_node(const AdjSymArgs &A, int64_t j, double *x, double *nv,
+                                             double2 &c, double2 &sw) {
+    if (j < A.n) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            x[d] = A.xy[d * A.n + j];
+            nv[d] = A.nrm ? A.nrm[d * A.n + j] : 0.0;
+        }
+        c = A.c[j];
+        sw = cs(A.sig[j], A.w[j]);
+    } else {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) { x[d] = kPadCoord; nv[d] = 0.0; }
+        c = sw = c2(0, 0);
+    }
+}
+
+// one ordered pair (target t, source s): adds to the source's sums S0, Y, N
+// and the target's X; returns Re(sc gk)
+template <int PDE, int DIM, int KIND>
+__device__ __forceinline__ double adj_sym_pair(const double *v, const double *ns, const AdjRadial &R,
+                                               const AdjParams &p, double2 ct, double2 ss,
+                                               double *src, double *trg) {
+    double2 K, gk, a_r, a_n, b_r;
+    adj_combine<PDE, DIM, KIND>(v, ns, R, p, K, gk, a_r, a_n, b_r);
+    const double2 sc = cm(ss, ct);
+    src[0] = __builtin_fma(ct.x, K.x, __builtin_fma(-ct.y, K.y, src[0]));
+    src[1] = __builtin_fma(ct.x, K.y, __builtin_fma(ct.y, K.x, src[1]));
+    const double al = sc.x * a_r.x - sc.y * a_r.y;
+    const double be = sc.x * a_n.x - sc.y * a_n.y;
+    const double ga = sc.x * b_r.x - sc.y * b_r.y;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const double gd = __builtin_fma(al, v[d], be * ns[d]);
+        src[2 + d] += gd;
+        trg[2 + DIM + d] -= gd;
+        src[2 + 2 * DIM + d] = __builtin_fma(ga, v[d], src[2 + 2 * DIM + d]);
+    }
+    return sc.x * gk.x - sc.y * gk.y;
+}
+
+template <int PDE, int DIM, int KIND>
+__global__ __launch_bounds__(kSymBS) void k_adj_sym(AdjSymArgs A) {
+    constexpr int NR = 3 * DIM + 2;
+    __shared__ double s_x[DIM][kSymBS], s_n[DIM][kSymBS];
+    __shared__ double2 s_c[kSymBS], s_s[kSymBS];
+    __shared__ double s_red[NR][kSymBS];
+    __shared__ double s_k[kSymBS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nxt = ((lane + 1) & 63) << 2;
+    int64_t a, b;
+    sym_item(blockIdx.x, A.ntiles, a, b);
+    const bool diag = a == b;
+    const int64_t n = A.n;
+    const AdjParams &p = A.p;
+    double xi[DIM], ni[DIM];
+    double2 ci, si;
+    adj_sym_node<DIM>(A, a * kSymBS + tid, xi, ni, ci, si);
+    {
+        double xj[DIM], nj[DIM];
+        double2 cj, sj;
+        adj_sym_node<DIM>(A, b * kSymBS + tid, xj, nj, cj, sj);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) { s_x[d][tid] = xj[d]; s_n[d][tid] = nj[d]; }
+        s_c[tid] = cj;
+        s_s[tid] = sj;
+    }
+    __syncthreads();
+    double own[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) own[r] = 0.0;
+    double kb = 0.0;
+    for (int u = 0; u < 4; ++u) {
+        double rot[NR];  // sums of source (lane + s) mod 64 of sub-tile u
+#pragma unroll
+        for (int r = 0; r < NR; ++r) rot[r] = 0.0;
+        for (int s = 0; s < 64; ++s) {
+            const int jl = u * 64 + ((lane + s) & 63);
+            double xj[DIM], nj[DIM], v[DIM], vn[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                xj[d] = s_x[d][jl];
+                nj[d] = s_n[d][jl];
+                v[d] = xi[d] - xj[d];
+                vn[d] = -v[d];
+            }
+            const double2 cj = s_c[jl], sj = s_s[jl];
+            const AdjRadial R = adj_radial<PDE, DIM, KIND>(dotv<DIM>(v, v), p);
+            kb += adj_sym_pair<PDE, DIM, KIND>(v, nj, R, p, ci, sj, rot, own);  // i target, j source
+            if (!diag) kb += adj_sym_pair<PDE, DIM, KIND>(vn, ni, R, p, cj, si, own, rot);  // j target, i source
+#pragma unroll
+            for (int r = 0; r < NR; ++r) rot[r] = lane_next(rot[r], nxt);
+        }
+        // add the waves' sums for sub-tile u in wave order
+        for (int w = 0; w < kSymBS / 64; ++w) {
+            if (wave == w) {
+#pragma unroll
+                for (int r = 0; r < NR; ++r)
+                    s_red[r][u * 64 + lane] = (w == 0 ? 0.0 : s_red[r][u * 64 + lane]) + rot[r];
+            }
+            __syncthreads();
+        }
+    }
+    // source tile b's sums go to slot a; target tile a's to slot b (in a
+    // diagonal block both are the same rows: this thread adds its own)
+    const int64_t j = b * kSymBS + tid;
+    if (j < n) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) A.part[(a * NR + r) * n + j] = s_red[r][tid];
+    }
+    const int64_t i = a * kSymBS + tid;
+    if (i < n) {
+        const int64_t slot = diag ? a : b;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            double v = own[r];
+            if (diag) v += A.part[(slot * NR + r) * n + i];
+            A.part[(slot * NR + r) * n + i] = v;
+        }
+    }
+    s_k[tid] = kb;
+    __syncthreads();
+    for (int off = kSymBS / 2; off > 0; off >>= 1) {
+        if (tid < off) s_k[tid] += s_k[tid + off];
+        __syncthreads();
+    }
+    if (tid == 0) A.kpart[blockIdx.x] = s_k[0];
+}
+
+// per node: sum the S slots and form the outputs
+template <int DIM>
+__global__ __launch_bounds__(256) void k_adj_sym_reduce(AdjOut o, int64_t S) {
+    constexpr int NR = 3 * DIM + 2;
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= o.n) return;
+    double R[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) R[r] = 0.0;
+    for (int64_t c = 0; c < S; ++c) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) R[r] += o.partr[(c * NR + r) * o.n + j];
+    }
+    const double w = o.sw[j];
+    const double2 sg = o.sig[j];
+    if (o.sigma_bar) {
+        if (o.cplx) {
+            o.sigma_bar[2 * j] = w * R[0];
+            o.sigma_bar[2 * j + 1] = -w * R[1];
+        } else {
+            o.sigma_bar[j] = w * R[0];
+        }
+    }
+    if (o.wbar) o.wbar[j] = sg.x * R[0] - sg.y * R[1];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        if (o.ybar) o.ybar[d * o.n + j] = R[2 + d];
+        if (o.xbar) o.xbar[d * o.n + j] = R[2 + DIM + d];
+        if (o.nbar) o.nbar[d * o.n + j] = R[2 + 2 * DIM + d];
+    }
+}
+
 // sigma_bar += corr^H ubar + conj(identity) ubar: one thread per row i
 // scatters conj(val_iq) ubar_i to column colind_q (atomics: rows share
 // columns only across elements, so the collisions are few)
@@ -428,6 +650,27 @@ void pick(int kind, SrcFn &fs, SrcFn &ft) {
         }
     }
 }
+
+using SymAdjFn = void (*)(const AdjSymArgs &, int64_t, hipStream_t);
+
+template <int PDE, int DIM, int KIND>
+void adj_sym_launch(const AdjSymArgs &a, int64_t nblocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_adj_sym<PDE, DIM, KIND>), dim3((unsigned)nblocks), dim3(kSymBS), 0, s, a);
+}
+
+template <int PDE, int DIM>
+SymAdjFn pick_sym(int kind) {
+    switch (kind) {
+        case INTI_SL: return adj_sym_launch<PDE, DIM, INTI_SL>;
+        case INTI_DL: return adj_sym_launch<PDE, DIM, INTI_DL>;
+        case INTI_COMB_SD: return adj_sym_launch<PDE, DIM, INTI_COMB_SD>;
+    }
+    return nullptr;
+}
+
+// partial slots of the symmetric pullback are capped: beyond this the two
+// passes run
+constexpr size_t kAdjSymMaxBytes = size_t(4) << 30;
 
 struct Dev {
     void *p = nullptr;
@@ -513,15 +756,29 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
     for (int64_t j = 0; j < n; ++j) hs[j] = cplx ? make_double2(sigma[2 * j], sigma[2 * j + 1]) : make_double2(sigma[j], 0.0);
 
     hipStream_t s = ctx->stream;
+    // symmetric pullback: an operator on one quadrature, every output in one
+    // pass over the unordered pairs
+    const int NRs = 3 * dim + 2;
+    const int64_t symT = (n + kSymBS - 1) / kSymBS;
+    SymAdjFn fsym = nullptr;
+    if (op->sym && op->trg == op->src && !ff && symT >= 2 &&
+        sizeof(double) * (size_t)symT * NRs * (size_t)n <= kAdjSymMaxBytes) {
+        if (op->pde == INTI_LAPLACE && dim == 2) fsym = pick_sym<INTI_LAPLACE, 2>(kind);
+        if (op->pde == INTI_LAPLACE && dim == 3) fsym = pick_sym<INTI_LAPLACE, 3>(kind);
+        if (op->pde == INTI_HELMHOLTZ && dim == 2) fsym = pick_sym<INTI_HELMHOLTZ, 2>(kind);
+        if (op->pde == INTI_HELMHOLTZ && dim == 3) fsym = pick_sym<INTI_HELMHOLTZ, 3>(kind);
+    }
+    const int64_t symB = symT * (symT + 1) / 2;
     int split_s, split_t = 0;
     int64_t chunk_s, chunk_t = 0;
     plan(n, m, ctx->num_cu, split_s, chunk_s);
     if (xbar) plan(m, n, ctx->num_cu, split_t, chunk_t);
     const int NVs = 3 + 2 * dim;
-    const int64_t nsb = (n + 255) / 256;
+    const int64_t nsb = fsym ? symB : (n + 255) / 256;  // kbar partial sums
     Dev dc, dsg, dpart, dsb, dyb, dnb, dwb, dkp, dxb;
-    const size_t part_bytes = std::max(sizeof(double2) * (size_t)split_s * n * NVs,
-                                       xbar ? sizeof(double2) * (size_t)split_t * m * dim : 0);
+    const size_t part_bytes = fsym ? sizeof(double) * (size_t)symT * NRs * (size_t)n
+                                   : std::max(sizeof(double2) * (size_t)split_s * n * NVs,
+                                              xbar ? sizeof(double2) * (size_t)split_t * m * dim : 0);
     if (!dc.alloc(s, sizeof(double2) * m) || !dsg.alloc(s, sizeof(double2) * n) || !dpart.alloc(s, part_bytes) ||
         !dsb.alloc(s, sizeof(double) * (cplx ? 2 : 1) * n) || !dyb.alloc(s, sizeof(double) * dim * n) ||
         !dnb.alloc(s, sizeof(double) * dim * n) || !dwb.alloc(s, sizeof(double) * n) ||
@@ -543,18 +800,38 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
     o.sn = op->src->normals;
     o.sigma_bar = (double *)dsb.p; o.ybar = (double *)dyb.p; o.nbar = (double *)dnb.p;
     o.wbar = (double *)dwb.p; o.kpart = (double *)dkp.p; o.xbar = (double *)dxb.p;
+    o.partr = (const double *)dpart.p;
 
+    if (fsym) {
+        AdjSymArgs sa;
+        sa.n = n;
+        sa.xy = op->src->coords; sa.nrm = op->src->normals; sa.w = op->src->weights;
+        sa.c = (const double2 *)dc.p; sa.sig = (const double2 *)dsg.p;
+        sa.part = (double *)dpart.p; sa.kpart = (double *)dkp.p;
+        sa.ntiles = symT;
+        sa.p = p;
+        if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev0, s));
+        fsym(sa, symB, s);
+        INTI_HIP(ctx, hipGetLastError());
+        if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev1, s));
+        const unsigned nb = (unsigned)((n + 255) / 256);
+        if (dim == 2) hipLaunchKernelGGL(k_adj_sym_reduce<2>, dim3(nb), dim3(256), 0, s, o, symT);
+        else hipLaunchKernelGGL(k_adj_sym_reduce<3>, dim3(nb), dim3(256), 0, s, o, symT);
+        INTI_HIP(ctx, hipGetLastError());
+    }
     // source-major pass
     a.split = split_s; a.chunk = chunk_s;
-    if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev0, s));
-    fs(a, dim3((unsigned)(((n + kAdjBS - 1) / kAdjBS) * split_s)), s);
+    if (!fsym && ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev0, s));
+    if (!fsym) fs(a, dim3((unsigned)(((n + kAdjBS - 1) / kAdjBS) * split_s)), s);
     INTI_HIP(ctx, hipGetLastError());
     o.split = split_s;
-    if (dim == 2) hipLaunchKernelGGL(k_adj_src_reduce<2>, dim3((unsigned)nsb), dim3(256), 0, s, o);
-    else hipLaunchKernelGGL(k_adj_src_reduce<3>, dim3((unsigned)nsb), dim3(256), 0, s, o);
-    INTI_HIP(ctx, hipGetLastError());
+    if (!fsym) {
+        if (dim == 2) hipLaunchKernelGGL(k_adj_src_reduce<2>, dim3((unsigned)nsb), dim3(256), 0, s, o);
+        else hipLaunchKernelGGL(k_adj_src_reduce<3>, dim3((unsigned)nsb), dim3(256), 0, s, o);
+        INTI_HIP(ctx, hipGetLastError());
+    }
     // target-major pass (reuses the partial buffer)
-    if (xbar) {
+    if (xbar && !fsym) {
         a.split = split_t; a.chunk = chunk_t;
         ft(a, dim3((unsigned)(((m + kAdjBS - 1) / kAdjBS) * split_t)), s);
         INTI_HIP(ctx, hipGetLastError());
@@ -564,7 +841,7 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
         else hipLaunchKernelGGL(k_adj_trg_reduce<3>, dim3(mb), dim3(256), 0, s, o);
         INTI_HIP(ctx, hipGetLastError());
     }
-    if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev1, s));
+    if (!fsym && ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev1, s));
     // corrected operator: sigma_bar also gets corr^H ubar and conj(identity) ubar
     if (sigma_bar && op->corr) {
         const Csr *cr = op->corr;

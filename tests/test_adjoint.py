@@ -262,3 +262,47 @@ def test_gpu_pullback_error_paths(inti):
         M.pullback(u, u)
     with pytest.raises(ValueError):
         inti.IntegralOperator(inti.SingleLayerKernel(op), Q).pullback(u, u, wrt=("nodes",))
+
+
+SYM_CASES = [(LAP, 2, SL), (LAP, 2, DL), (LAP, 3, SL), (LAP, 3, DL), (LAP, 3, COMB),
+             (HEL, 2, SL), (HEL, 2, DL), (HEL, 2, COMB), (HEL, 3, SL), (HEL, 3, DL), (HEL, 3, COMB)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pde,dim,kind", SYM_CASES)
+@pytest.mark.parametrize("n", [700, 1300])
+def test_gpu_symmetric_pullback(inti, oracle, pde, dim, kind, n):
+    """An operator on one quadrature (target is source): the symmetric
+    pullback (one pass over unordered pairs) against the oracle (1e-12 per
+    output, k gradient 1e-11) and against the two-pass pullback (1e-13),
+    bit-identical on repeat; node counts not a multiple of the tile."""
+    print(f"symmetric pullback pde={pde} dim={dim} kind={kind} n={n}", flush=True)
+    k = 2.3 if pde == HEL else 0.0
+    _, y, nv, w, _, s = _setup(dim, 1, n, 11 + kind + n)
+    rng = np.random.default_rng(n + kind)
+    u = rng.normal(size=n) + 1j * rng.normal(size=n)
+    if pde == LAP:
+        u, s = u.real, s.real
+    op, K, dco = _kernel(inti, pde, dim, kind, k)
+    Q = _quad(inti, y, nv, w)
+    wrt = ("sigma", "source_nodes", "source_normals", "source_weights", "target_nodes") + \
+        (("k",) if pde == HEL else ())
+    A = inti.IntegralOperator(K, Q)
+    g = A.pullback(u, s, wrt=wrt, dcoefs=dco)
+    g2 = A.pullback(u, s, wrt=wrt, dcoefs=dco)
+    B = inti.IntegralOperator(K, Q)
+    B.set_symmetric(0)
+    h = B.pullback(u, s, wrt=wrt, dcoefs=dco)
+    co, _ = _coefs(pde, kind, k)
+    o = oracle.adjoint(pde, dim, kind, k, Pts(y), Pts(y, nv, w), u, s, coefs=co, dcoefs=dco, want_xbar=True)
+    pairs = {"sigma": "sigma_bar", "source_nodes": "ybar", "source_normals": "nbar",
+             "source_weights": "wbar", "target_nodes": "xbar"}
+    for a, b in pairs.items():
+        ref = o[b] if a != "sigma" or pde == HEL else o[b].real
+        nr = np.linalg.norm(ref)
+        assert np.linalg.norm(g[a] - ref) <= 1e-12 * nr, a
+        assert np.linalg.norm(g[a] - h[a]) <= 1e-13 * nr, a
+        assert np.array_equal(g[a], g2[a]), a
+    if pde == HEL:
+        assert abs(g["k"] - o["kbar"]) <= 1e-11 * abs(o["kbar"])
+        assert abs(g["k"] - h["k"]) <= 1e-12 * abs(o["kbar"])
