@@ -15,7 +15,8 @@
 // 64 travels with the step: after each step lane l takes lane l+1's running
 // sum (ds_bpermute, a crossbar move with no LDS storage), and after 64 steps
 // lane l holds the sum for source l over the wave's targets. The waves' sums
-// are added in LDS in a fixed order.
+// are added in LDS: two waves onto zero by atomic adds (the sum of two terms
+// does not depend on their order), more waves in wave order.
 //
 // Partial sums. A row of group c receives exactly one partial per group:
 // from block (c, b), b >= c (direct part) in slot b, and from block (a, c),
@@ -144,8 +145,12 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
     constexpr bool CPLX = R::CPLX;
     constexpr int BS = kSymBS / TPL;  // threads
     constexpr int NW = BS / 64;       // waves
+    // two waves add their transposed sums onto zero with LDS atomics (two
+    // addends: the result does not depend on their order); more waves write
+    // their own rows, added in wave order
+    constexpr bool RED_ADD = NW == 2;
     __shared__ double2 s_src[R::NS][kSymBS];
-    __shared__ double2 s_red[NW][kSymBS];
+    __shared__ double2 s_red[RED_ADD ? 1 : NW][kSymBS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nxt = ((lane + 1) & 63) << 2;  // ds_bpermute address of lane l+1
     int64_t a, b;
@@ -167,6 +172,7 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
             double y[3], nv[3], qr, qi;
             sym_node<DIM, CPLX>(A, J * kSymBS + tid + t * BS, y, nv, qr, qi);
             R::put(s_src, tid + t * BS, y, nv, qr, qi);
+            if constexpr (RED_ADD) s_red[0][tid + t * BS] = make_double2(0.0, 0.0);
         }
         __syncthreads();
         const bool trans = !diag || gj > 0;  // some tile I < J feeds tile J
@@ -232,7 +238,16 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                     }
                 }
             }
-            if (trans) s_red[wave][u * 64 + lane] = make_double2(ajr, aji);
+            if (trans) {
+                if constexpr (RED_ADD) {
+                    double *sr = &s_red[0][u * 64 + lane].x;
+                    __hip_atomic_fetch_add(sr, ajr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if constexpr (CPLX)
+                        __hip_atomic_fetch_add(sr + 1, aji, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {
+                    s_red[wave][u * 64 + lane] = make_double2(ajr, aji);
+                }
+            }
         }
         if (trans) {
             __syncthreads();
@@ -242,8 +257,10 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                 const int64_t j = J * kSymBS + jt;
                 if (j < n) {
                     double vr = s_red[0][jt].x, vi = s_red[0][jt].y;
+                    if constexpr (!RED_ADD) {
 #pragma unroll
-                    for (int w = 1; w < NW; ++w) { vr += s_red[w][jt].x; vi += s_red[w][jt].y; }
+                        for (int w = 1; w < NW; ++w) { vr += s_red[w][jt].x; vi += s_red[w][jt].y; }
+                    }
                     const int64_t off = a * n + j;
                     if constexpr (CPLX) reinterpret_cast<double2 *>(A.part)[off] = make_double2(vr, vi);
                     else A.part[off] = vr;
