@@ -65,10 +65,42 @@ __device__ __forceinline__ double logv(double x) {
     else return log(x);
 }
 
-// (sin 2 pi u, cos 2 pi u) for a phase u given in revolutions. FP64: the
-// sinpi/cospi pair, whose argument reduction is exact. MIXED: u is reduced to
-// [-1/2, 1/2] in FP64 and fed to the FP32 sin/cos unit, whose argument is in
-// revolutions.
+// Taylor coefficients of sin(2 pi f) / f and cos(2 pi f) in z = f^2:
+// (-1)^k (2 pi)^(2k+1) / (2k+1)! and (-1)^k (2 pi)^(2k) / (2k)!, rounded to
+// double from 50-digit values. For |f| <= 1/8 (|2 pi f| <= pi/4) the first
+// omitted terms are 4.6e-17 and 2.0e-18.
+constexpr double kSinRev[8] = {0x1.921fb54442d18p+2,  -0x1.4abbce625be53p+5, 0x1.466bc6775aae2p+6,
+                               -0x1.32d2cce62bd86p+6, 0x1.50783487ee782p+5,  -0x1.e3074fde8871fp+3,
+                               0x1.e8f434d018d63p+1,  -0x1.6fadb9f155744p-1};
+constexpr double kCosRev[9] = {0x1.0000000000000p+0,  -0x1.3bd3cc9be45dep+4, 0x1.03c1f081b5ac4p+6,
+                               -0x1.55d3c7e3cbffap+6, 0x1.e1f506891babbp+5,  -0x1.a6d1f2a204a8cp+4,
+                               0x1.f9d38a3763cc3p+2,  -0x1.b6e24f44b128fp+0, 0x1.20c62c2f2d7f5p-2};
+
+// (sin 2 pi u, cos 2 pi u) in FP64 for a phase u in revolutions, |u| < 2^29:
+// u = q/4 + f with q = rint(4u) and |f| <= 1/8 (f is exact), the two
+// polynomials above in f, then the quarter turn q mod 4.
+__device__ __forceinline__ void sincos_rev64(double u, double &s, double &c) {
+    const double q = __builtin_rint(4.0 * u);
+    const double f = __builtin_fma(-0.25, q, u);
+    const double z = f * f;
+    double ps = kSinRev[7];
+#pragma unroll
+    for (int k = 6; k >= 0; --k) ps = __builtin_fma(ps, z, kSinRev[k]);
+    double pc = kCosRev[8];
+#pragma unroll
+    for (int k = 7; k >= 0; --k) pc = __builtin_fma(pc, z, kCosRev[k]);
+    const double S = f * ps, C = pc;
+    const int qi = (int)q & 3;
+    // quarter turns: (S, C) -> (C, -S) -> (-S, -C) -> (-C, S)
+    const double s0 = (qi & 1) ? C : S;
+    const double c0 = (qi & 1) ? S : C;
+    s = (qi & 2) ? -s0 : s0;
+    c = ((qi ^ (qi >> 1)) & 1) ? -c0 : c0;
+}
+
+// (sin 2 pi u, cos 2 pi u) for a phase u given in revolutions. FP64:
+// sincos_rev64. MIXED: u is reduced to [-1/2, 1/2] in FP64 and fed to the FP32
+// sin/cos unit, whose argument is in revolutions.
 template <bool MIXED>
 __device__ __forceinline__ void sincos_rev(double u, double &s, double &c) {
     if constexpr (MIXED) {
@@ -76,7 +108,7 @@ __device__ __forceinline__ void sincos_rev(double u, double &s, double &c) {
         s = (double)__builtin_amdgcn_sinf(ff);
         c = (double)__builtin_amdgcn_cosf(ff);
     } else {
-        sincospi(2.0 * u, &s, &c);
+        sincos_rev64(u, s, c);
     }
 }
 
@@ -153,7 +185,7 @@ __device__ __forceinline__ void hankel01_f64(double x, double &h0r, double &h0i,
         double amp = kSqrt2OverPi * rsq64(x);
         // phase x - pi/4 in revolutions: sin/cos(2 pi (x/(2 pi) - 1/8))
         double s, c;
-        sincospi(2.0 * __builtin_fma(x, kInv2Pi, -0.125), &s, &c);
+        sincos_rev64(__builtin_fma(x, kInv2Pi, -0.125), s, c);
         // H0 = amp (P0 + i Q0)(c + i s); H1 = amp (P1 + i Q1)(s - i c)
         h0r = amp * (P0 * c - Q0 * s);
         h0i = amp * (P0 * s + Q0 * c);
