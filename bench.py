@@ -230,7 +230,7 @@ def run_cfie(inti, lib, ctx, args):
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_VALU_TFLOPS, 4),
                      "traffic": traffic,
                      "kernel": ("k_apply<HELMHOLTZ,3,COMB_SD,MIXED>" if args.sym == 0 else
-                                "k_apply_sym<HELMHOLTZ,3,COMB_SD,MIXED,G=1>"),
+                                "k_apply_sym<HELMHOLTZ,3,COMB_SD,MIXED,G=1,TPL=2>"),
                      "kernel_ms": round(kmx, 4), "flops_per_pair": FLOPS_PER_PAIR,
                      "symmetric": args.sym != 0},
         "cpu_baseline": cpu,
