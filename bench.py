@@ -108,8 +108,9 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="profiler calibration: run a plain device copy of this many bytes 5 times")
     ap.add_argument("--nprob", type=int, default=256, help="batch workload: number of problems")
-    ap.add_argument("--sym", type=int, choices=[0, 1], default=None,
-                    help="symmetric evaluation of operators on one quadrature: 0 off, 1 on (default)")
+    ap.add_argument("--sym", type=int, choices=[0, 1, 2], default=None,
+                    help="symmetric evaluation of operators on one quadrature: 0 off, 1 automatic "
+                         "(default), 2 wherever it applies")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "

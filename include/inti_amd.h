@@ -123,10 +123,12 @@ int inti_op_set_launch(inti_op *op, int block, int tpt, int split);
 /* Symmetric evaluation of a square operator whose target and source are the
  * same quadrature handle (single right-hand side): each unordered pair's
  * shared part (distance, e^{ikd}, Hankel values) is computed once and applied
- * in both directions. mode 0: off; 1 (default): on wherever it applies.
- * group: tiles of 256 nodes per work group (1, 2 or 4; 0 = the smallest
- * whose partial sums fit 4 GiB). Results are deterministic in both modes. No
- * counterpart in Inti.jl (its matrices are assembled entry by entry). */
+ * in both directions. mode 0: off; 1 (default): where the work fills the
+ * GPU (at least two blocks of tile pairs per CU, about 8,000 nodes on
+ * MI355X); 2: wherever it applies. group: tiles of 256 nodes per work group
+ * (1, 2 or 4; 0 = the smallest whose partial sums fit 4 GiB). Results are
+ * deterministic in every mode. No counterpart in Inti.jl (its matrices are
+ * assembled entry by entry). */
 int inti_op_set_symmetric(inti_op *op, int mode, int group);
 
 /* y = alpha*(A + dA)*x + beta*y for nrhs columns; host pointers

@@ -761,7 +761,8 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
     const int NRs = 3 * dim + 2;
     const int64_t symT = (n + kSymBS - 1) / kSymBS;
     SymAdjFn fsym = nullptr;
-    if (op->sym && op->trg == op->src && !ff && symT >= 2 &&
+    const bool sym_fills = op->sym == 2 || symT * (symT + 1) / 2 >= 2LL * ctx->num_cu;
+    if (op->sym && sym_fills && op->trg == op->src && !ff && symT >= 2 &&
         sizeof(double) * (size_t)symT * NRs * (size_t)n <= kAdjSymMaxBytes) {
         if (op->pde == INTI_LAPLACE && dim == 2) fsym = pick_sym<INTI_LAPLACE, 2>(kind);
         if (op->pde == INTI_LAPLACE && dim == 3) fsym = pick_sym<INTI_LAPLACE, 3>(kind);

@@ -318,7 +318,7 @@ int inti_op_set_launch(inti_op *op, int block, int tpt, int split) {
 
 int inti_op_set_symmetric(inti_op *op, int mode, int group) {
     if (!op) return INTI_ERR_ARG;
-    if (mode != 0 && mode != 1) return set_error(op->ctx, INTI_ERR_ARG, "op_set_symmetric: mode 0 or 1");
+    if (mode < 0 || mode > 2) return set_error(op->ctx, INTI_ERR_ARG, "op_set_symmetric: mode 0, 1 or 2");
     if (group != 0 && group != 1 && group != 2 && group != 4)
         return set_error(op->ctx, INTI_ERR_ARG, "op_set_symmetric: group 0, 1, 2 or 4");
     op->sym = mode;

@@ -67,7 +67,8 @@ struct Op {
     Csr *corr = nullptr;  // owned merged correction (may be null)
     // launch configuration
     int block = 256, tpt = 1, split = 0;  // split 0 = automatic
-    // symmetric evaluation of square operators on one quadrature: 0 off, 1 on
+    // symmetric evaluation of square operators on one quadrature: 0 off,
+    // 1 automatic (where the work fills the GPU), 2 wherever it applies
     int sym = 1;
     int sym_group = 0;  // tiles per work group; 0 = automatic
 };

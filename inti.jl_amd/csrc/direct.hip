@@ -120,6 +120,10 @@ static bool plan_sym(const Op *op, int kind, int nr, int &G, int64_t &T, int64_t
     const int64_t n = op->src->n;
     T = (n + kSymBS - 1) / kSymBS;
     if (T < 2) return false;
+    // automatic mode: only where the triangle of tile pairs fills the GPU
+    // (two blocks per CU); below that the plain sum, whose source range is
+    // split across blocks, is faster
+    if (op->sym == 1 && T * (T + 1) / 2 < 2LL * op->ctx->num_cu) return false;
     const size_t cw = op_is_complex(op) ? 2 : 1;
     for (int g : {1, 2, 4}) {
         if (op->sym_group && g != op->sym_group) continue;

@@ -170,8 +170,9 @@ class IntegralOperator:
 
     def set_symmetric(self, mode, group=0):
         """Symmetric evaluation when target is source (single right-hand
-        side): mode 0 off, 1 on (default); group = tiles of 256 nodes per
-        work group (0 automatic)."""
+        side): mode 0 off, 1 automatic (default: where the work fills the
+        GPU), 2 wherever it applies; group = tiles of 256 nodes per work group
+        (0 automatic)."""
         self.ctx.check(self.ctx.lib.inti_op_set_symmetric(self.handle, int(mode), int(group)))
 
     def attach(self, corr, coef=1.0):

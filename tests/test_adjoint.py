@@ -288,6 +288,7 @@ def test_gpu_symmetric_pullback(inti, oracle, pde, dim, kind, n):
     wrt = ("sigma", "source_nodes", "source_normals", "source_weights", "target_nodes") + \
         (("k",) if pde == HEL else ())
     A = inti.IntegralOperator(K, Q)
+    A.set_symmetric(2)
     g = A.pullback(u, s, wrt=wrt, dcoefs=dco)
     g2 = A.pullback(u, s, wrt=wrt, dcoefs=dco)
     B = inti.IntegralOperator(K, Q)

@@ -72,13 +72,13 @@ def test_symmetric_matches_oracle(inti, oracle, pde, dim, kind):
     assert np.linalg.norm(y0 - ref) <= 1e-12 * nref
     for group in (1, 2, 4):
         A = inti.IntegralOperator(K, Q, Q)
-        A.set_symmetric(1, group)
+        A.set_symmetric(2, group)
         y = A @ x
         assert np.linalg.norm(y - ref) <= 1e-12 * nref, group
         assert np.linalg.norm(y - y0) <= 1e-13 * nref, group
         assert np.array_equal(y, A @ x)
         Am = inti.IntegralOperator(K, Q, Q, precision="mixed")
-        Am.set_symmetric(1, group)
+        Am.set_symmetric(2, group)
         assert np.linalg.norm(Am @ x - ref) <= 1e-6 * nref, group
 
 
@@ -93,7 +93,7 @@ def test_symmetric_ragged_sizes(inti, oracle, n):
     ref = oracle.matvec(HEL, 3, 4, 2.5, Q, Q, x, coefs=(c0, c1))
     for group in (1, 2, 4):
         A = inti.IntegralOperator(K, Q, Q)
-        A.set_symmetric(1, group)
+        A.set_symmetric(2, group)
         assert np.linalg.norm(A @ x - ref) <= 1e-12 * np.linalg.norm(ref), (n, group)
 
 
@@ -110,7 +110,7 @@ def test_symmetric_with_correction_identity_alpha_beta(inti):
     y0 = rng.normal(size=n) + 1j * rng.normal(size=n)
     L.iop.set_symmetric(0)
     ref = L.apply(x, y0, alpha=0.3 - 0.7j, beta=1.1 + 0.4j)
-    L.iop.set_symmetric(1)
+    L.iop.set_symmetric(2)
     got = L.apply(x, y0, alpha=0.3 - 0.7j, beta=1.1 + 0.4j)
     assert np.linalg.norm(got - ref) <= 1e-13 * np.linalg.norm(ref)
     # multi-RHS applies keep the plain kernels and agree column by column
