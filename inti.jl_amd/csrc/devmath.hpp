@@ -112,11 +112,16 @@ __device__ __forceinline__ void sincos_rev(double u, double &s, double &c) {
     }
 }
 
+// fused multiply-add in the operands' own precision (__builtin_fma is the
+// double one: on floats it would widen, fuse in FP64 and narrow again)
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename T, int N>
 __device__ __forceinline__ T horner(const T (&c)[N], T u) {
     T s = c[N - 1];
 #pragma unroll
-    for (int i = N - 2; i >= 0; --i) s = __builtin_fma(s, u, c[i]);
+    for (int i = N - 2; i >= 0; --i) s = fma_t(s, u, c[i]);
     return s;
 }
 
