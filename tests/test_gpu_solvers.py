@@ -88,3 +88,26 @@ def test_gmres_laplace_first_kind(inti, oracle):
     x_ref = np.linalg.solve(Sd, f)
     assert info["relres"] <= 1e-12
     assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= 1e-9
+
+
+def test_gmres_symmetric_operator_graph_and_refinement(inti, cfie):
+    """The same solves with the symmetric evaluation forced on (this operator
+    is below the automatic threshold): graph replay equals eager execution bit
+    for bit, both match the dense solve, and the mixed refinement reaches the
+    FP64 solution."""
+    say("[gpu] gmres symmetric graph vs eager")
+    Q, L64, Lmx, A, b, x_ref = cfie
+    L64.iop.set_symmetric(2)
+    Lmx.iop.set_symmetric(2)
+    try:
+        x1, i1 = inti.gmres(L64, b, restart=30, maxiter=200, rtol=1e-13, use_graph=False)
+        x2, i2 = inti.gmres(L64, b, restart=30, maxiter=200, rtol=1e-13, use_graph=True)
+        assert i1["iterations"] == i2["iterations"] and np.array_equal(x1, x2)
+        assert np.linalg.norm(x2 - x_ref) <= 1e-10 * np.linalg.norm(x_ref)
+        x, info = inti.refine_solve(L64, Lmx, b, restart=30, maxiter_inner=100, inner_rtol=1e-6,
+                                    max_outer=10, rtol=1e-12)
+        assert info["relres"] <= 1e-12
+        assert np.linalg.norm(x - x_ref) <= 1e-10 * np.linalg.norm(x_ref)
+    finally:
+        L64.iop.set_symmetric(1)
+        Lmx.iop.set_symmetric(1)
