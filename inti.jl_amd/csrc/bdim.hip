@@ -127,11 +127,18 @@ __global__ __launch_bounds__(kThetaBS) void k_theta(ThetaArgs a) {
             pair_value<PDE, DIM, KS>(r, nt, ny, a.p, sr, si);
             pair_value<PDE, DIM, KD>(r, nt, ny, a.p, dr, di);
             sr *= w; si *= w; dr *= w; di *= w;
+            if constexpr (PDE == INTI_LAPLACE) {
+                // real kernels and traces: two FMAs per monopole
 #pragma unroll
-            for (int k = 0; k < NS; ++k) {
-                const double2 b1 = s_b1[jj][k], b0 = s_b0[jj][k];
-                accr[k] += sr * b1.x - si * b1.y - (dr * b0.x - di * b0.y);
-                acci[k] += sr * b1.y + si * b1.x - (dr * b0.y + di * b0.x);
+                for (int k = 0; k < NS; ++k)
+                    accr[k] = __builtin_fma(sr, s_b1[jj][k].x, __builtin_fma(-dr, s_b0[jj][k].x, accr[k]));
+            } else {
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    const double2 b1 = s_b1[jj][k], b0 = s_b0[jj][k];
+                    accr[k] += sr * b1.x - si * b1.y - (dr * b0.x - di * b0.y);
+                    acci[k] += sr * b1.y + si * b1.x - (dr * b0.y + di * b0.x);
+                }
             }
         }
         __syncthreads();
@@ -260,7 +267,9 @@ template <int PDE, int DIM, int DERIV>
 static ThetaFn theta_ns(int nsp) {
     switch (nsp) {
         case 8: return theta_launcher<PDE, DIM, DERIV, 8>;
+        case 12: return theta_launcher<PDE, DIM, DERIV, 12>;
         case 16: return theta_launcher<PDE, DIM, DERIV, 16>;
+        case 20: return theta_launcher<PDE, DIM, DERIV, 20>;
         case 24: return theta_launcher<PDE, DIM, DERIV, 24>;
         case 32: return theta_launcher<PDE, DIM, DERIV, 32>;
     }
@@ -374,7 +383,7 @@ int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *q
         return set_error(ctx, INTI_ERR_SIZE, "bdim: nq*ne must equal the number of sources");
     for (int64_t q = 0; q < (int64_t)nq * ne; ++q)
         if (qtags[q] < 0 || qtags[q] >= n) return set_error(ctx, INTI_ERR_ARG, "bdim: qtags out of range");
-    const int nsp = ns <= 8 ? 8 : ns <= 16 ? 16 : ns <= 24 ? 24 : 32;
+    const int nsp = ns <= 8 ? 8 : ns <= 12 ? 12 : ns <= 16 ? 16 : ns <= 20 ? 20 : ns <= 24 ? 24 : 32;
     BdimFns fns;
     if (!pick_bdim(opS->pde, dim, deriv, nsp, fns))
         return set_error(ctx, INTI_ERR_UNSUPPORTED, "bdim: no kernel for this operator");
