@@ -110,3 +110,52 @@ def test_piece_double_layer_and_green_identity(inti, piece):
           f"green median {np.median(gerr):.2e} max {gerr.max():.2e}", flush=True)
     assert np.median(err) < 1e-3 and np.median(gerr) < 1e-3
     assert err.max() < 5e-2 and gerr.max() < 5e-2
+
+
+def test_square_area_and_perimeter(inti, tmp_path):
+    """test/gmsh_test.jl "Square area and perimeter": a P1 mesh of [-1, 1]^2
+    (tri3 + line2 boundary), written as MSH 4.1 and read back: area 4 and
+    perimeter 8 (exact for straight elements), and the divergence theorem
+    (1/2) sum w x.n = area on the boundary."""
+    G = inti.gmsh
+    m = 9
+    t = np.linspace(-1, 1, m)
+    X, Y = np.meshgrid(t, t, indexing="ij")
+    nodes = np.stack([X.ravel(), Y.ravel(), np.zeros(m * m)], axis=1)
+    idx = np.arange(m * m).reshape(m, m)
+    tris = []
+    for i in range(m - 1):
+        for j in range(m - 1):
+            a, b, c, d = idx[i, j], idx[i + 1, j], idx[i + 1, j + 1], idx[i, j + 1]
+            tris += [(a, b, c), (a, c, d)]
+    # counterclockwise boundary: outward normals (t2, -t1)
+    ring = [idx[i, 0] for i in range(m)] + [idx[m - 1, j] for j in range(1, m)] + \
+        [idx[i, m - 1] for i in range(m - 2, -1, -1)] + [idx[0, j] for j in range(m - 2, 0, -1)]
+    lines = [(ring[q], ring[(q + 1) % len(ring)]) for q in range(len(ring))]
+    path = str(tmp_path / "square.msh")
+    G.write_msh(path, nodes, [(2, 1, 2, np.array(tris)), (1, 1, 1, np.array(lines))])
+    mesh = G.read_msh(path)
+    V = G.entity_quadrature(mesh, 2, qorder=4)
+    B = G.entity_quadrature(mesh, 1, qorder=3)
+    assert abs(V.weights.sum() - 4.0) <= 1e-13
+    assert abs(B.weights.sum() - 8.0) <= 1e-13
+    flux = 0.5 * float((B.weights * (B.coords * B.normals).sum(axis=0)).sum())
+    assert abs(flux - 4.0) <= 1e-13
+
+
+def test_disk_area_and_perimeter(inti, tmp_path):
+    """test/gmsh_test.jl "Circle area and perimeter" (to 1e-2 there): a P2
+    mesh of the unit disk (4 rings) written and read back gives pi and 2 pi,
+    here to 1e-4 (the P2 geometry error, 3e-5, falls by 16 per refinement)."""
+    G = inti.gmsh
+    V, B = inti.disk_quadratures(4, a=1.0, b=1.0, qorder=4, boundary_qorder=6)
+    tri, lin = V.elements, B.elements
+    pts = np.concatenate([tri.reshape(-1, 2), lin.reshape(-1, 2)])
+    nodes = np.concatenate([pts, np.zeros((len(pts), 1))], axis=1)
+    nt = tri.shape[0] * 6
+    path = str(tmp_path / "disk.msh")
+    G.write_msh(path, nodes, [(2, 1, 9, np.arange(nt).reshape(-1, 6)),
+                              (1, 1, 8, nt + np.arange(lin.shape[0] * 3).reshape(-1, 3))])
+    mesh = G.read_msh(path)
+    assert abs(G.entity_quadrature(mesh, 2, qorder=4).weights.sum() - np.pi) <= 1e-4
+    assert abs(G.entity_quadrature(mesh, 1, qorder=6).weights.sum() - 2 * np.pi) <= 1e-4
