@@ -108,8 +108,9 @@ def test_piece_double_layer_and_green_identity(inti, piece):
     gerr = np.abs(S @ dudn - D @ u - 0.5 * u) / np.abs(u).max()
     print(f"piece: nodes {len(Q)}, |D1 + 1/2| median {np.median(err):.2e} max {err.max():.2e}; "
           f"green median {np.median(gerr):.2e} max {gerr.max():.2e}", flush=True)
-    assert np.median(err) < 1e-3 and np.median(gerr) < 1e-3
-    assert err.max() < 5e-2 and gerr.max() < 5e-2
+    # measured on MI355X: median 2.7e-11 / 3.2e-10, max 2.4e-7 / 4.1e-7 (at the sharp edges)
+    assert np.median(err) < 1e-8 and np.median(gerr) < 1e-8
+    assert err.max() < 1e-5 and gerr.max() < 1e-5
 
 
 def test_square_area_and_perimeter(inti, tmp_path):
