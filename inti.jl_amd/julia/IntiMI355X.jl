@@ -113,6 +113,43 @@ function gpu_operator(iop::Inti.IntegralOperator; precision = FP64)
 end
 
 """
+    set_symmetric!(iop, mode)
+
+Symmetric evaluation of an operator whose target is its source (the same
+`Quadrature` object, hence the same device quadrature): 0 off, 1 automatic
+(the default), 2 always.
+"""
+function set_symmetric!(iop::Inti.IntegralOperator, mode::Integer)
+    check(CTX[].ptr, ccall((:inti_op_set_symmetric, libinti), Cint, (Ptr{Cvoid}, Cint, Cint),
+        op_handle(iop), mode, 0))
+    return iop
+end
+
+"""
+    pullback(iop, ȳ, σ; dcoefs = nothing)
+
+Reverse mode of `y = iop * σ` on the GPU: the gradients of Re⟨ȳ, y⟩ with
+respect to σ (`iop' * ȳ`), the source nodes, normals and weights, the
+target nodes and k. For an operator on one quadrature (`iop.target ===
+iop.source`) the node gradient is the sum of the source and target parts.
+"""
+function pullback(iop::Inti.IntegralOperator, ȳ::Vector{T}, σ::Vector{T};
+                  dcoefs = nothing) where {T}
+    m, n = size(iop)
+    N = Inti.ambient_dimension(iop.source)
+    σ̄ = similar(σ)
+    ys, ns, xs, ws = zeros(N, n), zeros(N, n), zeros(N, m), zeros(n)
+    k̄ = Ref(0.0)
+    dc = dcoefs === nothing ? C_NULL : Float64[real(dcoefs[1]), imag(dcoefs[1]), real(dcoefs[2]), imag(dcoefs[2])]
+    check(CTX[].ptr, ccall((:inti_op_adjoint, libinti), Cint,
+        (Ptr{Cvoid}, Ptr{T}, Ptr{T}, Ptr{T}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+         Ptr{Float64}, Ptr{Float64}, Ref{Float64}),
+        op_handle(iop), ȳ, σ, σ̄, ys, ns, ws, xs, dc, k̄))
+    nodes = iop.target === iop.source ? ys .+ xs : ys
+    return (; sigma = σ̄, nodes, target_nodes = xs, normals = ns, weights = ws, k = k̄[])
+end
+
+"""
     assemble_matrix(iop) -> Matrix   (src/nystrom.jl:103-124 on the GPU)
 """
 function assemble_matrix(iop::Inti.IntegralOperator)
