@@ -138,7 +138,9 @@ function pullback(iop::Inti.IntegralOperator, ȳ::Vector{T}, σ::Vector{T};
     m, n = size(iop)
     N = Inti.ambient_dimension(iop.source)
     σ̄ = similar(σ)
-    ys, ns, xs, ws = zeros(N, n), zeros(N, n), zeros(N, m), zeros(n)
+    # gradients of points are n x N (column d = coordinate d), the layout of
+    # the uploaded coordinates
+    ys, ns, xs, ws = zeros(n, N), zeros(n, N), zeros(m, N), zeros(n)
     k̄ = Ref(0.0)
     dc = dcoefs === nothing ? C_NULL : Float64[real(dcoefs[1]), imag(dcoefs[1]), real(dcoefs[2]), imag(dcoefs[2])]
     check(CTX[].ptr, ccall((:inti_op_adjoint, libinti), Cint,
