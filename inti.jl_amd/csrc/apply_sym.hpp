@@ -297,12 +297,12 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
 
 using SymFn = void (*)(const SymArgs &, int64_t, hipStream_t);
 
-// two targets per lane (one LDS read of a source serves two pairs), except
-// for the 2D Helmholtz kernels, whose Hankel evaluation already fills the
-// register file
+// targets per lane (one LDS read of a source serves TPL pairs): 4 for the
+// cheap Laplace pairs, 2 for 3D Helmholtz, 1 for the 2D Helmholtz kernels,
+// whose Hankel evaluation already fills the register file (measured best)
 template <int PDE, int DIM, int KIND, bool MIXED, int G>
 static void sym_launcher(const SymArgs &a, int64_t nblocks, hipStream_t s) {
-    constexpr int TPL = (PDE == INTI_HELMHOLTZ && DIM == 2) ? 1 : 2;
+    constexpr int TPL = PDE == INTI_LAPLACE ? 4 : DIM == 2 ? 1 : 2;
     hipLaunchKernelGGL((k_apply_sym<PDE, DIM, KIND, MIXED, G, TPL>), dim3((unsigned)nblocks),
                        dim3(kSymBS / TPL), 0, s, a);
 }
