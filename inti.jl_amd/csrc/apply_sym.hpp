@@ -8,7 +8,7 @@
 // Work decomposition. The n nodes are cut into T tiles of 256 and the tiles
 // into S groups of G. Block (a, b), a <= b, takes the target tiles of group a
 // against the source tiles of group b (within a diagonal block, the tile
-// pairs I <= J only). A wave holds 64 x TPL targets (TPL = 1 or 2 per lane).
+// pairs I <= J only). A wave holds 64 x TPL targets (TPL = 1, 2 or 4 per lane).
 // For each 64-source sub-tile, lane l at step s pairs its targets with
 // source (l + s) mod 64, so the 64 lanes touch 64 different sources, read
 // from LDS without bank conflicts. The transposed sum for source (l + s) mod
