@@ -126,9 +126,30 @@ __device__ __forceinline__ AdjRadial adj_radial(double d2, const AdjParams &p) {
     constexpr bool WG = KIND != INTI_DL, WH = KIND != INTI_SL;
     AdjRadial R;
     R.ok = d2 > kSamePointTol2;
+    if constexpr (PDE == INTI_HELMHOLTZ && DIM == 3 && KIND == INTI_COMB_SD) {
+        // every radial function is E = e^{ikd}/(4 pi) times a simple factor:
+        // fold the coefficients into E once (E0 = c0 E, E1 = c1 E, D0 = dc0 E,
+        // D1 = dc1 E) instead of into each of the six values
+        const double d2s = fmax(d2, kClampD2);
+        const double id = rsq64(d2s), d = d2s * id, id2 = id * id, id3 = id2 * id;
+        double s, c;
+        sincos_rev<false>(p.kappa * d, s, c);
+        const double2 E = c2(kInv4Pi * c, kInv4Pi * s);
+        const double2 E0 = cm(p.c0, E), E1 = cm(p.c1, E), D0 = cm(p.dc0, E), D1 = cm(p.dc1, E);
+        const double kd = p.k * d, kid = p.k * id;
+        R.g = cs(E0, id);                                              // c0 E / d
+        R.gpd = cs(c2(-E0.x - kd * E0.y, kd * E0.x - E0.y), id3);      // c0 E (-1 + ikd) / d^3
+        R.gk = c2(D0.x * id - E0.y, D0.y * id + E0.x);                 // dc0 E / d + c0 i E
+        R.h = cs(c2(E1.x + kd * E1.y, E1.y - kd * E1.x), id3);         // c1 E (1 - ikd) / d^3
+        const double A = kd * kd - 3.0, B = 3.0 * kd;
+        R.hpd = cs(c2(E1.x * A - E1.y * B, E1.x * B + E1.y * A), id3 * id2);  // c1 E (k^2d^2 - 3 + 3ikd) / d^5
+        R.hk = c2((D1.x + kd * D1.y) * id3 + E1.x * kid,               // dc1 E (1 - ikd) / d^3 + c1 E k / d
+                  (D1.y - kd * D1.x) * id3 + E1.y * kid);
+        return R;
+    }
     R.g = R.gpd = R.gk = R.h = R.hpd = R.hk = c2(0, 0);
     radial<PDE, DIM, WG, WH>(fmax(d2, kClampD2), p, R.g, R.gpd, R.gk, R.h, R.hpd, R.hk);
-    if constexpr (KIND == INTI_COMB_SD) {
+    if constexpr (KIND == INTI_COMB_SD) {  // (3D Helmholtz returned above)
         const double2 g = R.g, h = R.h;
         R.gk = ca(cm(p.dc0, g), cm(p.c0, R.gk));
         R.g = cm(p.c0, g);
