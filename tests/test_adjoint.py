@@ -307,3 +307,28 @@ def test_gpu_symmetric_pullback(inti, oracle, pde, dim, kind, n):
     if pde == HEL:
         assert abs(g["k"] - o["kbar"]) <= 1e-11 * abs(o["kbar"])
         assert abs(g["k"] - h["k"]) <= 1e-12 * abs(o["kbar"])
+
+
+@pytest.mark.gpu
+def test_gpu_symmetric_pullback_automatic_on_a_large_operator(inti):
+    """A 3D Helmholtz combined field on a 20,280-node sphere: the automatic
+    mode takes the symmetric pullback (enough tile pairs to fill the GPU);
+    every output equals the two-pass pullback's to 1e-12."""
+    print("symmetric pullback, automatic mode, 20,280 nodes", flush=True)
+    Q = inti.sphere_quadrature(13)
+    n = len(Q)
+    k = 3.0
+    op = inti.Helmholtz(k=k, dim=3)
+    K = inti.CombinedFieldKernel(op, -1j * k, 1.0)
+    rng = np.random.default_rng(13)
+    u = rng.normal(size=n) + 1j * rng.normal(size=n)
+    s = rng.normal(size=n) + 1j * rng.normal(size=n)
+    wrt = ("sigma", "source_nodes", "source_normals", "source_weights", "target_nodes", "k")
+    A = inti.IntegralOperator(K, Q)
+    g = A.pullback(u, s, wrt=wrt, dcoefs=(-1j, 0.0))
+    B = inti.IntegralOperator(K, Q)
+    B.set_symmetric(0)
+    h = B.pullback(u, s, wrt=wrt, dcoefs=(-1j, 0.0))
+    for a in wrt[:-1]:
+        assert np.linalg.norm(g[a] - h[a]) <= 1e-12 * np.linalg.norm(h[a]), a
+    assert abs(g["k"] - h["k"]) <= 1e-12 * abs(h["k"])
