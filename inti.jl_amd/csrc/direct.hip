@@ -36,13 +36,18 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
     double *y = a.y + (CPLX ? 2 : 1) * (int64_t)r * a.ldy;
     const int64_t pbase = (int64_t)r * a.split * a.m;
     double sr = 0.0, si = 0.0;
+    // many partials per row (one per tile group of the symmetric sum): the
+    // unrolled loop keeps several loads in flight per thread (the order of
+    // the additions is unchanged)
     if constexpr (CPLX) {
         const double2 *pp = reinterpret_cast<const double2 *>(a.part) + pbase;
+#pragma unroll 8
         for (int c = 0; c < a.split; ++c) {
             double2 v = pp[(int64_t)c * a.m + i];
             sr += v.x; si += v.y;
         }
     } else {
+#pragma unroll 8
         for (int c = 0; c < a.split; ++c) sr += a.part[pbase + (int64_t)c * a.m + i];
     }
     {
