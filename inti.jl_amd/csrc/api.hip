@@ -1,8 +1,10 @@
 // C-ABI entry points of libinti_amd (declared in include/inti_amd.h):
 // context / stream / profiling, quadrature upload, operators, corrections.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 
 #include "common.hpp"
 
@@ -198,6 +200,51 @@ int inti_device_copy(inti_ctx *ctx, void *dst, const void *src, size_t bytes) {
 }
 
 // ------------------------------------------------------------- quadrature
+// True if every two of the n nodes are farther apart than 2e-14 (twice the
+// coincidence tolerance, so that rounding differences between this test and
+// the kernels' own distances cannot matter). Nodes are binned in cubes of
+// side h = 4e-14; a closer pair lies in the same or in neighbouring cubes.
+// O(n log n). Coordinates beyond 1e5 in magnitude (cube indices near the
+// int64 range) give false, which keeps the mask.
+static bool nodes_distinct(int dim, int64_t n, const double *c) {
+    const double h = 4.0e-14, tol2 = 4.0e-28;
+    std::vector<std::array<int64_t, 3>> key(n);
+    for (int64_t i = 0; i < n; ++i) {
+        for (int d = 0; d < 3; ++d) {
+            if (d >= dim) { key[i][d] = 0; continue; }
+            const double v = c[(int64_t)d * n + i];
+            if (!(std::fabs(v) < 1.0e5)) return false;
+            key[i][d] = (int64_t)std::floor(v / h);
+        }
+    }
+    std::vector<int64_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+    std::vector<std::array<int64_t, 3>> sk(n);
+    for (int64_t q = 0; q < n; ++q) sk[q] = key[idx[q]];
+    const int zr = dim == 3 ? 1 : 0;
+    for (int64_t q = 0; q < n; ++q) {
+        const int64_t i = idx[q];
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -zr; dz <= zr; ++dz) {
+                    const std::array<int64_t, 3> nb = {sk[q][0] + dx, sk[q][1] + dy, sk[q][2] + dz};
+                    auto rg = std::equal_range(sk.begin(), sk.end(), nb);
+                    for (auto it = rg.first; it != rg.second; ++it) {
+                        const int64_t j = idx[it - sk.begin()];
+                        if (j == i) continue;
+                        double d2 = 0.0;
+                        for (int d = 0; d < dim; ++d) {
+                            const double t = c[(int64_t)d * n + i] - c[(int64_t)d * n + j];
+                            d2 += t * t;
+                        }
+                        if (d2 <= tol2) return false;
+                    }
+                }
+    }
+    return true;
+}
+
 int inti_quad_upload(inti_ctx *ctx, int dim, int64_t n, const double *coords,
                      const double *normals, const double *weights, inti_quad **out) {
     if (!ctx) return INTI_ERR_ARG;
@@ -231,6 +278,7 @@ int inti_quad_upload(inti_ctx *ctx, int dim, int64_t n, const double *coords,
             }
             q->lo[d] = lo; q->hi[d] = hi;
         }
+        q->distinct = nodes_distinct(dim, n, coords);
         INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
     *out = q;

@@ -26,6 +26,8 @@
 // schedule; the reduction kernel of the plain direct sum (split = S) sums the
 // slots and applies the post factor, the correction and the identity.
 #pragma once
+#include <type_traits>
+
 #include "apply_kernel.hpp"
 
 namespace inti {
@@ -36,6 +38,7 @@ struct SymArgs {
     const double *x;             // density (complex interleaved for Helmholtz)
     double *part;                // S x n partial sums (complex interleaved)
     int64_t ntiles, ngroups;     // T, S
+    int distinct;                // no two nodes within twice the coincidence tolerance
     KParams p;
 };
 
@@ -207,35 +210,42 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                         }
                     }
                 } else {
+                    // off-diagonal tiles of a quadrature with distinct nodes hold
+                    // no coincident pairs: the pair values need no mask there
+                    auto pairs = [&](auto mask_tag) {
+                        constexpr bool MASK = decltype(mask_tag)::value;
 #pragma unroll 2
-                    for (int s = 0; s < 64; ++s) {
-                        const int jl = u * 64 + ((lane + s) & 63);
-                        double yj[3], nj[3], qjr, qji;
-                        R::get(s_src, jl, yj, nj, qjr, qji);
+                        for (int s = 0; s < 64; ++s) {
+                            const int jl = u * 64 + ((lane + s) & 63);
+                            double yj[3], nj[3], qjr, qji;
+                            R::get(s_src, jl, yj, nj, qjr, qji);
 #pragma unroll
-                        for (int t = 0; t < TPL; ++t) {
-                            double r[DIM];
+                            for (int t = 0; t < TPL; ++t) {
+                                double r[DIM];
 #pragma unroll
-                            for (int d = 0; d < DIM; ++d) r[d] = xi[t][d] - yj[d];
-                            double k1r, k1i, k2r, k2i;  // K(x_i, x_j), K(x_j, x_i)
-                            pair_raw_sym<PDE, DIM, KIND, MIXED>(r, ni[t], nj, p, k1r, k1i, k2r, k2i);
-                            if constexpr (CPLX) {
-                                adr[gi][t] = __builtin_fma(k1r, qjr, adr[gi][t]);
-                                adr[gi][t] = __builtin_fma(-k1i, qji, adr[gi][t]);
-                                adi[gi][t] = __builtin_fma(k1r, qji, adi[gi][t]);
-                                adi[gi][t] = __builtin_fma(k1i, qjr, adi[gi][t]);
-                                ajr = __builtin_fma(k2r, qir[t], ajr);
-                                ajr = __builtin_fma(-k2i, qii[t], ajr);
-                                aji = __builtin_fma(k2r, qii[t], aji);
-                                aji = __builtin_fma(k2i, qir[t], aji);
-                            } else {
-                                adr[gi][t] = __builtin_fma(k1r, qjr, adr[gi][t]);
-                                ajr = __builtin_fma(k2r, qir[t], ajr);
+                                for (int d = 0; d < DIM; ++d) r[d] = xi[t][d] - yj[d];
+                                double k1r, k1i, k2r, k2i;  // K(x_i, x_j), K(x_j, x_i)
+                                pair_raw_sym<PDE, DIM, KIND, MIXED, MASK>(r, ni[t], nj, p, k1r, k1i, k2r, k2i);
+                                if constexpr (CPLX) {
+                                    adr[gi][t] = __builtin_fma(k1r, qjr, adr[gi][t]);
+                                    adr[gi][t] = __builtin_fma(-k1i, qji, adr[gi][t]);
+                                    adi[gi][t] = __builtin_fma(k1r, qji, adi[gi][t]);
+                                    adi[gi][t] = __builtin_fma(k1i, qjr, adi[gi][t]);
+                                    ajr = __builtin_fma(k2r, qir[t], ajr);
+                                    ajr = __builtin_fma(-k2i, qii[t], ajr);
+                                    aji = __builtin_fma(k2r, qii[t], aji);
+                                    aji = __builtin_fma(k2i, qir[t], aji);
+                                } else {
+                                    adr[gi][t] = __builtin_fma(k1r, qjr, adr[gi][t]);
+                                    ajr = __builtin_fma(k2r, qir[t], ajr);
+                                }
                             }
+                            ajr = lane_next(ajr, nxt);
+                            if constexpr (CPLX) aji = lane_next(aji, nxt);
                         }
-                        ajr = lane_next(ajr, nxt);
-                        if constexpr (CPLX) aji = lane_next(aji, nxt);
-                    }
+                    };
+                    if (A.distinct) pairs(std::false_type{});
+                    else pairs(std::true_type{});
                 }
             }
             if (trans) {

@@ -44,6 +44,9 @@ struct Quad {
     double *normals = nullptr;  // dim*n SoA or null
     double *weights = nullptr;  // n
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // bounding box
+    // no two nodes within 2x the coincidence tolerance: pairs of different
+    // nodes then never need the coincident-point mask
+    bool distinct = false;
 };
 
 // CSR with 0-based int32 indices; values real (double) or complex

@@ -256,6 +256,7 @@ int launch_apply(const DirectLaunch &L) {
             a.xy = op->src->coords; a.nrm = op->src->normals; a.w = op->src->weights;
             a.x = xr; a.part = part;
             a.ntiles = symT; a.ngroups = symS;
+            a.distinct = op->src->distinct ? 1 : 0;
             a.p = kp;
             if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
             fn(a, symS * (symS + 1) / 2, ctx->stream);

@@ -127,12 +127,12 @@ __device__ __forceinline__ void helm3_raw(const double *r, double d2s, bool ok, 
 // out), and coincident points are masked on the FP32 amplitude factor. d2 is
 // the unclamped squared distance; FP32 clamping keeps every intermediate
 // finite for d2 = 0.
-template <int KIND>
+template <int KIND, bool MASK = true>
 __device__ __forceinline__ void helm3_raw_mixed(const double *r, double d2, const double *nx,
                                                 const double *ny, const KParams &p, double &kr,
                                                 double &ki) {
-    const bool ok = d2 > kSamePointTol2;
-    const float y32 = __builtin_amdgcn_rsqf(fmaxf((float)d2, 1.0e-30f));
+    const bool ok = !MASK || d2 > kSamePointTol2;
+    const float y32 = __builtin_amdgcn_rsqf(MASK ? fmaxf((float)d2, 1.0e-30f) : (float)d2);
     const double y = (double)y32;
     const double e = d2 * y;
     const double h = __builtin_fma(-e, y, 1.0);
@@ -240,18 +240,21 @@ __device__ __forceinline__ void helm_ff_raw(const double *xh, const double *y, c
     }
 }
 
-// raw value of the pair (post applied by the caller)
-template <int PDE, int DIM, int KIND, bool MIXED>
+// raw value of the pair (post applied by the caller). MASK = false is for
+// pairs known to be farther apart than the coincidence tolerance (the
+// quadrature's nodes are distinct and the two nodes differ): no mask, no
+// clamp, the same values.
+template <int PDE, int DIM, int KIND, bool MIXED, bool MASK = true>
 __device__ __forceinline__ void pair_raw(const double *r, const double *nx, const double *ny,
                                          const KParams &p, double &kr, double &ki) {
     static_assert(KIND < INTI_FF_SL, "far-field kinds go through pair_raw_xy");
     double d2 = dotv<DIM>(r, r);
-    bool ok = d2 > kSamePointTol2;
-    double d2s = fmax(d2, kClampD2);
+    bool ok = !MASK || d2 > kSamePointTol2;
+    double d2s = MASK ? fmax(d2, kClampD2) : d2;
     if constexpr (PDE == INTI_LAPLACE && DIM == 3 && MIXED) {
         // 1/d from the FP32 unit, masked in FP32: zero for coincident points,
         // and every kind below vanishes with it
-        const float y32 = __builtin_amdgcn_rsqf(fmaxf((float)d2, 1.0e-30f));
+        const float y32 = __builtin_amdgcn_rsqf(MASK ? fmaxf((float)d2, 1.0e-30f) : (float)d2);
         const double y = (double)(ok ? y32 : 0.0f);
         if constexpr (KIND == INTI_SL) {
             kr = y;
@@ -274,7 +277,7 @@ __device__ __forceinline__ void pair_raw(const double *r, const double *nx, cons
     } else if constexpr (DIM == 3 && MIXED &&
                          (KIND == INTI_SL || KIND == INTI_DL || KIND == INTI_ADL ||
                           KIND == INTI_COMB_SD)) {
-        helm3_raw_mixed<KIND>(r, d2, nx, ny, p, kr, ki);
+        helm3_raw_mixed<KIND, MASK>(r, d2, nx, ny, p, kr, ki);
     } else if constexpr (DIM == 3) {
         helm3_raw<KIND, MIXED>(r, d2s, ok, nx, ny, p, kr, ki);
     } else {
@@ -304,7 +307,7 @@ __device__ __forceinline__ void pair_raw_xy(const double *x, const double *nx, c
 // r = x_i - x_j. The shared radial parts are computed once: explicitly for
 // the 2D Helmholtz kernels (the Hankel evaluation branches, so two calls
 // would not be merged), as common subexpressions of two calls otherwise.
-template <int PDE, int DIM, int KIND, bool MIXED>
+template <int PDE, int DIM, int KIND, bool MIXED, bool MASK = true>
 __device__ __forceinline__ void pair_raw_sym(const double *r, const double *ni, const double *nj,
                                              const KParams &p, double &k1r, double &k1i,
                                              double &k2r, double &k2i) {
@@ -313,13 +316,13 @@ __device__ __forceinline__ void pair_raw_sym(const double *r, const double *ni, 
     for (int d = 0; d < DIM; ++d) rn[d] = -r[d];
     if constexpr (PDE == INTI_HELMHOLTZ && DIM == 2) {
         const double d2 = dotv<2>(r, r);
-        const bool ok = d2 > kSamePointTol2;
-        const Helm2Radial R = helm2_radial<MIXED>(fmax(d2, kClampD2), p);
+        const bool ok = !MASK || d2 > kSamePointTol2;
+        const Helm2Radial R = helm2_radial<MIXED>(MASK ? fmax(d2, kClampD2) : d2, p);
         helm2_combine<KIND, MIXED>(r, ok, ni, nj, p, R, k1r, k1i);
         helm2_combine<KIND, MIXED>(rn, ok, nj, ni, p, R, k2r, k2i);
     } else {
-        pair_raw<PDE, DIM, KIND, MIXED>(r, ni, nj, p, k1r, k1i);
-        pair_raw<PDE, DIM, KIND, MIXED>(rn, nj, ni, p, k2r, k2i);
+        pair_raw<PDE, DIM, KIND, MIXED, MASK>(r, ni, nj, p, k1r, k1i);
+        pair_raw<PDE, DIM, KIND, MIXED, MASK>(rn, nj, ni, p, k2r, k2i);
     }
 }
 

@@ -117,3 +117,28 @@ def test_symmetric_with_correction_identity_alpha_beta(inti):
     X = np.stack([x, y0], axis=1)
     Y = L.apply(X)
     assert np.linalg.norm(Y[:, 0] - L.apply(x)) <= 1e-13 * np.linalg.norm(Y[:, 0])
+
+
+@pytest.mark.parametrize("pde,dim,kind", [(LAP, 3, 0), (HEL, 3, 4), (HEL, 2, 1), (LAP, 2, 1)])
+def test_symmetric_with_duplicate_nodes(inti, oracle, pde, dim, kind):
+    """Nodes repeated in other tiles (coincident pairs across tiles): the
+    quadrature is not 'distinct', the kernels keep the coincidence mask, and
+    the symmetric sum still equals the oracle's (which zeroes those pairs)."""
+    say(f"[gpu] symmetric with duplicate nodes pde={pde} dim={dim} kind={KNAMES[kind]}")
+    base = cloud(inti, dim, 900, 21)
+    dup = np.arange(0, 900, 7)
+    X = np.concatenate([base.coords, base.coords[:, dup]], axis=1)
+    N = np.concatenate([base.normals, base.normals[:, dup]], axis=1)
+    w = np.concatenate([base.weights, base.weights[dup]])
+    n = X.shape[1]
+    Q = inti.Quadrature(X, N, w, np.arange(n)[None, :])
+    k = 0.0 if pde == LAP else 2.0
+    K, coefs = _kernel(inti, pde, dim, kind, k)
+    x = _density(pde, n, 5)
+    ref = oracle.matvec(pde, dim, kind, k, Q, Q, x, coefs=coefs)
+    for prec, tol in (("fp64", 1e-12), ("mixed", 1e-6)):
+        A = inti.IntegralOperator(K, Q, Q, precision=prec)
+        A.set_symmetric(2)
+        y = A @ x
+        assert np.all(np.isfinite(y))
+        assert np.linalg.norm(y - ref) <= tol * np.linalg.norm(ref), prec
