@@ -27,6 +27,7 @@
 //                h = (ik/4) H1(kd)/d, h_k = (ik/4) H0(kd),
 //                h'/d = (ik/4) (k H0(kd)/d^2 - 2 H1(kd)/d^3).
 #include <complex>
+#include <type_traits>
 #include <vector>
 
 #include "apply_sym.hpp"
@@ -121,16 +122,18 @@ struct AdjRadial {
     double2 g, gpd, gk, h, hpd, hk;
     bool ok;
 };
-template <int PDE, int DIM, int KIND>
+// MASK = false: the pair is known to be farther apart than the coincidence
+// tolerance (distinct nodes, different nodes): no clamp, no mask.
+template <int PDE, int DIM, int KIND, bool MASK = true>
 __device__ __forceinline__ AdjRadial adj_radial(double d2, const AdjParams &p) {
     constexpr bool WG = KIND != INTI_DL, WH = KIND != INTI_SL;
     AdjRadial R;
-    R.ok = d2 > kSamePointTol2;
+    R.ok = !MASK || d2 > kSamePointTol2;
     if constexpr (PDE == INTI_HELMHOLTZ && DIM == 3 && KIND == INTI_COMB_SD) {
         // every radial function is E = e^{ikd}/(4 pi) times a simple factor:
         // fold the coefficients into E once (E0 = c0 E, E1 = c1 E, D0 = dc0 E,
         // D1 = dc1 E) instead of into each of the six values
-        const double d2s = fmax(d2, kClampD2);
+        const double d2s = MASK ? fmax(d2, kClampD2) : d2;
         const double id = rsq64(d2s), d = d2s * id, id2 = id * id, id3 = id2 * id;
         double s, c;
         sincos_rev<false>(p.kappa * d, s, c);
@@ -148,7 +151,7 @@ __device__ __forceinline__ AdjRadial adj_radial(double d2, const AdjParams &p) {
         return R;
     }
     R.g = R.gpd = R.gk = R.h = R.hpd = R.hk = c2(0, 0);
-    radial<PDE, DIM, WG, WH>(fmax(d2, kClampD2), p, R.g, R.gpd, R.gk, R.h, R.hpd, R.hk);
+    radial<PDE, DIM, WG, WH>(MASK ? fmax(d2, kClampD2) : d2, p, R.g, R.gpd, R.gk, R.h, R.hpd, R.hk);
     if constexpr (KIND == INTI_COMB_SD) {  // (3D Helmholtz returned above)
         const double2 g = R.g, h = R.h;
         R.gk = ca(cm(p.dc0, g), cm(p.c0, R.gk));
@@ -165,7 +168,7 @@ __device__ __forceinline__ AdjRadial adj_radial(double d2, const AdjParams &p) {
 // kinds for v = x - y and the source normal ny, from the radial functions.
 // SL: K = g, dK/dy = -gpd r;  DL: K = h rn, dK/dy = -(hpd rn r + h n),
 // dK/dn = h r (rn = r . n_y); coincident points give zeros.
-template <int PDE, int DIM, int KIND>
+template <int PDE, int DIM, int KIND, bool MASK = true>
 __device__ __forceinline__ void adj_combine(const double *v, const double *ny, const AdjRadial &R,
                                             const AdjParams &p, double2 &K, double2 &gk,
                                             double2 &a_r, double2 &a_n, double2 &b_r) {
@@ -181,8 +184,10 @@ __device__ __forceinline__ void adj_combine(const double *v, const double *ny, c
         b_r = R.h;
         a_n = cs(b_r, -1.0);
     }
-    const double mk = R.ok ? 1.0 : 0.0;
-    K = cs(K, mk); gk = cs(gk, mk); a_r = cs(a_r, mk); a_n = cs(a_n, mk); b_r = cs(b_r, mk);
+    if constexpr (MASK) {
+        const double mk = R.ok ? 1.0 : 0.0;
+        K = cs(K, mk); gk = cs(gk, mk); a_r = cs(a_r, mk); a_n = cs(a_n, mk); b_r = cs(b_r, mk);
+    }
 }
 
 // Value and derivatives of one pair, x the target (or far-field direction),
@@ -456,6 +461,7 @@ struct AdjSymArgs {
     double *part;   // S x NR x n
     double *kpart;  // one per block
     int64_t ntiles;
+    int distinct;   // no two nodes within twice the coincidence tolerance
     AdjParams p;
 };
 
@@ -479,12 +485,12 @@ __device__ __forceinline__ void adj_sym_node(const AdjSymArgs &A, int64_t j, dou
 
 // one ordered pair (target t, source s): adds to the source's sums S0, Y, N
 // and the target's X; returns Re(sc gk)
-template <int PDE, int DIM, int KIND>
+template <int PDE, int DIM, int KIND, bool MASK>
 __device__ __forceinline__ double adj_sym_pair(const double *v, const double *ns, const AdjRadial &R,
                                                const AdjParams &p, double2 ct, double2 ss,
                                                double *src, double *trg) {
     double2 K, gk, a_r, a_n, b_r;
-    adj_combine<PDE, DIM, KIND>(v, ns, R, p, K, gk, a_r, a_n, b_r);
+    adj_combine<PDE, DIM, KIND, MASK>(v, ns, R, p, K, gk, a_r, a_n, b_r);
     const double2 sc = cm(ss, ct);
     src[0] = __builtin_fma(ct.x, K.x, __builtin_fma(-ct.y, K.y, src[0]));
     src[1] = __builtin_fma(ct.x, K.y, __builtin_fma(ct.y, K.x, src[1]));
@@ -536,23 +542,29 @@ __global__ __launch_bounds__(kSymBS) void k_adj_sym(AdjSymArgs A) {
         double rot[NR];  // sums of source (lane + s) mod 64 of sub-tile u
 #pragma unroll
         for (int r = 0; r < NR; ++r) rot[r] = 0.0;
-        for (int s = 0; s < 64; ++s) {
-            const int jl = u * 64 + ((lane + s) & 63);
-            double xj[DIM], nj[DIM], v[DIM], vn[DIM];
+        // off-diagonal tiles of distinct nodes hold no coincident pairs
+        auto pairs = [&](auto mask_tag) {
+            constexpr bool MASK = decltype(mask_tag)::value;
+            for (int s = 0; s < 64; ++s) {
+                const int jl = u * 64 + ((lane + s) & 63);
+                double xj[DIM], nj[DIM], v[DIM], vn[DIM];
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                xj[d] = s_x[d][jl];
-                nj[d] = s_n[d][jl];
-                v[d] = xi[d] - xj[d];
-                vn[d] = -v[d];
+                for (int d = 0; d < DIM; ++d) {
+                    xj[d] = s_x[d][jl];
+                    nj[d] = s_n[d][jl];
+                    v[d] = xi[d] - xj[d];
+                    vn[d] = -v[d];
+                }
+                const double2 cj = s_c[jl], sj = s_s[jl];
+                const AdjRadial R = adj_radial<PDE, DIM, KIND, MASK>(dotv<DIM>(v, v), p);
+                kb += adj_sym_pair<PDE, DIM, KIND, MASK>(v, nj, R, p, ci, sj, rot, own);  // i target, j source
+                if (!diag) kb += adj_sym_pair<PDE, DIM, KIND, MASK>(vn, ni, R, p, cj, si, own, rot);  // j target, i source
+#pragma unroll
+                for (int r = 0; r < NR; ++r) rot[r] = lane_next(rot[r], nxt);
             }
-            const double2 cj = s_c[jl], sj = s_s[jl];
-            const AdjRadial R = adj_radial<PDE, DIM, KIND>(dotv<DIM>(v, v), p);
-            kb += adj_sym_pair<PDE, DIM, KIND>(v, nj, R, p, ci, sj, rot, own);  // i target, j source
-            if (!diag) kb += adj_sym_pair<PDE, DIM, KIND>(vn, ni, R, p, cj, si, own, rot);  // j target, i source
-#pragma unroll
-            for (int r = 0; r < NR; ++r) rot[r] = lane_next(rot[r], nxt);
-        }
+        };
+        if (A.distinct && !diag) pairs(std::false_type{});
+        else pairs(std::true_type{});
         // add the waves' sums for sub-tile u in wave order
         for (int w = 0; w < kSymBS / 64; ++w) {
             if (wave == w) {
@@ -831,6 +843,7 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
         sa.c = (const double2 *)dc.p; sa.sig = (const double2 *)dsg.p;
         sa.part = (double *)dpart.p; sa.kpart = (double *)dkp.p;
         sa.ntiles = symT;
+        sa.distinct = op->src->distinct ? 1 : 0;
         sa.p = p;
         if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev0, s));
         fsym(sa, symB, s);
