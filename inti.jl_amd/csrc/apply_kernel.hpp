@@ -146,14 +146,163 @@ __device__ __forceinline__ void apply_block(const ApplyArgs &a, int64_t chunk, i
     }
 }
 
-template <int PDE, int DIM, int KIND, bool MIXED, int NR>
-__global__ __launch_bounds__(kApplyBS) void k_apply(ApplyArgs a) {
-    constexpr int TT = ApplyTile<NR>::TT;
+// ------------------------------------------- sixteen right-hand sides (MFMA)
+// Y (targets x 16) += K (targets x sources) Q (sources x 16) with the kernel
+// values made on the vector ALUs and the products on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64), which run beside the VALU: the 16 x 4 fragment of
+// K that one MFMA consumes is exactly one kernel value per lane. Lane l of a
+// wavefront evaluates target l % 16 of a 16-target fragment against source
+// l / 16 of the current group of 4 sources, which is the A-operand layout;
+// the B operand (source l / 16, right-hand side l % 16 of Q) is one LDS read.
+// A complex product takes four real MFMAs per fragment:
+//   Re Y += Re K Re Q + (-Im K) Im Q,   Im Y += Re K Im Q + Im K Re Q.
+// Each wavefront owns kMfmaNF fragments and a block 256 / (16 kMfmaNF)
+// wavefronts, so a block still covers 256 targets and the partial-sum layout
+// and grid are those of apply_block.
+// Accumulator layout: element v of lane l is Y[4 v + l / 16][l % 16].
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int kMfmaTT = 64;  // sources per LDS tile
+constexpr int kMfmaQLd = 17; // LDS row of one source's 16 densities, padded
+constexpr int kMfmaNF = 4;   // 16-target fragments per wavefront
+
+// threads of a block of the direct sum: 256 targets per block either way
+template <int NR>
+constexpr int apply_threads() { return NR == 16 ? 64 * (kApplyBS / 16) / kMfmaNF : kApplyBS; }
+
+template <int PDE, int DIM, int KIND, bool MIXED>
+__device__ __forceinline__ void apply_block_mfma(const ApplyArgs &a, int64_t chunk, int64_t tile,
+                                                 double2 *s_geo, double2 *s_q) {
+    constexpr bool CPLX = PDE == INTI_HELMHOLTZ;
+    constexpr int NR = 16, TT = kMfmaTT, NF = kMfmaNF, NT = apply_threads<16>();
     constexpr int NG = DIM == 3 ? 3 : 2;
-    __shared__ double2 s_geo[TT * NG];
-    __shared__ double2 s_q[TT * NR];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int64_t m = a.m, n = a.n;
+    const int64_t ts = a.tstride, ss = a.sstride;
+    const int64_t t0 = tile * kApplyBS + wave * 16 * NF;
+
+    double xt[NF][DIM], nt[NF][DIM];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const int64_t i = t0 + 16 * f + li;
+        const int64_t ic = i < m ? i : m - 1;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            xt[f][d] = a.tx[d * ts + ic];
+            nt[f][d] = a.tn ? a.tn[d * ts + ic] : 0.0;
+        }
+    }
+    v4d accr[NF], acci[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        accr[f] = v4d{0, 0, 0, 0};
+        acci[f] = v4d{0, 0, 0, 0};
+    }
+    const int64_t j0 = chunk * a.chunk;
+    const int64_t j1 = j0 + a.chunk < n ? j0 + a.chunk : n;
+    for (int64_t s0 = j0; s0 < j1; s0 += TT) {
+        if (tid < TT) {
+            const int64_t j = s0 + tid;
+            double y[3] = {kPadCoord, kPadCoord, kPadCoord}, ny[3] = {0, 0, 0};
+            if (j < j1) {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    y[d] = a.sx[d * ss + j];
+                    ny[d] = a.sn ? a.sn[d * ss + j] : 0.0;
+                }
+            }
+            double2 *rec = s_geo + tid * NG;
+            if constexpr (DIM == 3) {
+                rec[0] = make_double2(y[0], y[1]);
+                rec[1] = make_double2(y[2], ny[0]);
+                rec[2] = make_double2(ny[1], ny[2]);
+            } else {
+                rec[0] = make_double2(y[0], y[1]);
+                rec[1] = make_double2(ny[0], ny[1]);
+            }
+        }
+#pragma unroll
+        for (int q = tid; q < TT * NR; q += NT) {
+            const int jj = q % TT, r = q / TT;
+            const int64_t j = s0 + jj;
+            double qr = 0.0, qi = 0.0;
+            if (j < j1) {
+                const double w = a.sw[j];
+                if constexpr (CPLX) {
+                    const double *xp = a.x + 2 * (j + r * a.ldx);
+                    qr = w * xp[0];
+                    qi = w * xp[1];
+                } else {
+                    qr = w * a.x[j + r * a.ldx];
+                }
+            }
+            s_q[jj * kMfmaQLd + r] = make_double2(qr, qi);
+        }
+        __syncthreads();
+        for (int st = 0; st < TT / 4; ++st) {
+            const int jj = 4 * st + lk;
+            const double2 *rc = s_geo + jj * NG;
+            double ys[DIM], ns[DIM];
+            if constexpr (DIM == 3) {
+                const double2 v0 = rc[0], v1 = rc[1], v2 = rc[2];
+                ys[0] = v0.x; ys[1] = v0.y; ys[2] = v1.x;
+                ns[0] = v1.y; ns[1] = v2.x; ns[2] = v2.y;
+            } else {
+                const double2 v0 = rc[0], v1 = rc[1];
+                ys[0] = v0.x; ys[1] = v0.y;
+                ns[0] = v1.x; ns[1] = v1.y;
+            }
+            const double2 q = s_q[jj * kMfmaQLd + li];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                double kr, ki;
+                pair_raw_xy<PDE, DIM, KIND, MIXED>(xt[f], nt[f], ys, ns, a.p, kr, ki);
+                accr[f] = __builtin_amdgcn_mfma_f64_16x16x4f64(kr, q.x, accr[f], 0, 0, 0);
+                if constexpr (CPLX) {
+                    acci[f] = __builtin_amdgcn_mfma_f64_16x16x4f64(kr, q.y, acci[f], 0, 0, 0);
+                    accr[f] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ki, q.y, accr[f], 0, 0, 0);
+                    acci[f] = __builtin_amdgcn_mfma_f64_16x16x4f64(ki, q.x, acci[f], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t i = t0 + 16 * f + 4 * v + lk;
+            if (i < m) {
+                const int64_t off = ((int64_t)li * a.split + chunk) * m + i;
+                if constexpr (CPLX) reinterpret_cast<double2 *>(a.part)[off] = make_double2(accr[f][v], acci[f][v]);
+                else a.part[off] = accr[f][v];
+            }
+        }
+    }
+}
+
+// LDS of one block: the geometry of a source tile and its densities
+template <int DIM, int NR>
+struct ApplyLds {
+    static constexpr int NG = DIM == 3 ? 3 : 2;
+    static constexpr int GEO = (NR == 16 ? kMfmaTT : ApplyTile<NR>::TT) * NG;
+    static constexpr int Q = NR == 16 ? kMfmaTT * kMfmaQLd : ApplyTile<NR>::TT * NR;
+};
+
+template <int PDE, int DIM, int KIND, bool MIXED, int NR>
+__device__ __forceinline__ void apply_any(const ApplyArgs &a, int64_t chunk, int64_t tile, double2 *s_geo,
+                                          double2 *s_q) {
+    if constexpr (NR == 16) apply_block_mfma<PDE, DIM, KIND, MIXED>(a, chunk, tile, s_geo, s_q);
+    else apply_block<PDE, DIM, KIND, MIXED, NR>(a, chunk, tile, s_geo, s_q);
+}
+
+template <int PDE, int DIM, int KIND, bool MIXED, int NR>
+__global__ __launch_bounds__(apply_threads<NR>()) void k_apply(ApplyArgs a) {
+    __shared__ double2 s_geo[ApplyLds<DIM, NR>::GEO];
+    __shared__ double2 s_q[ApplyLds<DIM, NR>::Q];
     const int64_t b = blockIdx.x;
-    apply_block<PDE, DIM, KIND, MIXED, NR>(a, b % a.split, b / a.split, s_geo, s_q);
+    apply_any<PDE, DIM, KIND, MIXED, NR>(a, b % a.split, b / a.split, s_geo, s_q);
 }
 
 // ------------------------------------------------------------ batched apply
@@ -183,12 +332,10 @@ struct BatchArgs {
 };
 
 template <int PDE, int DIM, int KIND, bool MIXED, int NR>
-__global__ __launch_bounds__(kApplyBS) void k_apply_batch(BatchArgs b) {
-    constexpr int TT = ApplyTile<NR>::TT;
-    constexpr int NG = DIM == 3 ? 3 : 2;
+__global__ __launch_bounds__(apply_threads<NR>()) void k_apply_batch(BatchArgs b) {
     constexpr int CW = PDE == INTI_HELMHOLTZ ? 2 : 1;
-    __shared__ double2 s_geo[TT * NG];
-    __shared__ double2 s_q[TT * NR];
+    __shared__ double2 s_geo[ApplyLds<DIM, NR>::GEO];
+    __shared__ double2 s_q[ApplyLds<DIM, NR>::Q];
     for (int64_t it = blockIdx.x; it < b.nitems; it += gridDim.x) {
         const BatchItem w = b.items[it];
         const BatchProb P = b.probs[w.p];
@@ -201,7 +348,7 @@ __global__ __launch_bounds__(kApplyBS) void k_apply_batch(BatchArgs b) {
         a.part = b.part + CW * P.poff;
         a.split = P.split; a.chunk = P.chunk;
         a.p = P.kp;
-        apply_block<PDE, DIM, KIND, MIXED, NR>(a, w.chunk, w.tile, s_geo, s_q);
+        apply_any<PDE, DIM, KIND, MIXED, NR>(a, w.chunk, w.tile, s_geo, s_q);
     }
 }
 
@@ -209,7 +356,7 @@ using ApplyFn = void (*)(const ApplyArgs &, dim3, hipStream_t);
 
 template <int PDE, int DIM, int KIND, bool MIXED, int NR>
 static void apply_launcher(const ApplyArgs &a, dim3 grid, hipStream_t s) {
-    hipLaunchKernelGGL((k_apply<PDE, DIM, KIND, MIXED, NR>), grid, dim3(kApplyBS), 0, s, a);
+    hipLaunchKernelGGL((k_apply<PDE, DIM, KIND, MIXED, NR>), grid, dim3(apply_threads<NR>()), 0, s, a);
 }
 
 template <int PDE, int DIM, bool MIXED, int NR>
@@ -251,7 +398,7 @@ using BatchFn = void (*)(const BatchArgs &, dim3, hipStream_t);
 
 template <int PDE, int DIM, int KIND, bool MIXED, int NR>
 static void batch_launcher(const BatchArgs &a, dim3 grid, hipStream_t s) {
-    hipLaunchKernelGGL((k_apply_batch<PDE, DIM, KIND, MIXED, NR>), grid, dim3(kApplyBS), 0, s, a);
+    hipLaunchKernelGGL((k_apply_batch<PDE, DIM, KIND, MIXED, NR>), grid, dim3(apply_threads<NR>()), 0, s, a);
 }
 
 // batched kinds: SL, DL and c0 SL + c1 DL (the second-kind equations)
