@@ -766,11 +766,11 @@ def run_dense(inti, lib, ctx, args):
         ctx.synchronize()
         t1 = time.perf_counter()
         kmf = []
-        for _ in range(3):
+        for _ in range(reps):
             ctx.check(lib.inti_op_apply_device(Dop.handle, R, dx, n, dy, n, one, zero))
             kmf.append(ctx.last_kernel_ms())
         ctx.synchronize()
-        t_mf = (time.perf_counter() - t1) / 3
+        t_mf = (time.perf_counter() - t1) / reps
         Ym = _fetch(ctx, lib, dy, (R, n), np.complex128)
         ctx.check(lib.inti_device_free(ctx.handle, dx))
         ctx.check(lib.inti_device_free(ctx.handle, dy))
@@ -780,6 +780,8 @@ def run_dense(inti, lib, ctx, args):
                   "tflops": round(flops / (k * 1e-3) / 1e12, 2),
                   "matrix_GBps": round(16.0 * n * n / (k * 1e-3) / 1e9, 1),
                   "matrix_free_ms": round(t_mf * 1e3, 3),
+                  # the library's events time the last launch of 16 columns
+                  "matrix_free_kernel_ms_per_16": round(float(np.sum(kmf)) / reps, 3),
                   "rel_diff_vs_matrix_free": float(np.linalg.norm(Yg - Ym) / np.linalg.norm(Ym))}
         log(f"bench dense: R={R} {res[R]}")
     r16 = res[16]
