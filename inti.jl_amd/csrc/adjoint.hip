@@ -734,6 +734,121 @@ void plan(int64_t own, int64_t inner, int num_cu, int &split, int64_t &chunk) {
 
 }  // namespace
 
+// sigma_bar_j = conj(w_j post z_j) (complex) or w_j post z_j (real), z the
+// transposed sum's S partial slots added in slot order
+__global__ __launch_bounds__(256) void k_adj_trans_reduce(int64_t n, int64_t S, int cplx, const double *part,
+                                                          double pr, double pi, const double *w, double *sb) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    if (cplx) {
+        const double2 *pp = reinterpret_cast<const double2 *>(part);
+        double zr = 0.0, zi = 0.0;
+        for (int64_t q = 0; q < S; ++q) {
+            const double2 v = pp[q * n + j];
+            zr += v.x;
+            zi += v.y;
+        }
+        const double wj = w[j];
+        sb[2 * j] = wj * (pr * zr - pi * zi);
+        sb[2 * j + 1] = -wj * (pr * zi + pi * zr);
+    } else {
+        double z = 0.0;
+        for (int64_t q = 0; q < S; ++q) z += part[q * n + j];
+        sb[j] = w[j] * pr * z;
+    }
+}
+
+// y += cid sigma  =>  sigma_bar += conj(cid) ubar (host arrays)
+static void adjoint_identity(const Op *op, int cplx, int64_t n, const double *ubar, double *sigma_bar) {
+    if (op->cid[0] == 0.0 && op->cid[1] == 0.0) return;
+    for (int64_t j = 0; j < n; ++j) {
+        if (cplx) {
+            const double ur = ubar[2 * j], ui = ubar[2 * j + 1];
+            sigma_bar[2 * j] += op->cid[0] * ur + op->cid[1] * ui;
+            sigma_bar[2 * j + 1] += op->cid[0] * ui - op->cid[1] * ur;
+        } else {
+            sigma_bar[j] += op->cid[0] * ubar[j];
+        }
+    }
+}
+
+// sigma_bar alone of an operator on one quadrature (the product A^H ubar of
+// adjoint solves and density gradients): A_ij = post K(x_i, x_j) w_j, so
+// (A^H ubar)_j = conj(w_j post sum_i K(x_i, x_j) c_i) with c = conj(ubar),
+// the transposed symmetric sum (every unordered pair once, the kernel values
+// only; no derivatives). Returns -1 where it does not apply (the general
+// pullback then runs), else a status.
+static int adjoint_sigma_only(Op *op, const double *ubar, double *sigma_bar) {
+    Ctx *ctx = op->ctx;
+    if (op->trg != op->src || op->sym == 0 || op->kind >= INTI_FF_SL) return -1;
+    const int64_t n = op->src->n;
+    const int64_t T = (n + kSymBS - 1) / kSymBS;
+    if (T < 2) return -1;
+    if (op->sym == 1 && T * (T + 1) / 2 < 2LL * ctx->num_cu) return -1;
+    const int cplx = op_is_complex(op) ? 1 : 0;
+    const size_t cw = cplx ? 2 : 1;
+    int G = 0;
+    int64_t S = 0;
+    for (int g : {1, 2, 4}) {
+        if (op->sym_group && g != op->sym_group) continue;
+        S = (T + g - 1) / g;
+        if ((size_t)S * (size_t)n * cw * sizeof(double) <= kAdjSymMaxBytes) {
+            G = g;
+            break;
+        }
+    }
+    if (!G) return -1;
+    int kind = op->kind;
+    const KParams kp = op_params(op, &kind);
+    SymFn fn = nullptr;
+    if (op->pde == INTI_LAPLACE && op->dim == 2) fn = sym_trans_pick_lap2(kind, G);
+    if (op->pde == INTI_LAPLACE && op->dim == 3) fn = sym_trans_pick_lap3(kind, G);
+    if (op->pde == INTI_HELMHOLTZ && op->dim == 2) fn = sym_trans_pick_helm2(kind, G);
+    if (op->pde == INTI_HELMHOLTZ && op->dim == 3) fn = sym_trans_pick_helm3(kind, G);
+    if (!fn) return -1;
+
+    hipStream_t s = ctx->stream;
+    std::vector<double2> hc(n);  // conj(ubar), as the correction kernel reads it
+    for (int64_t i = 0; i < n; ++i) hc[i] = cplx ? make_double2(ubar[2 * i], -ubar[2 * i + 1]) : make_double2(ubar[i], 0.0);
+    std::vector<double> hx;
+    if (!cplx) {
+        hx.resize(n);
+        for (int64_t i = 0; i < n; ++i) hx[i] = ubar[i];
+    }
+    Dev dc, dx, dpart, dsb;
+    if (!dc.alloc(s, sizeof(double2) * n) || (!cplx && !dx.alloc(s, sizeof(double) * n)) ||
+        !dpart.alloc(s, sizeof(double) * cw * (size_t)S * (size_t)n) || !dsb.alloc(s, sizeof(double) * cw * n))
+        return set_error(ctx, INTI_ERR_NOMEM, "adjoint: allocation failed");
+    INTI_HIP(ctx, hipMemcpyAsync(dc.p, hc.data(), sizeof(double2) * n, hipMemcpyHostToDevice, s));
+    if (!cplx) INTI_HIP(ctx, hipMemcpyAsync(dx.p, hx.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    SymArgs a;
+    a.n = n;
+    a.xy = op->src->coords; a.nrm = op->src->normals; a.w = nullptr;  // q_i = c_i
+    a.x = cplx ? (const double *)dc.p : (const double *)dx.p;
+    a.part = (double *)dpart.p;
+    a.ntiles = T; a.ngroups = S;
+    a.distinct = op->src->distinct ? 1 : 0;
+    a.p = kp;
+    if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev0, s));
+    fn(a, S * (S + 1) / 2, s);
+    INTI_HIP(ctx, hipGetLastError());
+    if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev1, s));
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_adj_trans_reduce, dim3(nb), dim3(256), 0, s, n, S, cplx, (const double *)dpart.p, kp.postr,
+                       kp.posti, op->src->weights, (double *)dsb.p);
+    INTI_HIP(ctx, hipGetLastError());
+    if (op->corr) {
+        const Csr *cr = op->corr;
+        hipLaunchKernelGGL(k_adj_corr, dim3(nb), dim3(256), 0, s, n, cr->rowptr, cr->colind, cr->val, cplx,
+                           (const double2 *)dc.p, (double *)dsb.p);
+        INTI_HIP(ctx, hipGetLastError());
+    }
+    INTI_HIP(ctx, hipMemcpyAsync(sigma_bar, dsb.p, sizeof(double) * cw * n, hipMemcpyDeviceToHost, s));
+    INTI_HIP(ctx, hipStreamSynchronize(s));
+    adjoint_identity(op, cplx, n, ubar, sigma_bar);
+    return INTI_OK;
+}
+
 int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_bar, double *ybar,
                double *nbar, double *wbar, double *xbar, const double *dcoefs, double *kbar) {
     Ctx *ctx = op->ctx;
@@ -760,6 +875,10 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
         if (xbar) std::fill(xbar, xbar + dim * m, 0.0);
         if (kbar) *kbar = 0.0;
         return INTI_OK;
+    }
+    if (sigma_bar && !geo) {
+        const int st = adjoint_sigma_only(op, ubar, sigma_bar);
+        if (st >= 0) return st;
     }
     SrcFn fs = nullptr, ft = nullptr;
     if (op->pde == INTI_LAPLACE && dim == 2) pick<INTI_LAPLACE, 2>(kind, fs, ft);
@@ -892,18 +1011,7 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
     if (xbar) INTI_HIP(ctx, hipMemcpyAsync(xbar, dxb.p, sizeof(double) * dim * m, hipMemcpyDeviceToHost, s));
     if (kbar) INTI_HIP(ctx, hipMemcpyAsync(kp.data(), dkp.p, sizeof(double) * nsb, hipMemcpyDeviceToHost, s));
     INTI_HIP(ctx, hipStreamSynchronize(s));
-    if (sigma_bar && (op->cid[0] != 0.0 || op->cid[1] != 0.0) && m == n) {
-        // y += cid sigma  =>  sigma_bar += conj(cid) ubar
-        for (int64_t j = 0; j < n; ++j) {
-            if (cplx) {
-                const double ur = ubar[2 * j], ui = ubar[2 * j + 1];
-                sigma_bar[2 * j] += op->cid[0] * ur + op->cid[1] * ui;
-                sigma_bar[2 * j + 1] += op->cid[0] * ui - op->cid[1] * ur;
-            } else {
-                sigma_bar[j] += op->cid[0] * ubar[j];
-            }
-        }
-    }
+    if (sigma_bar && m == n) adjoint_identity(op, cplx, n, ubar, sigma_bar);
     if (kbar) {
         double sum = 0.0;
         for (double v : kp) sum += v;

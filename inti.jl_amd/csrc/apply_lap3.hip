@@ -6,4 +6,5 @@ namespace inti {
 ApplyFn apply_pick_lap3(int kind, int mixed, int nr) { return apply_pick_pd<INTI_LAPLACE, 3>(kind, mixed, nr); }
 BatchFn batch_pick_lap3(int kind, int mixed, int nr) { return batch_pick_pd<INTI_LAPLACE, 3>(kind, mixed, nr); }
 SymFn sym_pick_lap3(int kind, int mixed, int G) { return sym_pick_pd<INTI_LAPLACE, 3>(kind, mixed, G); }
+SymFn sym_trans_pick_lap3(int kind, int G) { return sym_trans_pick_pd<INTI_LAPLACE, 3>(kind, G); }
 }  // namespace inti

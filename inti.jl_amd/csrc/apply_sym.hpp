@@ -34,7 +34,7 @@ namespace inti {
 
 struct SymArgs {
     int64_t n;
-    const double *xy, *nrm, *w;  // SoA coordinates / normals (stride n), weights
+    const double *xy, *nrm, *w;  // SoA coordinates / normals (stride n), weights (null: all 1)
     const double *x;             // density (complex interleaved for Helmholtz)
     double *part;                // S x n partial sums (complex interleaved)
     int64_t ntiles, ngroups;     // T, S
@@ -56,8 +56,8 @@ __device__ __forceinline__ void sym_item(int64_t L, int64_t S, int64_t &a, int64
     b = r + (L - start(r));
 }
 
-// node j's geometry and q_j = w_j x_j; far-away padding with zero weight
-// beyond n
+// node j's geometry and q_j = w_j x_j (q_j = x_j without weights); far-away
+// padding with zero weight beyond n
 template <int DIM, bool CPLX>
 __device__ __forceinline__ void sym_node(const SymArgs &A, int64_t j, double *y, double *nv,
                                          double &qr, double &qi) {
@@ -67,7 +67,7 @@ __device__ __forceinline__ void sym_node(const SymArgs &A, int64_t j, double *y,
             y[d] = A.xy[d * A.n + j];
             nv[d] = A.nrm ? A.nrm[d * A.n + j] : 0.0;
         }
-        const double w = A.w[j];
+        const double w = A.w ? A.w[j] : 1.0;
         if constexpr (CPLX) {
             qr = w * A.x[2 * j];
             qi = w * A.x[2 * j + 1];
@@ -142,7 +142,9 @@ struct SymRec {
 // TPL targets per lane (one LDS read of a source then serves TPL pairs): a
 // block of 256 / TPL threads still covers a tile of 256 targets; thread t
 // holds targets t, t + 256 / TPL, ...
-template <int PDE, int DIM, int KIND, bool MIXED, int G, int TPL>
+// TRANS: the transposed sum z_j = sum_i K(x_i, x_j) q_i (the kernel matrix
+// transposed, for A^H ubar): the same pairs with the two values exchanged.
+template <int PDE, int DIM, int KIND, bool MIXED, int G, int TPL, bool TRANS = false>
 __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
     using R = SymRec<PDE, DIM, KIND>;
     constexpr bool CPLX = R::CPLX;
@@ -198,7 +200,8 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
 #pragma unroll
                         for (int t = 0; t < TPL; ++t) {
                             double kr, ki;
-                            pair_raw_xy<PDE, DIM, KIND, MIXED>(xi[t], ni[t], yj, nj, p, kr, ki);
+                            if constexpr (TRANS) pair_raw_xy<PDE, DIM, KIND, MIXED>(yj, nj, xi[t], ni[t], p, kr, ki);
+                            else pair_raw_xy<PDE, DIM, KIND, MIXED>(xi[t], ni[t], yj, nj, p, kr, ki);
                             if constexpr (CPLX) {
                                 adr[gi][t] = __builtin_fma(kr, qjr, adr[gi][t]);
                                 adr[gi][t] = __builtin_fma(-ki, qji, adr[gi][t]);
@@ -225,7 +228,10 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
 #pragma unroll
                                 for (int d = 0; d < DIM; ++d) r[d] = xi[t][d] - yj[d];
                                 double k1r, k1i, k2r, k2i;  // K(x_i, x_j), K(x_j, x_i)
-                                pair_raw_sym<PDE, DIM, KIND, MIXED, MASK>(r, ni[t], nj, p, k1r, k1i, k2r, k2i);
+                                if constexpr (TRANS)
+                                    pair_raw_sym<PDE, DIM, KIND, MIXED, MASK>(r, ni[t], nj, p, k2r, k2i, k1r, k1i);
+                                else
+                                    pair_raw_sym<PDE, DIM, KIND, MIXED, MASK>(r, ni[t], nj, p, k1r, k1i, k2r, k2i);
                                 if constexpr (CPLX) {
                                     adr[gi][t] = __builtin_fma(k1r, qjr, adr[gi][t]);
                                     adr[gi][t] = __builtin_fma(-k1i, qji, adr[gi][t]);
@@ -310,10 +316,10 @@ using SymFn = void (*)(const SymArgs &, int64_t, hipStream_t);
 // targets per lane (one LDS read of a source serves TPL pairs): 4 for the
 // cheap Laplace pairs, 2 for 3D Helmholtz, 1 for the 2D Helmholtz kernels,
 // whose Hankel evaluation already fills the register file (measured best)
-template <int PDE, int DIM, int KIND, bool MIXED, int G>
+template <int PDE, int DIM, int KIND, bool MIXED, int G, bool TRANS = false>
 static void sym_launcher(const SymArgs &a, int64_t nblocks, hipStream_t s) {
     constexpr int TPL = PDE == INTI_LAPLACE ? 4 : DIM == 2 ? 1 : 2;
-    hipLaunchKernelGGL((k_apply_sym<PDE, DIM, KIND, MIXED, G, TPL>), dim3((unsigned)nblocks),
+    hipLaunchKernelGGL((k_apply_sym<PDE, DIM, KIND, MIXED, G, TPL, TRANS>), dim3((unsigned)nblocks),
                        dim3(kSymBS / TPL), 0, s, a);
 }
 
@@ -344,6 +350,31 @@ static SymFn sym_pick_pd(int kind, int mixed, int G) {
     }
     return nullptr;
 }
+
+// transposed FP64 sums of the kinds with a pullback: SL is its own
+// transpose; DL and c0 SL + c1 DL need the exchanged values
+template <int PDE, int DIM, int G>
+static SymFn sym_trans_kind(int kind) {
+    switch (kind) {
+        case INTI_SL: return sym_launcher<PDE, DIM, INTI_SL, false, G>;
+        case INTI_DL: return sym_launcher<PDE, DIM, INTI_DL, false, G, true>;
+        case INTI_COMB_SD: return sym_launcher<PDE, DIM, INTI_COMB_SD, false, G, true>;
+    }
+    return nullptr;
+}
+
+template <int PDE, int DIM>
+static SymFn sym_trans_pick_pd(int kind, int G) {
+    if (G == 1) return sym_trans_kind<PDE, DIM, 1>(kind);
+    if (G == 2) return sym_trans_kind<PDE, DIM, 2>(kind);
+    if (G == 4) return sym_trans_kind<PDE, DIM, 4>(kind);
+    return nullptr;
+}
+
+SymFn sym_trans_pick_lap2(int kind, int G);
+SymFn sym_trans_pick_lap3(int kind, int G);
+SymFn sym_trans_pick_helm2(int kind, int G);
+SymFn sym_trans_pick_helm3(int kind, int G);
 
 SymFn sym_pick_lap2(int kind, int mixed, int G);
 SymFn sym_pick_lap3(int kind, int mixed, int G);

@@ -310,6 +310,60 @@ def test_gpu_symmetric_pullback(inti, oracle, pde, dim, kind, n):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("pde,dim,kind", SYM_CASES)
+@pytest.mark.parametrize("n", [700, 1300])
+def test_gpu_sigma_only_pullback_transposed_sum(inti, oracle, pde, dim, kind, n):
+    """sigma_bar alone (A^H ubar) of an operator on one quadrature: the
+    transposed symmetric sum (kernel values only) against the oracle (1e-12)
+    and the general pullback (1e-13), bit-identical on repeat."""
+    print(f"sigma-only pullback pde={pde} dim={dim} kind={kind} n={n}", flush=True)
+    k = 2.3 if pde == HEL else 0.0
+    _, y, nv, w, _, s = _setup(dim, 1, n, 31 + kind + n)
+    rng = np.random.default_rng(n + 7 * kind)
+    u = rng.normal(size=n) + 1j * rng.normal(size=n)
+    if pde == LAP:
+        u, s = u.real, s.real
+    op, K, dco = _kernel(inti, pde, dim, kind, k)
+    A = inti.IntegralOperator(K, _quad(inti, y, nv, w))
+    A.set_symmetric(2)
+    g = A.pullback(u, s)["sigma"]
+    assert np.array_equal(g, A.pullback(u, s)["sigma"])
+    B = inti.IntegralOperator(K, _quad(inti, y, nv, w))
+    B.set_symmetric(0)
+    h = B.pullback(u, s)["sigma"]
+    co, _ = _coefs(pde, kind, k)
+    o = oracle.adjoint(pde, dim, kind, k, Pts(y), Pts(y, nv, w), u, s, coefs=co, dcoefs=dco, want_xbar=False)
+    ref = o["sigma_bar"] if pde == HEL else o["sigma_bar"].real
+    assert np.linalg.norm(g - ref) <= 1e-12 * np.linalg.norm(ref)
+    assert np.linalg.norm(g - h) <= 1e-13 * np.linalg.norm(ref)
+
+
+@pytest.mark.gpu
+def test_gpu_sigma_only_pullback_of_the_corrected_operator_transposed(inti):
+    """The corrected CFIE I/2 + D - i k S (DIM correction, identity) with the
+    transposed symmetric sum: A^H ubar equals the general pullback's and the
+    assembled matrix's conjugate transpose applied to ubar."""
+    print("sigma-only pullback, corrected operator, transposed sum", flush=True)
+    k = 3.0
+    Q = inti.sphere_quadrature(4)
+    op = inti.Helmholtz(k=k, dim=3)
+    L, (dS, dD) = inti.combined_field(op=op, quad=Q, c_single=-1j * k, c_double=1.0, c_identity=0.5)
+    rng = np.random.default_rng(6)
+    u = rng.normal(size=len(Q)) + 1j * rng.normal(size=len(Q))
+    s = rng.normal(size=len(Q)) + 1j * rng.normal(size=len(Q))
+    L.iop.set_symmetric(2)
+    g = L.iop.pullback(u, s, wrt=("sigma",))["sigma"]
+    L.iop.set_symmetric(0)
+    h = L.iop.pullback(u, s, wrt=("sigma",))["sigma"]
+    Sm = inti.assemble_matrix(inti.IntegralOperator(inti.SingleLayerKernel(op), Q))
+    Dm = inti.assemble_matrix(inti.IntegralOperator(inti.DoubleLayerKernel(op), Q))
+    M = 0.5 * np.eye(len(Q)) + Dm + dD.to_scipy().toarray() - 1j * k * (Sm + dS.to_scipy().toarray())
+    ref = M.conj().T @ u
+    assert np.linalg.norm(g - ref) <= 1e-12 * np.linalg.norm(ref)
+    assert np.linalg.norm(g - h) <= 1e-13 * np.linalg.norm(ref)
+
+
+@pytest.mark.gpu
 def test_gpu_symmetric_pullback_automatic_on_a_large_operator(inti):
     """A 3D Helmholtz combined field on a 20,280-node sphere: the automatic
     mode takes the symmetric pullback (enough tile pairs to fill the GPU);
