@@ -772,80 +772,111 @@ static void adjoint_identity(const Op *op, int cplx, int64_t n, const double *ub
     }
 }
 
-// sigma_bar alone of an operator on one quadrature (the product A^H ubar of
-// adjoint solves and density gradients): A_ij = post K(x_i, x_j) w_j, so
-// (A^H ubar)_j = conj(w_j post sum_i K(x_i, x_j) c_i) with c = conj(ubar),
-// the transposed symmetric sum (every unordered pair once, the kernel values
-// only; no derivatives). Returns -1 where it does not apply (the general
-// pullback then runs), else a status.
+// sigma_bar alone (the product A^H ubar of adjoint solves and density
+// gradients): A_ij = post K(x_i, y_j) w_j, so (A^H ubar)_j =
+// conj(w_j post sum_i K(x_i, y_j) c_i) with c = conj(ubar), a sum of kernel
+// values only (no derivatives). An operator on one quadrature takes the
+// transposed symmetric sum (every unordered pair once), any other operator
+// the transposed one-direction sum (one thread per source, the targets
+// staged in LDS). Returns -1 where neither applies (the general pullback then
+// runs), else a status.
 static int adjoint_sigma_only(Op *op, const double *ubar, double *sigma_bar) {
     Ctx *ctx = op->ctx;
-    if (op->trg != op->src || op->sym == 0 || op->kind >= INTI_FF_SL) return -1;
-    const int64_t n = op->src->n;
-    const int64_t T = (n + kSymBS - 1) / kSymBS;
-    if (T < 2) return -1;
-    if (op->sym == 1 && T * (T + 1) / 2 < 2LL * ctx->num_cu) return -1;
+    const int64_t m = op->trg->n, n = op->src->n;
     const int cplx = op_is_complex(op) ? 1 : 0;
     const size_t cw = cplx ? 2 : 1;
-    int G = 0;
-    int64_t S = 0;
-    for (int g : {1, 2, 4}) {
-        if (op->sym_group && g != op->sym_group) continue;
-        S = (T + g - 1) / g;
-        if ((size_t)S * (size_t)n * cw * sizeof(double) <= kAdjSymMaxBytes) {
-            G = g;
-            break;
-        }
-    }
-    if (!G) return -1;
     int kind = op->kind;
     const KParams kp = op_params(op, &kind);
-    SymFn fn = nullptr;
-    if (op->pde == INTI_LAPLACE && op->dim == 2) fn = sym_trans_pick_lap2(kind, G);
-    if (op->pde == INTI_LAPLACE && op->dim == 3) fn = sym_trans_pick_lap3(kind, G);
-    if (op->pde == INTI_HELMHOLTZ && op->dim == 2) fn = sym_trans_pick_helm2(kind, G);
-    if (op->pde == INTI_HELMHOLTZ && op->dim == 3) fn = sym_trans_pick_helm3(kind, G);
-    if (!fn) return -1;
+
+    // symmetric plan: tiles of 256 in groups of G
+    SymFn fsym = nullptr;
+    int64_t T = (n + kSymBS - 1) / kSymBS, S = 0;
+    if (op->trg == op->src && op->sym && kind < INTI_FF_SL && T >= 2 &&
+        (op->sym == 2 || T * (T + 1) / 2 >= 2LL * ctx->num_cu)) {
+        int G = 0;
+        for (int g : {1, 2, 4}) {
+            if (op->sym_group && g != op->sym_group) continue;
+            S = (T + g - 1) / g;
+            if ((size_t)S * (size_t)n * cw * sizeof(double) <= kAdjSymMaxBytes) {
+                G = g;
+                break;
+            }
+        }
+        if (G && op->pde == INTI_LAPLACE && op->dim == 2) fsym = sym_trans_pick_lap2(kind, G);
+        if (G && op->pde == INTI_LAPLACE && op->dim == 3) fsym = sym_trans_pick_lap3(kind, G);
+        if (G && op->pde == INTI_HELMHOLTZ && op->dim == 2) fsym = sym_trans_pick_helm2(kind, G);
+        if (G && op->pde == INTI_HELMHOLTZ && op->dim == 3) fsym = sym_trans_pick_helm3(kind, G);
+    }
+    // one-direction plan: the sum's targets are the n sources (tiles of
+    // 256), its sources the m targets cut into `split` chunks
+    ApplyFn fone = nullptr;
+    int split = 0;
+    int64_t chunk = 0;
+    const int64_t ntiles = (n + kApplyBS - 1) / kApplyBS;
+    if (!fsym) {
+        if (op->pde == INTI_LAPLACE && op->dim == 2) fone = apply_trans_pick_lap2(kind);
+        if (op->pde == INTI_LAPLACE && op->dim == 3) fone = apply_trans_pick_lap3(kind);
+        if (op->pde == INTI_HELMHOLTZ && op->dim == 2) fone = apply_trans_pick_helm2(kind);
+        if (op->pde == INTI_HELMHOLTZ && op->dim == 3) fone = apply_trans_pick_helm3(kind);
+        if (!fone) return -1;
+        const int64_t stiles = (m + kApplyBS - 1) / kApplyBS;
+        int64_t sp = (8LL * ctx->num_cu + ntiles - 1) / ntiles;
+        if (sp >= 8) sp = (sp + 7) / 8 * 8;
+        if (sp > stiles) sp = stiles;
+        if (sp < 1) sp = 1;
+        chunk = (stiles + sp - 1) / sp * kApplyBS;
+        split = (int)((m + chunk - 1) / chunk);
+        S = split;
+    }
 
     hipStream_t s = ctx->stream;
-    std::vector<double2> hc(n);  // conj(ubar), as the correction kernel reads it
-    for (int64_t i = 0; i < n; ++i) hc[i] = cplx ? make_double2(ubar[2 * i], -ubar[2 * i + 1]) : make_double2(ubar[i], 0.0);
+    std::vector<double2> hc(m);  // conj(ubar), as the correction kernel reads it
+    for (int64_t i = 0; i < m; ++i) hc[i] = cplx ? make_double2(ubar[2 * i], -ubar[2 * i + 1]) : make_double2(ubar[i], 0.0);
     std::vector<double> hx;
-    if (!cplx) {
-        hx.resize(n);
-        for (int64_t i = 0; i < n; ++i) hx[i] = ubar[i];
-    }
+    if (!cplx) hx.assign(ubar, ubar + m);
     Dev dc, dx, dpart, dsb;
-    if (!dc.alloc(s, sizeof(double2) * n) || (!cplx && !dx.alloc(s, sizeof(double) * n)) ||
+    if (!dc.alloc(s, sizeof(double2) * m) || (!cplx && !dx.alloc(s, sizeof(double) * m)) ||
         !dpart.alloc(s, sizeof(double) * cw * (size_t)S * (size_t)n) || !dsb.alloc(s, sizeof(double) * cw * n))
         return set_error(ctx, INTI_ERR_NOMEM, "adjoint: allocation failed");
-    INTI_HIP(ctx, hipMemcpyAsync(dc.p, hc.data(), sizeof(double2) * n, hipMemcpyHostToDevice, s));
-    if (!cplx) INTI_HIP(ctx, hipMemcpyAsync(dx.p, hx.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
-    SymArgs a;
-    a.n = n;
-    a.xy = op->src->coords; a.nrm = op->src->normals; a.w = nullptr;  // q_i = c_i
-    a.x = cplx ? (const double *)dc.p : (const double *)dx.p;
-    a.part = (double *)dpart.p;
-    a.ntiles = T; a.ngroups = S;
-    a.distinct = op->src->distinct ? 1 : 0;
-    a.p = kp;
+    INTI_HIP(ctx, hipMemcpyAsync(dc.p, hc.data(), sizeof(double2) * m, hipMemcpyHostToDevice, s));
+    if (!cplx) INTI_HIP(ctx, hipMemcpyAsync(dx.p, hx.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+    const double *xc = cplx ? (const double *)dc.p : (const double *)dx.p;  // c, unweighted
     if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev0, s));
-    fn(a, S * (S + 1) / 2, s);
+    if (fsym) {
+        SymArgs a;
+        a.n = n;
+        a.xy = op->src->coords; a.nrm = op->src->normals; a.w = nullptr;
+        a.x = xc;
+        a.part = (double *)dpart.p;
+        a.ntiles = T; a.ngroups = S;
+        a.distinct = op->src->distinct ? 1 : 0;
+        a.p = kp;
+        fsym(a, S * (S + 1) / 2, s);
+    } else {
+        ApplyArgs a;
+        a.m = n; a.n = m;
+        a.tstride = n; a.sstride = m;
+        a.tx = op->src->coords; a.tn = op->src->normals;
+        a.sx = op->trg->coords; a.sn = op->trg->normals; a.sw = nullptr;
+        a.x = xc; a.ldx = m;
+        a.part = (double *)dpart.p; a.split = split; a.chunk = chunk;
+        a.p = kp;
+        fone(a, dim3((unsigned)(ntiles * split)), s);
+    }
     INTI_HIP(ctx, hipGetLastError());
     if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev1, s));
-    const unsigned nb = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_adj_trans_reduce, dim3(nb), dim3(256), 0, s, n, S, cplx, (const double *)dpart.p, kp.postr,
-                       kp.posti, op->src->weights, (double *)dsb.p);
+    hipLaunchKernelGGL(k_adj_trans_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, S, cplx,
+                       (const double *)dpart.p, kp.postr, kp.posti, op->src->weights, (double *)dsb.p);
     INTI_HIP(ctx, hipGetLastError());
     if (op->corr) {
         const Csr *cr = op->corr;
-        hipLaunchKernelGGL(k_adj_corr, dim3(nb), dim3(256), 0, s, n, cr->rowptr, cr->colind, cr->val, cplx,
-                           (const double2 *)dc.p, (double *)dsb.p);
+        hipLaunchKernelGGL(k_adj_corr, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m, cr->rowptr, cr->colind,
+                           cr->val, cplx, (const double2 *)dc.p, (double *)dsb.p);
         INTI_HIP(ctx, hipGetLastError());
     }
     INTI_HIP(ctx, hipMemcpyAsync(sigma_bar, dsb.p, sizeof(double) * cw * n, hipMemcpyDeviceToHost, s));
     INTI_HIP(ctx, hipStreamSynchronize(s));
-    adjoint_identity(op, cplx, n, ubar, sigma_bar);
+    if (m == n) adjoint_identity(op, cplx, n, ubar, sigma_bar);
     return INTI_OK;
 }
 

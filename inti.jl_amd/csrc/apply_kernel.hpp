@@ -39,8 +39,11 @@ struct ApplyTile {
 
 // The work of one block: targets [tile*256, tile*256 + 256) against the
 // sources of chunk `chunk`; partial sums to a.part. s_geo / s_q are the
-// block's LDS staging buffers.
-template <int PDE, int DIM, int KIND, bool MIXED, int NR>
+// block's LDS staging buffers. Without weights (a.sw null) q_j = x_j.
+// TRANS: the sum of K(source, target) instead of K(target, source), i.e.
+// the kernel matrix of the operator whose targets and sources are these
+// sources and targets, transposed (for A^H ubar).
+template <int PDE, int DIM, int KIND, bool MIXED, int NR, bool TRANS = false>
 __device__ __forceinline__ void apply_block(const ApplyArgs &a, int64_t chunk, int64_t tile,
                                             double2 *s_geo, double2 *s_q) {
     constexpr bool CPLX = PDE == INTI_HELMHOLTZ;
@@ -90,7 +93,7 @@ __device__ __forceinline__ void apply_block(const ApplyArgs &a, int64_t chunk, i
             const int64_t j = s0 + jj;
             double qr = 0.0, qi = 0.0;
             if (j < j1) {
-                const double w = a.sw[j];
+                const double w = a.sw ? a.sw[j] : 1.0;
                 if constexpr (CPLX) {
                     const double *xp = a.x + 2 * (j + r * a.ldx);
                     qr = w * xp[0];
@@ -120,7 +123,8 @@ __device__ __forceinline__ void apply_block(const ApplyArgs &a, int64_t chunk, i
                 ns[0] = v1.x; ns[1] = v1.y;
             }
             double kr, ki;
-            pair_raw_xy<PDE, DIM, KIND, MIXED>(xt, nt, ys, ns, a.p, kr, ki);
+            if constexpr (TRANS) pair_raw_xy<PDE, DIM, KIND, MIXED>(ys, ns, xt, nt, a.p, kr, ki);
+            else pair_raw_xy<PDE, DIM, KIND, MIXED>(xt, nt, ys, ns, a.p, kr, ki);
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 const double2 q = s_q[jj * NR + r];
@@ -290,19 +294,20 @@ struct ApplyLds {
     static constexpr int Q = NR == 16 ? kMfmaTT * kMfmaQLd : ApplyTile<NR>::TT * NR;
 };
 
-template <int PDE, int DIM, int KIND, bool MIXED, int NR>
+template <int PDE, int DIM, int KIND, bool MIXED, int NR, bool TRANS = false>
 __device__ __forceinline__ void apply_any(const ApplyArgs &a, int64_t chunk, int64_t tile, double2 *s_geo,
                                           double2 *s_q) {
+    static_assert(!(TRANS && NR == 16), "the transposed sum has one right-hand side");
     if constexpr (NR == 16) apply_block_mfma<PDE, DIM, KIND, MIXED>(a, chunk, tile, s_geo, s_q);
-    else apply_block<PDE, DIM, KIND, MIXED, NR>(a, chunk, tile, s_geo, s_q);
+    else apply_block<PDE, DIM, KIND, MIXED, NR, TRANS>(a, chunk, tile, s_geo, s_q);
 }
 
-template <int PDE, int DIM, int KIND, bool MIXED, int NR>
+template <int PDE, int DIM, int KIND, bool MIXED, int NR, bool TRANS = false>
 __global__ __launch_bounds__(apply_threads<NR>()) void k_apply(ApplyArgs a) {
     __shared__ double2 s_geo[ApplyLds<DIM, NR>::GEO];
     __shared__ double2 s_q[ApplyLds<DIM, NR>::Q];
     const int64_t b = blockIdx.x;
-    apply_any<PDE, DIM, KIND, MIXED, NR>(a, b % a.split, b / a.split, s_geo, s_q);
+    apply_any<PDE, DIM, KIND, MIXED, NR, TRANS>(a, b % a.split, b / a.split, s_geo, s_q);
 }
 
 // ------------------------------------------------------------ batched apply
@@ -354,9 +359,27 @@ __global__ __launch_bounds__(apply_threads<NR>()) void k_apply_batch(BatchArgs b
 
 using ApplyFn = void (*)(const ApplyArgs &, dim3, hipStream_t);
 
-template <int PDE, int DIM, int KIND, bool MIXED, int NR>
+template <int PDE, int DIM, int KIND, bool MIXED, int NR, bool TRANS = false>
 static void apply_launcher(const ApplyArgs &a, dim3 grid, hipStream_t s) {
-    hipLaunchKernelGGL((k_apply<PDE, DIM, KIND, MIXED, NR>), grid, dim3(apply_threads<NR>()), 0, s, a);
+    hipLaunchKernelGGL((k_apply<PDE, DIM, KIND, MIXED, NR, TRANS>), grid, dim3(apply_threads<NR>()), 0, s, a);
+}
+
+// transposed FP64 sums (one right-hand side) of the kinds with a pullback
+template <int PDE, int DIM>
+static ApplyFn apply_trans_pick_pd(int kind) {
+    switch (kind) {
+        case INTI_SL: return apply_launcher<PDE, DIM, INTI_SL, false, 1, true>;
+        case INTI_DL: return apply_launcher<PDE, DIM, INTI_DL, false, 1, true>;
+        case INTI_COMB_SD: return apply_launcher<PDE, DIM, INTI_COMB_SD, false, 1, true>;
+    }
+    if constexpr (PDE == INTI_HELMHOLTZ) {
+        switch (kind) {
+            case INTI_FF_SL: return apply_launcher<PDE, DIM, INTI_FF_SL, false, 1, true>;
+            case INTI_FF_DL: return apply_launcher<PDE, DIM, INTI_FF_DL, false, 1, true>;
+            case INTI_FF_COMB: return apply_launcher<PDE, DIM, INTI_FF_COMB, false, 1, true>;
+        }
+    }
+    return nullptr;
 }
 
 template <int PDE, int DIM, bool MIXED, int NR>
@@ -430,6 +453,11 @@ BatchFn batch_pick_lap2(int kind, int mixed, int nr);
 BatchFn batch_pick_lap3(int kind, int mixed, int nr);
 BatchFn batch_pick_helm2(int kind, int mixed, int nr);
 BatchFn batch_pick_helm3(int kind, int mixed, int nr);
+
+ApplyFn apply_trans_pick_lap2(int kind);
+ApplyFn apply_trans_pick_lap3(int kind);
+ApplyFn apply_trans_pick_helm2(int kind);
+ApplyFn apply_trans_pick_helm3(int kind);
 
 ApplyFn apply_pick_lap2(int kind, int mixed, int nr);
 ApplyFn apply_pick_lap3(int kind, int mixed, int nr);

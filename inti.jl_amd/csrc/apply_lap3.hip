@@ -7,4 +7,5 @@ ApplyFn apply_pick_lap3(int kind, int mixed, int nr) { return apply_pick_pd<INTI
 BatchFn batch_pick_lap3(int kind, int mixed, int nr) { return batch_pick_pd<INTI_LAPLACE, 3>(kind, mixed, nr); }
 SymFn sym_pick_lap3(int kind, int mixed, int G) { return sym_pick_pd<INTI_LAPLACE, 3>(kind, mixed, G); }
 SymFn sym_trans_pick_lap3(int kind, int G) { return sym_trans_pick_pd<INTI_LAPLACE, 3>(kind, G); }
+ApplyFn apply_trans_pick_lap3(int kind) { return apply_trans_pick_pd<INTI_LAPLACE, 3>(kind); }
 }  // namespace inti

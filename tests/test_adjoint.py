@@ -168,6 +168,30 @@ def test_gpu_pullback_matches_oracle(inti, oracle, pde, dim, kind):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("pde,dim,kind", CASES)
+@pytest.mark.parametrize("m,n", [(300, 700), (1300, 257)])
+def test_gpu_sigma_only_pullback_one_direction(inti, oracle, pde, dim, kind, m, n):
+    """sigma_bar alone of an operator whose targets are not its sources (and
+    of the far fields): the transposed one-direction sum (kernel values only,
+    one thread per source) against the oracle, 1e-12; bit-identical on
+    repeat; ragged tiles on both sides."""
+    print(f"sigma-only pullback one direction pde={pde} dim={dim} kind={kind} m={m} n={n}", flush=True)
+    ff = kind >= FF_SL
+    k = 2.3 if pde == HEL else 0.0
+    x, y, nv, w, u, s = _setup(dim, m, n, 17 + kind + m, farfield=ff)
+    if pde == LAP:
+        u, s = u.real, s.real
+    op, K, dco = _kernel(inti, pde, dim, kind, k)
+    A = inti.IntegralOperator(K, inti.points(x), _quad(inti, y, nv, w))
+    g = A.pullback(u, s)["sigma"]
+    assert np.array_equal(g, A.pullback(u, s)["sigma"])
+    co, _ = _coefs(pde, kind, k)
+    o = oracle.adjoint(pde, dim, kind, k, Pts(x), Pts(y, nv, w), u, s, coefs=co, dcoefs=dco, want_xbar=False)
+    ref = o["sigma_bar"] if pde == HEL else o["sigma_bar"].real
+    assert np.linalg.norm(g - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.gpu
 def test_gpu_farfield_objective_gradient_by_finite_differences(inti):
     """The shape-gradient objective at a small size: sound-soft-like density
     sigma on a sphere perturbed by 0.1 sin(3 theta) cos(2 phi), k = 10,
