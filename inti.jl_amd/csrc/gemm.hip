@@ -21,6 +21,7 @@
 // needs many wavefronts in flight, and a narrow product (16 right-hand
 // sides: 32 columns) alone gives one block per 128 rows.
 #include <algorithm>
+#include <cmath>
 
 #include "common.hpp"
 
@@ -203,12 +204,35 @@ int dense_gemm(Ctx *ctx, int cplx, int64_t m, int64_t n, int nrhs, const double 
     const int64_t N = (cols + COLS - 1) / COLS * COLS;
     const int64_t M = cplx ? 2 * m : m;
     const int64_t K = n;
-    // K slices: enough blocks for 4 per CU, slices of whole LDS chunks
+    // K slices of whole LDS chunks. Blocks run in rounds of `cap` resident
+    // blocks (occupancy x CUs), so a grid just past a multiple of cap leaves
+    // the last round nearly empty; every slice also writes and re-reads an
+    // M x N partial product. The slice count minimises the modelled traffic
+    //   A bytes x (rounds x cap / blocks) + 16 M N nsl.
     const int64_t base_blocks = (M + kGmRows - 1) / kGmRows * (N / COLS);
     const int64_t kchunks = std::max<int64_t>(1, (K + KC - 1) / KC);
-    int64_t nsl = std::min<int64_t>(kchunks, std::max<int64_t>(1, (4LL * ctx->num_cu + base_blocks - 1) / base_blocks));
-    const int64_t kslice = (kchunks + nsl - 1) / nsl * KC;
-    nsl = std::max<int64_t>(1, (K + kslice - 1) / kslice);
+    int per_cu = 0;
+    const void *kfn = NC == 2 ? (const void *)k_gemm_f64<2> : NC == 4 ? (const void *)k_gemm_f64<4>
+                                                                        : (const void *)k_gemm_f64<8>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    const double cap = (double)per_cu * ctx->num_cu;
+    const double abytes = 8.0 * (double)M * (double)K;
+    int64_t nsl = 1, kslice = kchunks * KC;
+    double best = -1.0;
+    for (int64_t t = 1; t <= std::min<int64_t>(kchunks, 64); ++t) {
+        const int64_t ks = (kchunks + t - 1) / t * KC;
+        const int64_t ns = std::max<int64_t>(1, (K + ks - 1) / ks);
+        // the partials stay well below the matrix itself
+        if (ns > 1 && 8.0 * (double)M * (double)N * (double)ns > std::max(abytes / 4.0, 64.0e6)) break;
+        const double blocks = (double)(base_blocks * ns);
+        const double rounds = std::ceil(blocks / cap);
+        const double cost = abytes * rounds * cap / blocks + 16.0 * (double)M * (double)N * (double)ns;
+        if (best < 0.0 || cost < best) {
+            best = cost;
+            nsl = ns;
+            kslice = ks;
+        }
+    }
     const size_t bbytes = sizeof(double) * (size_t)std::max<int64_t>(K, 1) * N;
     const size_t zbytes = sizeof(double) * (size_t)M * N * nsl;
     void *scr = nullptr;
