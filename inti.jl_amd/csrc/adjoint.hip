@@ -861,7 +861,7 @@ static int adjoint_sigma_only(Op *op, const double *ubar, double *sigma_bar) {
         a.x = xc; a.ldx = m;
         a.part = (double *)dpart.p; a.split = split; a.chunk = chunk;
         a.p = kp;
-        fone(a, dim3((unsigned)(ntiles * split)), s);
+        fone(&a, dim3((unsigned)(ntiles * split)), s);
     }
     INTI_HIP(ctx, hipGetLastError());
     if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev1, s));

@@ -357,11 +357,25 @@ __global__ __launch_bounds__(apply_threads<NR>()) void k_apply_batch(BatchArgs b
     }
 }
 
-using ApplyFn = void (*)(const ApplyArgs &, dim3, hipStream_t);
+// Launches the kernel; with a null `a` it launches nothing and returns the
+// kernel's resident blocks per CU (for the launch plan).
+using ApplyFn = int (*)(const ApplyArgs *a, dim3, hipStream_t);
 
 template <int PDE, int DIM, int KIND, bool MIXED, int NR, bool TRANS = false>
-static void apply_launcher(const ApplyArgs &a, dim3 grid, hipStream_t s) {
-    hipLaunchKernelGGL((k_apply<PDE, DIM, KIND, MIXED, NR, TRANS>), grid, dim3(apply_threads<NR>()), 0, s, a);
+static int apply_launcher(const ApplyArgs *a, dim3 grid, hipStream_t s) {
+    if (!a) {
+        static int cached = 0;  // one gfx950 occupancy per kernel; a race only repeats the query
+        if (cached == 0) {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &nb, (const void *)k_apply<PDE, DIM, KIND, MIXED, NR, TRANS>, apply_threads<NR>(), 0) != hipSuccess)
+                nb = 1;
+            cached = nb < 1 ? 1 : nb;
+        }
+        return cached;
+    }
+    hipLaunchKernelGGL((k_apply<PDE, DIM, KIND, MIXED, NR, TRANS>), grid, dim3(apply_threads<NR>()), 0, s, *a);
+    return 0;
 }
 
 // transposed FP64 sums (one right-hand side) of the kinds with a pullback

@@ -180,20 +180,41 @@ KParams op_params(const Op *op, int *eff_kind) {
 
 // number of source chunks: enough workgroups to fill 256 CUs several
 // times over, a multiple of the 8 XCDs when more than 8.
-static void plan_split(const Op *op, int64_t m, int64_t n, int &split, int64_t &chunk) {
-    // chunks are whole 256-source tiles, which also covers the smaller
-    // tiles of the multi-right-hand-side kernels
+// Source chunks of the one-direction sum: block b works on target tile
+// b / split and chunk b % split. The base split gives about 8 blocks per CU
+// in whole 256-source tiles, a multiple of 8 from 8 on (all blocks of one XCD
+// then share a chunk in their L2); an explicit split (inti_op_set_launch)
+// replaces it. Without one, the plan then looks at splits from half to twice
+// the base (multiples of 8 from 8 on) with chunks of whole LDS source tiles of
+// the kernel (256, 128 or 64 sources for 1, 4 or 16 right-hand sides) and
+// takes the one with the least rounds x chunk, where blocks run in rounds of
+// `cap` resident blocks (occupancy x CUs): a grid just past a multiple of cap
+// leaves its last round nearly empty.
+static void plan_split(const Op *op, int64_t m, int64_t n, ApplyFn fn, int nr, int &split, int64_t &chunk) {
     const int64_t ntiles = (m + kApplyBS - 1) / kApplyBS;
     const int64_t stiles = (n + kApplyBS - 1) / kApplyBS;
-    int64_t s = op->split;
-    if (s <= 0) {
-        const int64_t want = 8LL * op->ctx->num_cu;
-        s = (want + ntiles - 1) / ntiles;
-        if (s >= 8) s = (s + 7) / 8 * 8;
+    int64_t s0 = op->split;
+    if (s0 <= 0) {
+        s0 = (8LL * op->ctx->num_cu + ntiles - 1) / ntiles;
+        if (s0 >= 8) s0 = (s0 + 7) / 8 * 8;
     }
-    if (s > stiles) s = stiles;
-    if (s < 1) s = 1;
-    chunk = (stiles + s - 1) / s * kApplyBS;
+    s0 = std::max<int64_t>(1, std::min(s0, stiles));
+    chunk = (stiles + s0 - 1) / s0 * kApplyBS;
+    if (op->split <= 0 && fn && n > 0) {
+        const int64_t gran = nr == 16 ? kMfmaTT : nr == 4 ? ApplyTile<4>::TT : ApplyTile<1>::TT;
+        const int64_t gtiles = (n + gran - 1) / gran;
+        const double cap = (double)fn(nullptr, dim3(0), nullptr) * op->ctx->num_cu;
+        auto cost_of = [&](int64_t c) { return std::ceil((double)(ntiles * ((n + c - 1) / c)) / cap) * (double)c; };
+        double best = cost_of(chunk);  // the base plan stays unless a candidate beats it
+        for (int64_t s = std::max<int64_t>(1, s0 / 2); s <= std::min(2 * s0, gtiles); s += s < 8 ? 1 : 8) {
+            const int64_t c = (gtiles + s - 1) / s * gran;
+            const double cost = cost_of(c);
+            if (cost < best) {
+                best = cost;
+                chunk = c;
+            }
+        }
+    }
     split = (int)((n + chunk - 1) / chunk);
     if (split < 1) split = 1;
 }
@@ -201,14 +222,18 @@ static void plan_split(const Op *op, int64_t m, int64_t n, int &split, int64_t &
 int apply_reserve(const Op *op, int nrhs) {
     const int cw = op_is_complex(op) ? 2 : 1;
     const int64_t m = op->trg->n, n = op->src->n;
-    int split;
-    int64_t chunk;
-    plan_split(op, m, n, split, chunk);
-    const int nrmax = nrhs >= 16 ? 16 : nrhs >= 4 ? 4 : 1;
-    size_t bytes = sizeof(double) * cw * (size_t)m * split * nrmax;
     int kind = op->kind, G;
     int64_t T, S;
     (void)op_params(op, &kind);
+    // the largest partial buffer of the column blocks (16, 4, 1) a call uses
+    size_t bytes = 0;
+    for (int nr : {16, 4, 1}) {
+        if (nrhs < nr && nr != 1) continue;
+        int split;
+        int64_t chunk;
+        plan_split(op, m, n, pick_apply(op->pde, op->dim, kind, op->precision == INTI_MIXED, nr), nr, split, chunk);
+        bytes = std::max(bytes, sizeof(double) * cw * (size_t)m * split * nr);
+    }
     if (nrhs >= 1 && plan_sym(op, kind, 1, G, T, S))
         bytes = std::max(bytes, sizeof(double) * cw * (size_t)S * (size_t)n);
     void *scr = nullptr;
@@ -224,12 +249,10 @@ int launch_apply(const DirectLaunch &L) {
     if (m == 0 || L.nrhs == 0) return INTI_OK;
     int kind = op->kind;
     const KParams kp = op_params(op, &kind);
-    int split;
-    int64_t chunk;
-    plan_split(op, m, n, split, chunk);
-    const int nrmax = L.nrhs >= 16 ? 16 : L.nrhs >= 4 ? 4 : 1;
     void *scr = nullptr;
-    int st = ctx_scratch(ctx, kScrPartial, sizeof(double) * cw * (size_t)m * split * nrmax, &scr);
+    int st = apply_reserve(op, L.nrhs);
+    if (st) return st;
+    st = ctx_scratch(ctx, kScrPartial, 0, &scr);
     if (st) return st;
     double *part = (double *)scr;
     const int64_t ntiles = (m + kApplyBS - 1) / kApplyBS;
@@ -247,10 +270,14 @@ int launch_apply(const DirectLaunch &L) {
         const int nr = left >= 16 ? 16 : left >= 4 ? 4 : 1;
         const double *xr = L.x + (int64_t)r0 * L.ldx * cw;
         double *yr = L.y + (int64_t)r0 * L.ldy * cw;
+        ApplyFn fn = sym && nr == 1 ? nullptr : pick_apply(op->pde, op->dim, kind, op->precision == INTI_MIXED, nr);
+        int split = 1;
+        int64_t chunk = n;
+        if (fn) plan_split(op, m, n, fn, nr, split, chunk);
         int nsplit = split;
         if (n > 0 && nr == 1 && sym) {
-            SymFn fn = pick_sym(op->pde, op->dim, kind, op->precision == INTI_MIXED, symG);
-            if (!fn) return set_error(ctx, INTI_ERR_UNSUPPORTED, "no symmetric kernel for this operator");
+            SymFn sfn = pick_sym(op->pde, op->dim, kind, op->precision == INTI_MIXED, symG);
+            if (!sfn) return set_error(ctx, INTI_ERR_UNSUPPORTED, "no symmetric kernel for this operator");
             SymArgs a;
             a.n = n;
             a.xy = op->src->coords; a.nrm = op->src->normals; a.w = op->src->weights;
@@ -259,12 +286,11 @@ int launch_apply(const DirectLaunch &L) {
             a.distinct = op->src->distinct ? 1 : 0;
             a.p = kp;
             if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-            fn(a, symS * (symS + 1) / 2, ctx->stream);
+            sfn(a, symS * (symS + 1) / 2, ctx->stream);
             INTI_HIP(ctx, hipGetLastError());
             if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
             nsplit = (int)symS;
         } else if (n > 0) {
-            ApplyFn fn = pick_apply(op->pde, op->dim, kind, op->precision == INTI_MIXED, nr);
             if (!fn) return set_error(ctx, INTI_ERR_UNSUPPORTED, "no direct-sum kernel for this operator");
             ApplyArgs a;
             a.m = m; a.n = n;
@@ -275,7 +301,7 @@ int launch_apply(const DirectLaunch &L) {
             a.p = kp;
             dim3 grid((unsigned)(ntiles * split));
             if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-            fn(a, grid, ctx->stream);
+            fn(&a, grid, ctx->stream);
             INTI_HIP(ctx, hipGetLastError());
             if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         }
