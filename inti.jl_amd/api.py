@@ -13,6 +13,7 @@ correction: {"method": "none"} or {"method": "dim", ...} (api.jl:96-158);
 import numpy as np
 
 from .bdim import DimParameters, bdim_correction, etype_to_nearest_points
+from .dense import DenseOperator
 from .kernels import (AdjointDoubleLayerKernel, CombinedFieldKernel, DoubleLayerKernel,
                       Helmholtz, HyperSingularKernel, SingleLayerKernel)
 from .nystrom import (DoubleLayerPotential, IntegralOperator, SingleLayerPotential,
@@ -93,12 +94,12 @@ def single_double_layer(*, op, target, source, compression=None, correction=None
     if correction["method"] == "dim":
         dS, dD = _dim_corrections(op, target, source, Sop, Dop, correction, derivative)
     if compression["method"] == "none":
-        Smat = assemble_matrix(Sop)
-        Dmat = assemble_matrix(Dop)
-        if dS is not None:
-            Smat = Smat + dS.to_scipy()
-            Dmat = Dmat + dD.to_scipy()
-        return np.asarray(Smat), np.asarray(Dmat)
+        # assemble_matrix and axpy!(true, dS, Smat) (api.jl:84-85,171-173) on
+        # the device, one download per matrix
+        if dS is None:
+            return assemble_matrix(Sop), assemble_matrix(Dop)
+        return (DenseOperator(Sop, corrections=[(dS, 1.0)]).to_numpy(),
+                DenseOperator(Dop, corrections=[(dD, 1.0)]).to_numpy())
     if dS is not None:
         Sop.attach(dS)
         Dop.attach(dD)
