@@ -134,9 +134,10 @@ __device__ __forceinline__ void helm3_raw_mixed(const double *r, double d2, cons
     const bool ok = !MASK || d2 > kSamePointTol2;
     const float y32 = __builtin_amdgcn_rsqf(MASK ? fmaxf((float)d2, 1.0e-30f) : (float)d2);
     const double y = (double)y32;
+    // kappa d from one Newton step on the estimate e = d2 y of d:
+    // d = e (1 + h / 2) with h = 1 - e y, i.e. kappa d = (kappa / 2) e (3 - e y)
     const double e = d2 * y;
-    const double h = __builtin_fma(-e, y, 1.0);
-    const double t = __builtin_fma(p.kappa_half * e, h, p.kappa * e);
+    const double t = (p.kappa_half * e) * __builtin_fma(-e, y, 3.0);
     const float u = (float)__builtin_amdgcn_fract(t);
     const float ym = ok ? y32 : 0.0f;
     const double cf = (double)(__builtin_amdgcn_cosf(u) * ym);
@@ -324,6 +325,70 @@ __device__ __forceinline__ void pair_raw_sym(const double *r, const double *ni, 
         pair_raw<PDE, DIM, KIND, MIXED, MASK>(r, ni, nj, p, k1r, k1i);
         pair_raw<PDE, DIM, KIND, MIXED, MASK>(rn, nj, ni, p, k2r, k2i);
     }
+}
+
+// Fused value-and-accumulate of both directions of one unordered pair for
+// the 3D Helmholtz DL and DL + ch*SL kinds. Their raw value times a density
+// q factors as
+//   raw q = (E/d) [ch q + a (1/d - i k) q],   a = (r . n_s) / d,
+// with E/d = e^{ikd}/d shared by the two directions, so with ch q, k Im q
+// and -k Re q staged per node beside q the source term costs two FMAs for
+// w = (1/d - ik) q, two for ch q + a w and four for the complex accumulation
+// (instead of forming the complex kernel value first). r = x_i - x_j; the
+// direction (i <- j) adds to (a1r, a1i) with the source normal nj and q_j,
+// the direction (j <- i) to (a2r, a2i) with ni and q_i. TRANS accumulates
+// the transposed sums (K(x_j, x_i) q_j at i, K(x_i, x_j) q_i at j).
+struct QExt {
+    double qr, qi, cr, ci, kqi, mkqr;  // q, ch q, k Im q, -k Re q
+};
+__device__ __forceinline__ QExt qext(double qr, double qi, const KParams &p) {
+    QExt q;
+    q.qr = qr; q.qi = qi;
+    q.cr = p.chr * qr - p.chi * qi;
+    q.ci = p.chr * qi + p.chi * qr;
+    q.kqi = p.k * qi;
+    q.mkqr = -p.k * qr;
+    return q;
+}
+template <bool MIXED, bool MASK, bool TRANS>
+__device__ __forceinline__ void helm3_sym_fused(const double *r, const double *ni, const double *nj,
+                                                const KParams &p, const QExt &qj, const QExt &qi,
+                                                double &a1r, double &a1i, double &a2r, double &a2i) {
+    const double d2 = dotv<3>(r, r);
+    double id, cf, sf;
+    if constexpr (MIXED) {
+        const bool ok = !MASK || d2 > kSamePointTol2;
+        const float y32 = __builtin_amdgcn_rsqf(MASK ? fmaxf((float)d2, 1.0e-30f) : (float)d2);
+        id = (double)y32;
+        const double e = d2 * id;
+        const double t = (p.kappa_half * e) * __builtin_fma(-e, id, 3.0);  // kappa d, one Newton step
+        const float u = (float)__builtin_amdgcn_fract(t);
+        const float ym = ok ? y32 : 0.0f;
+        cf = (double)(__builtin_amdgcn_cosf(u) * ym);
+        sf = (double)(__builtin_amdgcn_sinf(u) * ym);
+    } else {
+        const bool ok = !MASK || d2 > kSamePointTol2;
+        const double d2s = MASK ? fmax(d2, kClampD2) : d2;
+        id = rsqp<false>(d2s);
+        double s, c;
+        sincos_rev<false>(p.kappa * (d2s * id), s, c);
+        const double f = MASK ? mask(ok, id) : id;
+        cf = c * f;
+        sf = s * f;
+    }
+    const double aj = dotv<3>(r, nj) * id;    // K(x_i, x_j): source normal nj
+    const double ai = -dotv<3>(r, ni) * id;   // K(x_j, x_i): source normal ni, r -> -r
+    const double b1 = TRANS ? ai : aj, b2 = TRANS ? aj : ai;
+    // (i <- j): q_j
+    const double w1r = __builtin_fma(id, qj.qr, qj.kqi), w1i = __builtin_fma(id, qj.qi, qj.mkqr);
+    const double z1r = __builtin_fma(b1, w1r, qj.cr), z1i = __builtin_fma(b1, w1i, qj.ci);
+    a1r = __builtin_fma(cf, z1r, __builtin_fma(-sf, z1i, a1r));
+    a1i = __builtin_fma(cf, z1i, __builtin_fma(sf, z1r, a1i));
+    // (j <- i): q_i
+    const double w2r = __builtin_fma(id, qi.qr, qi.kqi), w2i = __builtin_fma(id, qi.qi, qi.mkqr);
+    const double z2r = __builtin_fma(b2, w2r, qi.cr), z2i = __builtin_fma(b2, w2i, qi.ci);
+    a2r = __builtin_fma(cf, z2r, __builtin_fma(-sf, z2i, a2r));
+    a2i = __builtin_fma(cf, z2i, __builtin_fma(sf, z2r, a2i));
 }
 
 inline KParams basic_params(double k) {
