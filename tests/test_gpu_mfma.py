@@ -121,3 +121,24 @@ def test_sixteen_rhs_source_chunks(inti, oracle, pde, dim, kind):
         A.ctx.check(A.ctx.lib.inti_op_set_launch(A.handle, 256, 1, split))
         assert _colerr(A.apply(X[:, :17]), ref[:, :17]) <= 1e-12, split
         assert _colerr(A.apply(X), ref) <= 1e-12, split
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (5, 3), (17, 64), (64, 5), (300, 4)])
+def test_sixteen_rhs_tiny_sizes(inti, oracle, m, n):
+    """Fewer targets than a fragment and fewer sources than a step of 4:
+    clamped target rows, padded sources, 16 and 19 columns; and the
+    sigma-only pullback (transposed one-direction sum) at the same sizes."""
+    say(f"[gpu] 16 rhs tiny m={m} n={n}")
+    src = cloud(inti, 3, n, 31 + n)
+    trg = cloud(inti, 3, m, 41 + m, shift=0.07)
+    K, coefs = _kernel(inti, HEL, 3, 4, 2.0)
+    X = _dens(HEL, n, 19, 5)
+    ref = oracle.matvec(HEL, 3, 4, 2.0, trg, src, X, coefs=coefs)
+    for prec, tol in (("fp64", 1e-12), ("mixed", 1e-6)):
+        A = inti.IntegralOperator(K, trg, src, precision=prec)
+        assert _colerr(A.apply(X[:, :16]), ref[:, :16]) <= tol, prec
+        assert _colerr(A.apply(X), ref) <= tol, prec
+    u = np.resize(_dens(HEL, max(m, n), 1, 6)[:, 0], m)
+    g = inti.IntegralOperator(K, trg, src).pullback(u, X[:, 0])["sigma"]
+    o = oracle.adjoint(HEL, 3, 4, 2.0, trg, src, u, X[:, 0], coefs=coefs, dcoefs=(0.0, 0.0), want_xbar=False)
+    assert np.linalg.norm(g - o["sigma_bar"]) <= 1e-12 * np.linalg.norm(o["sigma_bar"])
