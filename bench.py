@@ -64,13 +64,20 @@ def cpu_baseline(oracle, Q, x, k, target_seconds=15.0):
     rows = 16
     coefs = (-1j * k, 1.0)
     t = 0.0
+    # grow a calibration sample to about a second, then run once at the size
+    # that takes about target_seconds
     while True:
         t0 = time.perf_counter()
         oracle.matvec(1, 3, 4, k, Q, Q, x, coefs=coefs, rows=(0, rows), nthreads=threads)
         t = time.perf_counter() - t0
-        if t > target_seconds / 4 or rows >= n:
+        if t > 1.0 or rows >= n:
             break
-        rows = min(n, int(rows * max(2.0, target_seconds / 4 / max(t, 1e-3))))
+        rows = min(n, rows * 4)
+    if rows < n:
+        rows = min(n, int(rows * target_seconds / max(t, 1e-3)))
+        t0 = time.perf_counter()
+        oracle.matvec(1, 3, 4, k, Q, Q, x, coefs=coefs, rows=(0, rows), nthreads=threads)
+        t = time.perf_counter() - t0
     rate = rows * n / t
     return {"value": rate / 1e9, "unit": "Gpair-evals/s", "cores": threads, "kind": "port",
             "sample": f"oracle FP64 direct sum of the same combined-field operator, rows 0..{rows} "
@@ -358,7 +365,7 @@ def run_batch(inti, lib, ctx, args):
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         p = int(np.argmax(sizes))
         Q = qs[p]
-        rows = 64
+        rows = len(Q)  # the whole largest problem, one right-hand side (a few seconds)
         t1 = time.perf_counter()
         oracle.matvec(1, 2, 1, ks[p], Q, Q, Xh[B64.offsets[p]:B64.offsets[p + 1], 0], rows=(0, rows),
                       nthreads=threads)
