@@ -210,6 +210,14 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
         }
         __syncthreads();
         const bool trans = !diag || gj > 0;  // some tile I < J feeds tile J
+        // with one target tile per block its nodes are read once per source
+        // tile, not once per 64-source sub-tile
+        double xh[TPL][3], nh[TPL][3], qhr[TPL], qhi[TPL];
+        if constexpr (G == 1) {
+#pragma unroll
+            for (int t = 0; t < TPL; ++t)
+                sym_node<DIM, CPLX>(A, a * kSymBS + tid + t * BS, xh[t], nh[t], qhr[t], qhi[t]);
+        }
         for (int u = 0; u < 4; ++u) {
             double ajr = 0.0, aji = 0.0;  // running transposed sum of source (lane + s) mod 64
 #pragma unroll
@@ -218,8 +226,16 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                 if (I >= T || (diag && gi > gj)) continue;
                 double xi[TPL][3], ni[TPL][3], qir[TPL], qii[TPL];
 #pragma unroll
-                for (int t = 0; t < TPL; ++t)
-                    sym_node<DIM, CPLX>(A, I * kSymBS + tid + t * BS, xi[t], ni[t], qir[t], qii[t]);
+                for (int t = 0; t < TPL; ++t) {
+                    if constexpr (G == 1) {
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) { xi[t][d] = xh[t][d]; ni[t][d] = nh[t][d]; }
+                        qir[t] = qhr[t];
+                        qii[t] = qhi[t];
+                    } else {
+                        sym_node<DIM, CPLX>(A, I * kSymBS + tid + t * BS, xi[t], ni[t], qir[t], qii[t]);
+                    }
+                }
                 if (diag && gi == gj) {
                     // the tile against itself: every ordered pair once
                     for (int s = 0; s < 64; ++s) {
