@@ -850,6 +850,7 @@ static int adjoint_sigma_only(Op *op, const double *ubar, double *sigma_bar) {
         a.part = (double *)dpart.p;
         a.ntiles = T; a.ngroups = S;
         a.distinct = op->src->distinct ? 1 : 0;
+        a.ch_ik = 0;  // the transposed sums are FP64 (no staged ch q)
         a.p = kp;
         fsym(a, S * (S + 1) / 2, s);
     } else {

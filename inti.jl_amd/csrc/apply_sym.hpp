@@ -39,6 +39,7 @@ struct SymArgs {
     double *part;                // S x n partial sums (complex interleaved)
     int64_t ntiles, ngroups;     // T, S
     int distinct;                // no two nodes within twice the coincidence tolerance
+    int ch_ik;                   // combined field with ch = -i k (ch q = (k Im q, -k Re q))
     KParams p;
 };
 
@@ -153,12 +154,23 @@ struct SymRecF {
         s[4][j] = make_double2(q.cr, q.ci);
         s[5][j] = make_double2(q.kqi, q.mkqr);
     }
+    // CHIK: ch = -i k, so ch q equals the (k Im q, -k Re q) slot and slot 4
+    // is not read
+    template <bool CHIK = false>
     __device__ static __forceinline__ void get(const double2 (*s)[kSymBS], int j, double *y, double *nv,
                                                QExt &q) {
-        const double2 v0 = s[0][j], v1 = s[1][j], v2 = s[2][j], v3 = s[3][j], v4 = s[4][j], v5 = s[5][j];
+        const double2 v0 = s[0][j], v1 = s[1][j], v2 = s[2][j], v3 = s[3][j], v5 = s[5][j];
         y[0] = v0.x; y[1] = v0.y; y[2] = v1.x;
         nv[0] = v1.y; nv[1] = v2.x; nv[2] = v2.y;
-        q.qr = v3.x; q.qi = v3.y; q.cr = v4.x; q.ci = v4.y; q.kqi = v5.x; q.mkqr = v5.y;
+        q.qr = v3.x; q.qi = v3.y; q.kqi = v5.x; q.mkqr = v5.y;
+        if constexpr (CHIK) {
+            q.cr = v5.x;
+            q.ci = v5.y;
+        } else {
+            const double2 v4 = s[4][j];
+            q.cr = v4.x;
+            q.ci = v4.y;
+        }
     }
 };
 
@@ -243,7 +255,7 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                         double yj[3], nj[3], qjr, qji;
                         if constexpr (FUSED) {
                             QExt qx;
-                            R::get(s_src, jl, yj, nj, qx);
+                            R::template get<false>(s_src, jl, yj, nj, qx);
                             qjr = qx.qr;
                             qji = qx.qi;
                         } else {
@@ -267,8 +279,9 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                 } else {
                     // off-diagonal tiles of a quadrature with distinct nodes hold
                     // no coincident pairs: the pair values need no mask there
-                    auto pairs = [&](auto mask_tag) {
+                    auto pairs = [&](auto mask_tag, auto chik_tag) {
                         constexpr bool MASK = decltype(mask_tag)::value;
+                        constexpr bool CHIK = decltype(chik_tag)::value;
                         if constexpr (FUSED) {
                             QExt qit[TPL];
 #pragma unroll
@@ -278,7 +291,7 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                                 const int jl = u * 64 + ((lane + s) & 63);
                                 double yj[3], nj[3];
                                 QExt qj;
-                                R::get(s_src, jl, yj, nj, qj);
+                                R::template get<CHIK>(s_src, jl, yj, nj, qj);
 #pragma unroll
                                 for (int t = 0; t < TPL; ++t) {
                                     double r[3];
@@ -325,8 +338,14 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                             }
                         }
                     };
-                    if (A.distinct) pairs(std::false_type{});
-                    else pairs(std::true_type{});
+                    if constexpr (FUSED) {
+                        if (A.distinct && A.ch_ik) pairs(std::false_type{}, std::true_type{});
+                        else if (A.distinct) pairs(std::false_type{}, std::false_type{});
+                        else pairs(std::true_type{}, std::false_type{});
+                    } else {
+                        if (A.distinct) pairs(std::false_type{}, std::false_type{});
+                        else pairs(std::true_type{}, std::false_type{});
+                    }
                 }
             }
             if (trans) {

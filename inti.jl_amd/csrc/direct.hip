@@ -284,6 +284,12 @@ int launch_apply(const DirectLaunch &L) {
             a.x = xr; a.part = part;
             a.ntiles = symT; a.ngroups = symS;
             a.distinct = op->src->distinct ? 1 : 0;
+            // c0 SL + c1 DL with c0 = -i k c1 (the combined field of the
+            // reference's examples): ch q needs no LDS slot of its own
+            a.ch_ik = kind == INTI_COMB_SD && op->pde == INTI_HELMHOLTZ && op->dim == 3 &&
+                              std::fabs(kp.chr) <= 1e-15 * kp.k && std::fabs(kp.chi + kp.k) <= 1e-15 * kp.k
+                          ? 1
+                          : 0;
             a.p = kp;
             if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
             sfn(a, symS * (symS + 1) / 2, ctx->stream);

@@ -142,3 +142,27 @@ def test_symmetric_with_duplicate_nodes(inti, oracle, pde, dim, kind):
         y = A @ x
         assert np.all(np.isfinite(y))
         assert np.linalg.norm(y - ref) <= tol * np.linalg.norm(ref), prec
+
+
+@pytest.mark.parametrize("n", [700, 1300, 2305])
+def test_symmetric_mixed_cfie_minus_ik(inti, oracle, n):
+    """The combined field D - i k S (c0 = -i k c1, the examples' CFIE) in
+    mixed precision: the symmetric kernel's staged ch q then shares the
+    k q slot; against the oracle (1e-6) and the one-direction mixed sum."""
+    say(f"[gpu] symmetric mixed CFIE c0 = -ik n={n}")
+    Q = cloud(inti, 3, n, 50 + n)
+    k = 3.5
+    op = inti.Helmholtz(k=k, dim=3)
+    for c1 in (1.0, 0.7 - 0.2j):
+        c0 = -1j * k * c1
+        K = inti.CombinedFieldKernel(op, c0, c1, "SD")
+        x = _density(HEL, n, n + 3)
+        ref = oracle.matvec(HEL, 3, 4, k, Q, Q, x, coefs=(c0, c1))
+        A = inti.IntegralOperator(K, Q, Q, precision="mixed")
+        A.set_symmetric(2)
+        y = A @ x
+        assert np.linalg.norm(y - ref) <= 1e-6 * np.linalg.norm(ref), c1
+        assert np.array_equal(y, A @ x)
+        B = inti.IntegralOperator(K, Q, Q, precision="mixed")
+        B.set_symmetric(0)
+        assert np.linalg.norm(y - B @ x) <= 1e-6 * np.linalg.norm(ref), c1
