@@ -109,6 +109,44 @@ class BatchOperator:
                                                         C.byref(ms)))
         return {"chunk_len": c.value, "grid": g.value, "ms": ms.value}
 
+    def _system_norms(self, X):
+        """2-norm of every system's block (problem p, column r): (nprob, nrhs)."""
+        sq = np.abs(X) ** 2
+        cs = np.concatenate([np.zeros((1, sq.shape[1])), np.cumsum(sq, axis=0)])
+        return np.sqrt(np.maximum(cs[self.offsets[1:]] - cs[self.offsets[:-1]], 0.0))
+
+    def refine_solve(self, lo, B, X0=None, *, restart=100, maxiter_inner=100, inner_rtol=1e-6,
+                     max_outer=10, rtol=1e-10, use_graph=True):
+        """Iterative refinement of every system of A X = B: FP64 residuals
+        with this operator, corrections by the batched GMRES of `lo` (the
+        same problems, e.g. in mixed precision). Systems whose residual is
+        at rtol get a zero right-hand side (no further corrections).
+        Returns (X, info) with info["relres"] of shape (nprob, nrhs)."""
+        if lo.nprob != self.nprob or lo.ntot != self.ntot:
+            raise ValueError("refine_solve: the two batches must hold the same problems")
+        B = np.asarray(B, dtype=self.dtype)
+        vec = B.ndim == 1
+        B2 = B.reshape(self.ntot, -1)
+        X = np.zeros_like(B2) if X0 is None else np.array(np.asarray(X0, dtype=self.dtype).reshape(self.ntot, -1))
+        bn = self._system_norms(B2)
+        bn[bn == 0] = 1.0
+        inner, outer = 0, 0
+        while True:
+            R = B2 - self.apply(X).reshape(self.ntot, -1)
+            rel = self._system_norms(R) / bn
+            if outer >= max_outer or np.all(rel <= rtol):
+                break
+            done = rel <= rtol
+            for p in range(self.nprob):  # converged systems: zero right-hand side
+                if done[p].any():
+                    R[self.offsets[p]:self.offsets[p + 1], done[p]] = 0
+            D, info = lo.gmres(R, restart=restart, maxiter=maxiter_inner, rtol=inner_rtol, use_graph=use_graph)
+            X += D.reshape(self.ntot, -1)
+            inner += info["iterations"]
+            outer += 1
+        X = X.reshape(-1) if vec else X
+        return X, {"outer": outer, "inner_iterations": inner, "relres": rel}
+
     def gmres(self, B, X0=None, *, restart=30, maxiter=100, rtol=1e-6, atol=0.0, use_graph=True):
         """Solve every system (p, r) of A X = B; returns (X, info) with
         info["relres"] of shape (nprob, nrhs)."""

@@ -110,6 +110,38 @@ def test_batch_gmres_matches_dense_solves(inti, use_graph):
           f"max relres {info['relres'].max():.2e}", flush=True)
 
 
+def test_batch_refined_solve_matches_dense_solves(inti):
+    """Iterative refinement of a batch (FP64 residuals, GMRES corrections
+    with the mixed-precision batch of the same problems): every system
+    against numpy's dense solve of its problem (1e-10 relative) with every
+    FP64 residual below 1e-12."""
+    print("batch refined solve", flush=True)
+    qs, ks = _ellipses(inti, [20, 70, 35, 90, 50], seed=8)
+    ops = [inti.Helmholtz(k=k, dim=2) for k in ks]
+    Ks = [inti.DoubleLayerKernel(o) for o in ops]
+    B64 = inti.BatchOperator(Ks, qs)
+    B32 = inti.BatchOperator(Ks, qs, precision="mixed")
+    dense = []
+    for Bt in (B64, B32):
+        Bt.set_identity(0.5)
+    for p, (o, Q) in enumerate(zip(ops, qs)):
+        Sop = inti.IntegralOperator(inti.SingleLayerKernel(o), Q)
+        Dop = inti.IntegralOperator(inti.DoubleLayerKernel(o), Q)
+        dS, dD = inti.bdim_correction(o, Q, Q, Sop, Dop, green_multiplier=np.full(len(Q), -0.5))
+        B64.attach(p, dD)
+        B32.attach(p, dD)
+        dense.append(0.5 * np.eye(len(Q)) + inti.assemble_matrix(Dop) + dD.to_scipy().toarray())
+    b = _rand(B64.ntot, 16, True, 9)
+    X, info = B64.refine_solve(B32, b, restart=60, maxiter_inner=200, inner_rtol=1e-6, rtol=1e-12)
+    print(f"batch refined: {info['outer']} outer, {info['inner_iterations']} inner, "
+          f"max relres {info['relres'].max():.2e}", flush=True)
+    assert info["relres"].max() <= 1e-12
+    assert info["outer"] >= 2  # the mixed corrections alone stop near 1e-7
+    for p, M in enumerate(dense):
+        ref = np.linalg.solve(M, B64.split(b)[p])
+        assert np.linalg.norm(B64.split(X)[p] - ref) <= 1e-10 * np.linalg.norm(ref), p
+
+
 def test_batch_gmres_graph_is_deterministic(inti):
     print("batch gmres determinism", flush=True)
     qs, ks = _ellipses(inti, [30, 64, 45], seed=11)
