@@ -37,6 +37,15 @@ struct ApplyTile {
     static constexpr int TT = NR == 1 ? 256 : NR <= 4 ? 128 : 64;
 };
 
+// One right-hand side of the mixed 3D Helmholtz DL / DL + ch*SL kinds: the
+// pair value and its accumulation are fused (helm3_fused_one), with q, ch q
+// and (k Im q, -k Re q) staged per source (three double2 slots).
+template <int PDE, int DIM, int KIND, bool MIXED, int NR, bool TRANS>
+constexpr bool apply_fused() {
+    return NR == 1 && !TRANS && MIXED && PDE == INTI_HELMHOLTZ && DIM == 3 &&
+           (KIND == INTI_DL || KIND == INTI_COMB_SD);
+}
+
 // The work of one block: targets [tile*256, tile*256 + 256) against the
 // sources of chunk `chunk`; partial sums to a.part. s_geo / s_q are the
 // block's LDS staging buffers. Without weights (a.sw null) q_j = x_j.
@@ -47,6 +56,7 @@ template <int PDE, int DIM, int KIND, bool MIXED, int NR, bool TRANS = false>
 __device__ __forceinline__ void apply_block(const ApplyArgs &a, int64_t chunk, int64_t tile,
                                             double2 *s_geo, double2 *s_q) {
     constexpr bool CPLX = PDE == INTI_HELMHOLTZ;
+    constexpr bool FUSED = apply_fused<PDE, DIM, KIND, MIXED, NR, TRANS>();
     constexpr int BS = kApplyBS;
     constexpr int TT = ApplyTile<NR>::TT;
     constexpr int NG = DIM == 3 ? 3 : 2;  // double2 slots of a geometry record
@@ -102,7 +112,14 @@ __device__ __forceinline__ void apply_block(const ApplyArgs &a, int64_t chunk, i
                     qr = w * a.x[j + r * a.ldx];
                 }
             }
-            s_q[jj * NR + r] = make_double2(qr, qi);
+            if constexpr (FUSED) {
+                const QExt qx = qext(qr, qi, a.p);
+                s_q[3 * jj] = make_double2(qx.qr, qx.qi);
+                s_q[3 * jj + 1] = make_double2(qx.cr, qx.ci);
+                s_q[3 * jj + 2] = make_double2(qx.kqi, qx.mkqr);
+            } else {
+                s_q[jj * NR + r] = make_double2(qr, qi);
+            }
         }
         __syncthreads();
         // two pairs in flight for one right-hand side; one for many (the
@@ -121,6 +138,16 @@ __device__ __forceinline__ void apply_block(const ApplyArgs &a, int64_t chunk, i
                 const double2 v0 = rc[0], v1 = rc[1];
                 ys[0] = v0.x; ys[1] = v0.y;
                 ns[0] = v1.x; ns[1] = v1.y;
+            }
+            if constexpr (FUSED) {
+                const double2 v0 = s_q[3 * jj], v1 = s_q[3 * jj + 1], v2 = s_q[3 * jj + 2];
+                QExt qx;
+                qx.qr = v0.x; qx.qi = v0.y; qx.cr = v1.x; qx.ci = v1.y; qx.kqi = v2.x; qx.mkqr = v2.y;
+                double r3[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) r3[d] = xt[d] - ys[d];
+                helm3_fused_one(r3, ns, a.p, qx, accr[0], acci[0]);
+                continue;
             }
             double kr, ki;
             if constexpr (TRANS) pair_raw_xy<PDE, DIM, KIND, MIXED>(ys, ns, xt, nt, a.p, kr, ki);
@@ -286,12 +313,13 @@ __device__ __forceinline__ void apply_block_mfma(const ApplyArgs &a, int64_t chu
     }
 }
 
-// LDS of one block: the geometry of a source tile and its densities
-template <int DIM, int NR>
+// LDS of one block: the geometry of a source tile and its densities (three
+// slots per source for the fused pair)
+template <int DIM, int NR, bool FUSED = false>
 struct ApplyLds {
     static constexpr int NG = DIM == 3 ? 3 : 2;
     static constexpr int GEO = (NR == 16 ? kMfmaTT : ApplyTile<NR>::TT) * NG;
-    static constexpr int Q = NR == 16 ? kMfmaTT * kMfmaQLd : ApplyTile<NR>::TT * NR;
+    static constexpr int Q = NR == 16 ? kMfmaTT * kMfmaQLd : ApplyTile<NR>::TT * NR * (FUSED ? 3 : 1);
 };
 
 template <int PDE, int DIM, int KIND, bool MIXED, int NR, bool TRANS = false>
@@ -304,8 +332,9 @@ __device__ __forceinline__ void apply_any(const ApplyArgs &a, int64_t chunk, int
 
 template <int PDE, int DIM, int KIND, bool MIXED, int NR, bool TRANS = false>
 __global__ __launch_bounds__(apply_threads<NR>()) void k_apply(ApplyArgs a) {
-    __shared__ double2 s_geo[ApplyLds<DIM, NR>::GEO];
-    __shared__ double2 s_q[ApplyLds<DIM, NR>::Q];
+    using L = ApplyLds<DIM, NR, apply_fused<PDE, DIM, KIND, MIXED, NR, TRANS>()>;
+    __shared__ double2 s_geo[L::GEO];
+    __shared__ double2 s_q[L::Q];
     const int64_t b = blockIdx.x;
     apply_any<PDE, DIM, KIND, MIXED, NR, TRANS>(a, b % a.split, b / a.split, s_geo, s_q);
 }
@@ -339,8 +368,9 @@ struct BatchArgs {
 template <int PDE, int DIM, int KIND, bool MIXED, int NR>
 __global__ __launch_bounds__(apply_threads<NR>()) void k_apply_batch(BatchArgs b) {
     constexpr int CW = PDE == INTI_HELMHOLTZ ? 2 : 1;
-    __shared__ double2 s_geo[ApplyLds<DIM, NR>::GEO];
-    __shared__ double2 s_q[ApplyLds<DIM, NR>::Q];
+    using L = ApplyLds<DIM, NR, apply_fused<PDE, DIM, KIND, MIXED, NR, false>()>;
+    __shared__ double2 s_geo[L::GEO];
+    __shared__ double2 s_q[L::Q];
     for (int64_t it = blockIdx.x; it < b.nitems; it += gridDim.x) {
         const BatchItem w = b.items[it];
         const BatchProb P = b.probs[w.p];

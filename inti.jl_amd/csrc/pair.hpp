@@ -391,6 +391,28 @@ __device__ __forceinline__ void helm3_sym_fused(const double *r, const double *n
     a2i = __builtin_fma(cf, z2i, __builtin_fma(sf, z2r, a2i));
 }
 
+// One direction of the fused 3D Helmholtz DL / DL + ch*SL pair (target x,
+// source y with normal ny, r = x - y), mixed precision, coincident points
+// masked: acc += raw q = (E/d) [ch q + a (1/d - i k) q], a = (r . ny) / d.
+__device__ __forceinline__ void helm3_fused_one(const double *r, const double *ny, const KParams &p,
+                                                const QExt &q, double &accr, double &acci) {
+    const double d2 = dotv<3>(r, r);
+    const bool ok = d2 > kSamePointTol2;
+    const float y32 = __builtin_amdgcn_rsqf(fmaxf((float)d2, 1.0e-30f));
+    const double id = (double)y32;
+    const double e = d2 * id;
+    const double t = (p.kappa_half * e) * __builtin_fma(-e, id, 3.0);  // kappa d, one Newton step
+    const float u = (float)__builtin_amdgcn_fract(t);
+    const float ym = ok ? y32 : 0.0f;
+    const double cf = (double)(__builtin_amdgcn_cosf(u) * ym);
+    const double sf = (double)(__builtin_amdgcn_sinf(u) * ym);
+    const double b = dotv<3>(r, ny) * id;
+    const double wr = __builtin_fma(id, q.qr, q.kqi), wi = __builtin_fma(id, q.qi, q.mkqr);
+    const double zr = __builtin_fma(b, wr, q.cr), zi = __builtin_fma(b, wi, q.ci);
+    accr = __builtin_fma(cf, zr, __builtin_fma(-sf, zi, accr));
+    acci = __builtin_fma(cf, zi, __builtin_fma(sf, zr, acci));
+}
+
 inline KParams basic_params(double k) {
     KParams p;
     p.k = k;
