@@ -120,8 +120,9 @@ def main():
                          "(default), 2 wherever it applies")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the direct sum would shard by target rows; see the design notes).", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path "
+              "(the direct sum would shard by target rows, the symmetric sum by blocks of tile pairs).",
+              file=sys.stderr)
         sys.exit(2)
 
     from __graft_entry__ import load_package
