@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the CPU
-baseline leg of bench.py as the checker. The product never imports it.
+baseline leg of bench.py as the CPU reference. The product never imports it.
 
 All values are returned as complex128 (Laplace values have zero imaginary
 part). Point sets are passed as objects with .coords (dim, n), .normals

@@ -1,6 +1,6 @@
 /*
  * CPU FP64 restatement of Inti.jl's corrected Nystrom hot path.
- * TEST INFRASTRUCTURE ONLY: the checker for the GPU library, never shipped
+ * TEST INFRASTRUCTURE ONLY: the CPU reference for the GPU library, never shipped
  * or called by it. Each function cites the reference lines it follows.
  */
 #include "inti_oracle.h"

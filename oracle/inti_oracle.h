@@ -1,8 +1,8 @@
 /*
  * CPU restatement of Inti.jl's corrected Nystrom hot path, in FP64.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker the GPU library is
- * compared against (tests, smoke check, the CPU-baseline leg of the bench).
+ * TEST INFRASTRUCTURE ONLY. This library is the CPU reference the GPU library
+ * is compared against (tests, smoke check, the CPU-baseline leg of the bench).
  * The product never links, loads or calls it.
  *
  * Reference files restated (Inti.jl):

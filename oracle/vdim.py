@@ -2,7 +2,7 @@
 restated in numpy for 2D, on top of the C oracle's direct sums and boundary
 correction.
 
-TEST INFRASTRUCTURE ONLY (the checker; the product never imports it).
+TEST INFRASTRUCTURE ONLY (a CPU reference; the product never imports it).
 
 Independent of the product's closed-form particular solutions: polynomials
 are numpy coefficient arrays c[i, j] (x^i y^j) and P is obtained by solving the
