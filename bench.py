@@ -183,6 +183,13 @@ def run_cfie(inti, lib, ctx, args):
     err_mx_inf = float(np.abs(ymx - y64).max() / np.abs(y64).max())
     log(f"bench: fp64 {t64 * 1e3:.3f} ms/apply (kernel {k64:.3f} ms), "
         f"mixed {tmx * 1e3:.3f} ms/apply (kernel {kmx:.3f} ms), rel err {err_mx:.2e}")
+    # the plain FP64 kernel (one direction per pair, no symmetric evaluation)
+    t64p = None
+    if args.sym != 0:
+        L64.iop.set_symmetric(0)
+        t64p, _ = time_apply(ctx, lib, L64.iop.handle, dx, dy, n, max(3, args.steps // 4), 1)
+        L64.iop.set_symmetric(args.sym if args.sym is not None else 1)
+        log(f"bench: plain fp64 (one direction per pair) {t64p * 1e3:.3f} ms/apply")
 
     cpu = None
     err64 = None
@@ -231,6 +238,9 @@ def run_cfie(inti, lib, ctx, args):
         },
         "fp64_value": round(pairs / t64 / 1e9, 3),
         "mixed_over_fp64": round(t64 / tmx, 3),
+        # the same ratio against the plain FP64 kernel (one direction per pair)
+        "fp64_one_direction_value": round(pairs / t64p / 1e9, 3) if t64p else None,
+        "mixed_over_fp64_one_direction": round(t64p / tmx, 3) if t64p else None,
         "rel_err_mixed_vs_fp64": err_mx,
         "relinf_err_mixed_vs_fp64": err_mx_inf,
         "rel_err_fp64_vs_oracle_rows": err64,
