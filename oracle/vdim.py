@@ -11,6 +11,12 @@ linear equations on its coefficients:
   Helmholtz (Delta + k^2) P = p with deg P = |I|.
 The per-element least squares use numpy's SVD-based lstsq, as the reference
 solves with svd(L) (vdim.jl:104-107).
+
+Parity unpinned: which particular solution P the reference picks. It comes
+from ElementaryPDESolutions (compat 0.2 - 0.3 in the reference's Project.toml,
+not vendored), and P is not unique, so no fixture fixes that choice. The
+corrected volume potential itself is pinned: its convergence rate against a
+manufactured solution's Green representation.
 """
 import math
 
