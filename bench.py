@@ -32,12 +32,59 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_workload(inti, freq):
+SEED = 20261202
+
+
+def make_workload(inti, freq=23):
+    """Unit sphere, geodesic icosphere of frequency freq (P2 triangles x
+    VR4N6), complex Gaussian density: the combined-field workload."""
     Q = inti.sphere_quadrature(freq, qorder=4, base="ico")
-    rng = np.random.default_rng(20261202)
+    rng = np.random.default_rng(SEED)
     n = len(Q)
     x = (rng.normal(size=n) + 1j * rng.normal(size=n)) * np.sqrt(0.5)
     return Q, x
+
+
+def circle_workload(inti):
+    """Unit circle, 128 P2 elements x 5 Gauss nodes (N = 640), density
+    cos(3 theta) + 0.1 N(0, 1)."""
+    Q = inti.circle_quadrature(128, qorder=9)
+    rng = np.random.default_rng(SEED)
+    th = np.arctan2(Q.coords[1], Q.coords[0])
+    sig = np.cos(3 * th) + 0.1 * rng.normal(size=len(Q))
+    return Q, sig
+
+
+def batch_workload(inti, P=256):
+    """P ellipses a ~ U[1,3], b ~ U[0.3,1], k ~ U[1,40], U{50..2000} P2
+    elements x 6 Gauss nodes. Returns (quadratures, a, b, k, rng)."""
+    rng = np.random.default_rng(SEED)
+    a = rng.uniform(1, 3, P)
+    b = rng.uniform(0.3, 1, P)
+    ks = rng.uniform(1, 40, P)
+    ne = rng.integers(50, 2001, P)
+    qs = [inti.ellipse_quadrature(a[p], b[p], int(ne[p]), qorder=11) for p in range(P)]
+    return qs, a, b, ks, rng
+
+
+def bumpy_quadrature(inti, freq=64):
+    """Bumpy sphere r = 1 + 0.15 sin(5 theta) cos(4 phi), icosphere frequency
+    64: 81,920 P2 triangles x VR4N6 = 491,520 nodes."""
+    return inti.sphere_quadrature(freq, radial=lambda th, ph: 1.0 + 0.15 * np.sin(5 * th) * np.cos(4 * ph))
+
+
+def shape_quadrature(inti, freq=29):
+    """Sphere r = 1 + 0.1 sin(3 theta) cos(2 phi), icosphere frequency 29:
+    16,820 P2 triangles x VR4N6 = 100,920 nodes."""
+    return inti.sphere_quadrature(freq, radial=lambda th, ph: 1.0 + 0.1 * np.sin(3 * th) * np.cos(2 * ph))
+
+
+def fibonacci_directions(nd=64):
+    """nd unit directions on a Fibonacci lattice of the sphere (3 x nd)."""
+    i = np.arange(nd) + 0.5
+    pol = np.arccos(1 - 2 * i / nd)
+    az = np.pi * (1 + 5 ** 0.5) * i
+    return np.stack([np.cos(az) * np.sin(pol), np.sin(az) * np.sin(pol), np.cos(pol)])
 
 
 def time_apply(ctx, lib, op, dx, dy, n, steps, warmup):
@@ -282,13 +329,8 @@ def run_batch(inti, lib, ctx, args):
     16 complex Gaussian right-hand sides each, I/2 + D + dD per problem:
     one persistent batched apply (autotuned work items), mixed vs FP64, and
     the batched hipGraph GMRES to 1e-10."""
-    rng = np.random.default_rng(20261202)
     P = args.nprob
-    a = rng.uniform(1, 3, P)
-    b = rng.uniform(0.3, 1, P)
-    ks = rng.uniform(1, 40, P)
-    ne = rng.integers(50, 2001, P)
-    qs = [inti.ellipse_quadrature(a[p], b[p], int(ne[p]), qorder=11) for p in range(P)]
+    qs, a, b, ks, rng = batch_workload(inti, P)
     ops = [inti.Helmholtz(k=ks[p], dim=2) for p in range(P)]
     t0 = time.perf_counter()
     corr = []
@@ -497,7 +539,7 @@ def run_bumpy(inti, lib, ctx, args):
     GMRES to 1e-10 against the all-FP64 GMRES solution. The right-hand side
     is sum_(l<=8) c_lm Y_lm, c ~ N(0, 1)."""
     freq = 64
-    Q = inti.sphere_quadrature(freq, radial=lambda th, ph: 1.0 + 0.15 * np.sin(5 * th) * np.cos(4 * ph))
+    Q = bumpy_quadrature(inti, freq)
     n = len(Q)
     S64, S32, t_dim = _sl_setup(inti, ctx, Q)
     rng = np.random.default_rng(20261202)
@@ -570,7 +612,7 @@ def run_shape(inti, lib, ctx, args):
     (h = 1e-6) along 20 seeded random directions; the adjoint of the full
     N x N combined-field direct sum is timed against its forward."""
     k = 10.0
-    Q = inti.sphere_quadrature(29, radial=lambda th, ph: 1.0 + 0.1 * np.sin(3 * th) * np.cos(2 * ph))
+    Q = shape_quadrature(inti)
     n = len(Q)
     op = inti.Helmholtz(k=k, dim=3)
     if args.only:  # profiling runs: a random density instead of the solve, kernels only, no JSON
@@ -589,12 +631,7 @@ def run_shape(inti, lib, ctx, args):
     t0 = time.perf_counter()
     sig, info = inti.gmres(L, b, restart=100, maxiter=300, rtol=1e-8)
     t_solve = time.perf_counter() - t0
-    nd = 64
-    i = np.arange(nd) + 0.5
-    pol = np.arccos(1 - 2 * i / nd)
-    az = np.pi * (1 + 5 ** 0.5) * i
-    Xh = np.stack([np.cos(az) * np.sin(pol), np.sin(az) * np.sin(pol), np.cos(pol)])
-    P = inti.points(Xh)
+    P = inti.points(fibonacci_directions(64))
 
     def quad(y, nv, w):
         return inti.Quadrature(y, nv, w, np.arange(n, dtype=np.int64).reshape(1, -1))
@@ -694,11 +731,8 @@ def run_circle(inti, lib, ctx, args):
     assemble_matrix + one corrected matvec on the GPU, and the same three
     steps on the CPU oracle (the reference CPU path, restated) as the CPU
     baseline leg."""
-    Q = inti.circle_quadrature(128, qorder=9)
+    Q, sig = circle_workload(inti)
     n = len(Q)
-    rng = np.random.default_rng(20261202)
-    th = np.arctan2(Q.coords[1], Q.coords[0])
-    sig = np.cos(3 * th) + 0.1 * rng.normal(size=n)
     op = inti.Laplace(dim=2)
     times = []
     for it in range(12):  # 2 untimed warm-up pipelines (first-call scratch allocations), median of 10

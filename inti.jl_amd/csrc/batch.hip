@@ -650,12 +650,17 @@ struct BWork {
     BGState *st = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
+    // Teardown order: drain the stream (an error return may leave launches
+    // queued), destroy the executable graph and the graph whose kernel nodes
+    // hold pointers into the buffers, and only then release the buffers.
     ~BWork() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
         void *ps[] = {V, w, vk, r, partial, h, H, sn, g, y, cs, res, npart, nrm, tol, active, kk, segp, segptr, segr, st};
         for (void *p : ps)
             if (p) (void)hipFree(p);
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
     }
 };
 
@@ -713,6 +718,7 @@ static int batch_gmres_t(Batch *B, int R, const double *b, double *x, int restar
     const size_t es = CPLX ? 16 : 8;
     const int m = restart;
     BWork W;
+    W.stream = s;
     W.m = m; W.R = R; W.nsys = B->nprob * R;
     // reduction segments: rows of one problem, at most kSegRows each
     std::vector<int32_t> segp, segptr(B->nprob + 1, 0);

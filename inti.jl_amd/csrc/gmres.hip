@@ -272,17 +272,23 @@ struct GmresWork {
     int nb = 1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
+    // Teardown order: drain the stream (an error return may leave launches
+    // queued), destroy the executable graph and the graph whose kernel nodes
+    // hold pointers into the buffers, and only then release the buffers.
     ~GmresWork() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
         void *ps[] = {V, w, vk, r, partial, h, H, sn, g, y, cs, res, npart, st};
         for (void *p : ps)
             if (p) (void)hipFree(p);
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
     }
 };
 
 static int gwork_alloc(Ctx *ctx, GmresWork &W, int m, int64_t n, int cplx) {
     W.m = m; W.n = n; W.cplx = cplx;
+    W.stream = ctx->stream;
     const size_t es = cplx ? 16 : 8;
     W.nb = (int)((n + kGRows - 1) / kGRows);
     const size_t hm = (size_t)(m + 1) * m;
