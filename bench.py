@@ -157,8 +157,9 @@ def main():
                          "adjoint on a perturbed sphere; circle: 2D Laplace DL on 640 nodes; dense: the "
                          "assembled corrected operator on many right-hand sides (FP64 matrix cores)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
-                    help="profiling runs: time one leg only, no checks, no JSON")
+    ap.add_argument("--only", choices=["mixed", "fp64", "gmres"], default=None,
+                    help="profiling runs: time one leg only, no checks, no JSON (gmres: the batch "
+                         "workload's hipGraph-captured batched GMRES, --steps Arnoldi steps)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="profiler calibration: run a plain device copy of this many bytes 5 times")
     ap.add_argument("--nprob", type=int, default=256, help="batch workload: number of problems")
@@ -376,6 +377,13 @@ def run_batch(inti, lib, ctx, args):
         return (time.perf_counter() - t0) / steps, float(np.mean(kms))
 
     steps = max(3, args.steps // 4)
+    if args.only == "gmres":  # profiling run of the batched GMRES (graph capture and replay)
+        t0 = time.perf_counter()
+        _, info = B64.gmres(X.T, restart=100, maxiter=args.steps, rtol=1e-10)
+        log(f"bench batch: gmres only: {info['iterations']} Arnoldi steps in {time.perf_counter() - t0:.2f} s")
+        ctx.check(lib.inti_device_free(ctx.handle, dx))
+        ctx.check(lib.inti_device_free(ctx.handle, dy))
+        return None
     if args.only:  # profiling runs: one apply leg, no solver, no JSON
         t, kms = timed(B32 if args.only == "mixed" else B64, steps, 1)
         log(f"bench batch: {args.only} only: {t * 1e3:.2f} ms/apply, kernel {kms:.2f} ms")

@@ -633,25 +633,33 @@ __global__ __launch_bounds__(256) void k_adj_sym_reduce(AdjOut o, int64_t S) {
     }
 }
 
-// sigma_bar += corr^H ubar + conj(identity) ubar: one thread per row i
-// scatters conj(val_iq) ubar_i to column colind_q (atomics: rows share
-// columns only across elements, so the collisions are few)
-__global__ __launch_bounds__(256) void k_adj_corr(int64_t m, const int32_t *rowptr, const int32_t *colind,
-                                                  const double *val, int cplx, const double2 *c,
-                                                  double *sigma_bar) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= m) return;
-    const double2 cc = c[i];  // conj(ubar_i)
-    for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
-        const int32_t j = colind[q];
+// sigma_bar += corr^H ubar: one thread per column j gathers
+// conj(val_ij) ubar_i over the column's rows (the transposed pattern, rows
+// increasing) and adds the sum to sigma_bar_j: one writer per entry, a
+// fixed order, deterministic. (The identity term is added on the host.)
+__global__ __launch_bounds__(256) void k_adj_corr(int64_t n, const int32_t *tptr, const int32_t *trow,
+                                                  const int32_t *tperm, const double *val, int cplx,
+                                                  const double2 *c, double *sigma_bar) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    double sr = 0.0, si = 0.0;
+    for (int32_t p = tptr[j]; p < tptr[j + 1]; ++p) {
+        const double2 cc = c[trow[p]];  // conj(ubar_i)
+        const int32_t q = tperm[p];
         if (cplx) {
             // conj(v) ubar = conj(v c)
             const double vr = val[2 * q], vi = val[2 * q + 1];
-            atomicAdd(&sigma_bar[2 * j], vr * cc.x - vi * cc.y);
-            atomicAdd(&sigma_bar[2 * j + 1], -(vr * cc.y + vi * cc.x));
+            sr += vr * cc.x - vi * cc.y;
+            si -= vr * cc.y + vi * cc.x;
         } else {
-            atomicAdd(&sigma_bar[j], val[q] * cc.x);
+            sr += val[q] * cc.x;
         }
+    }
+    if (cplx) {
+        sigma_bar[2 * j] += sr;
+        sigma_bar[2 * j + 1] += si;
+    } else {
+        sigma_bar[j] += sr;
     }
 }
 
@@ -870,9 +878,9 @@ static int adjoint_sigma_only(Op *op, const double *ubar, double *sigma_bar) {
                        (const double *)dpart.p, kp.postr, kp.posti, op->src->weights, (double *)dsb.p);
     INTI_HIP(ctx, hipGetLastError());
     if (op->corr) {
-        const Csr *cr = op->corr;
-        hipLaunchKernelGGL(k_adj_corr, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m, cr->rowptr, cr->colind,
-                           cr->val, cplx, (const double2 *)dc.p, (double *)dsb.p);
+        hipLaunchKernelGGL(k_adj_corr, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, op->corr_tptr,
+                           op->corr_trow, op->corr_tperm, op->corr->val, cplx, (const double2 *)dc.p,
+                           (double *)dsb.p);
         INTI_HIP(ctx, hipGetLastError());
     }
     INTI_HIP(ctx, hipMemcpyAsync(sigma_bar, dsb.p, sizeof(double) * cw * n, hipMemcpyDeviceToHost, s));
@@ -1030,9 +1038,9 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
     if (!fsym && ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev1, s));
     // corrected operator: sigma_bar also gets corr^H ubar and conj(identity) ubar
     if (sigma_bar && op->corr) {
-        const Csr *cr = op->corr;
-        hipLaunchKernelGGL(k_adj_corr, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m, cr->rowptr, cr->colind,
-                           cr->val, cplx, (const double2 *)dc.p, (double *)dsb.p);
+        hipLaunchKernelGGL(k_adj_corr, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, op->corr_tptr,
+                           op->corr_trow, op->corr_tperm, op->corr->val, cplx, (const double2 *)dc.p,
+                           (double *)dsb.p);
         INTI_HIP(ctx, hipGetLastError());
     }
     std::vector<double> kp(nsb);

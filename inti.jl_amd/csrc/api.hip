@@ -62,6 +62,15 @@ void csr_free(Csr *c) {
     delete static_cast<inti_csr *>(c);
 }
 
+void op_free_correction(Op *op) {
+    csr_free(op->corr);
+    op->corr = nullptr;
+    int32_t *ps[] = {op->corr_tptr, op->corr_trow, op->corr_tperm};
+    for (int32_t *p : ps)
+        if (p) (void)hipFree(p);
+    op->corr_tptr = op->corr_trow = op->corr_tperm = nullptr;
+}
+
 int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *qtags,
                const int64_t *near_ptr, const int64_t *near_idx, const double *mu,
                const double *params, Csr **outS, Csr **outD, double *diag_out);
@@ -340,7 +349,7 @@ int inti_op_destroy(inti_op *op) {
     {
         LOCK(op->ctx);
         (void)hipStreamSynchronize(op->ctx->stream);
-        csr_free(op->corr);
+        op_free_correction(op);
     }
     delete op;
     return INTI_OK;
@@ -547,13 +556,49 @@ int inti_op_attach_correction(inti_op *op, const inti_csr *csr, double coef_re, 
             return set_error(ctx, INTI_ERR_UNSUPPORTED, "attach: corrections with different patterns cannot be merged");
         accumulate = 1;
     } else {
+        // transposed pattern for the adjoint's gather (counting sort by
+        // column; rows stay increasing within a column)
+        const int64_t m = csr->m, n = csr->n, nnz = csr->nnz;
+        std::vector<int32_t> rp(m + 1), ci(nnz), tptr(n + 1, 0), trow(nnz), tperm(nnz);
+        INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        INTI_HIP(ctx, hipMemcpy(rp.data(), csr->rowptr, sizeof(int32_t) * (m + 1), hipMemcpyDeviceToHost));
+        if (nnz > 0) INTI_HIP(ctx, hipMemcpy(ci.data(), csr->colind, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+        for (int64_t q = 0; q < nnz; ++q) ++tptr[ci[q] + 1];
+        for (int64_t j = 0; j < n; ++j) tptr[j + 1] += tptr[j];
+        std::vector<int32_t> fill(tptr.begin(), tptr.end() - 1);
+        for (int64_t i = 0; i < m; ++i)
+            for (int32_t q = rp[i]; q < rp[i + 1]; ++q) {
+                const int32_t p = fill[ci[q]]++;
+                trow[p] = (int32_t)i;
+                tperm[p] = q;
+            }
+        int32_t *dt[3] = {nullptr, nullptr, nullptr};
+        const size_t tb[3] = {sizeof(int32_t) * (size_t)(n + 1), sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1),
+                              sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)};
+        for (int t = 0; t < 3; ++t)
+            if (hipMalloc((void **)&dt[t], tb[t]) != hipSuccess) {
+                for (int u = 0; u < 3; ++u)
+                    if (dt[u]) (void)hipFree(dt[u]);
+                return set_error(ctx, INTI_ERR_NOMEM, "attach: transposed pattern allocation failed");
+            }
         Csr *c = nullptr;
         int st = csr_alloc(ctx, csr->m, csr->n, csr->nnz, cplx, &c);
-        if (st) return st;
+        if (st) {
+            for (int u = 0; u < 3; ++u) (void)hipFree(dt[u]);
+            return st;
+        }
+        op->corr = c;
+        op->corr_tptr = dt[0]; op->corr_trow = dt[1]; op->corr_tperm = dt[2];
         INTI_HIP(ctx, hipMemcpyAsync(c->rowptr, csr->rowptr, sizeof(int32_t) * (csr->m + 1), hipMemcpyDeviceToDevice, ctx->stream));
         if (csr->nnz > 0)
             INTI_HIP(ctx, hipMemcpyAsync(c->colind, csr->colind, sizeof(int32_t) * csr->nnz, hipMemcpyDeviceToDevice, ctx->stream));
-        op->corr = c;
+        INTI_HIP(ctx, hipMemcpyAsync(dt[0], tptr.data(), tb[0], hipMemcpyHostToDevice, ctx->stream));
+        if (nnz > 0) {
+            INTI_HIP(ctx, hipMemcpyAsync(dt[1], trow.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, ctx->stream));
+            INTI_HIP(ctx, hipMemcpyAsync(dt[2], tperm.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, ctx->stream));
+        }
+        // the host vectors above must outlive the asynchronous uploads
+        INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
     if (csr->nnz > 0) {
         hipLaunchKernelGGL(k_axpy_vals, dim3((unsigned)((csr->nnz + 255) / 256)), dim3(256), 0, ctx->stream,
@@ -568,8 +613,7 @@ int inti_op_detach_correction(inti_op *op) {
     if (!op) return INTI_ERR_ARG;
     LOCK(op->ctx);
     (void)hipStreamSynchronize(op->ctx->stream);
-    csr_free(op->corr);
-    op->corr = nullptr;
+    op_free_correction(op);
     return INTI_OK;
 }
 

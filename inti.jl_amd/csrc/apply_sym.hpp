@@ -9,12 +9,14 @@
 // into S groups of G. Block (a, b), a <= b, takes the target tiles of group a
 // against the source tiles of group b (within a diagonal block, the tile
 // pairs I <= J only). A wave holds 64 x TPL targets (TPL = 1, 2 or 4 per lane).
-// For each 64-source sub-tile, lane l at step s pairs its targets with
-// source (l + s) mod 64, so the 64 lanes touch 64 different sources, read
-// from LDS without bank conflicts. The transposed sum for source (l + s) mod
-// 64 travels with the step: after each step lane l takes lane l+1's running
-// sum (ds_bpermute, a crossbar move with no LDS storage), and after 64 steps
-// lane l holds the sum for source l over the wave's targets. The waves' sums
+// For each 64-source sub-tile, lane l at step s < 32 pairs its targets with
+// sources (l + s) mod 64 and (l + s + 32) mod 64, so the 64 lanes touch 64
+// different sources per read, without LDS bank conflicts. The transposed
+// sums of the two sources travel with the steps in two independent chains:
+// after each step lane l takes lane l+1's running sums (ds_bpermute, a
+// crossbar move with no LDS storage), one chain's move overlapping the other
+// chain's arithmetic; after 32 steps one chain's sum for source l is in lane
+// l and the other's in lane l + 32, added with one last move. The waves' sums
 // are added in LDS: two waves onto zero by atomic adds (the sum of two terms
 // does not depend on their order), more waves in wave order.
 //
@@ -142,24 +144,25 @@ struct SymRec {
 
 // LDS record of a source node for the fused 3D Helmholtz DL / DL + ch*SL
 // pairs (helm3_sym_fused): coordinates, normal and the staged density terms
-// q, ch q, k Im q, -k Re q, in six double2 slots.
+// q, ch q, k Im q, -k Re q, in six double2 slots. CHIK: ch = -i k, so ch q
+// equals the (k Im q, -k Re q) slot; the record has five slots (20 KB of LDS
+// per tile instead of 24: one more block per CU).
+template <bool CHIK>
 struct SymRecF {
-    static constexpr int NS = 6;
+    static constexpr int NS = CHIK ? 5 : 6;
+    static constexpr int SK = CHIK ? 4 : 5;  // slot of (k Im q, -k Re q)
     __device__ static __forceinline__ void put(double2 (*s)[kSymBS], int j, const double *y, const double *nv,
                                                const QExt &q) {
         s[0][j] = make_double2(y[0], y[1]);
         s[1][j] = make_double2(y[2], nv[0]);
         s[2][j] = make_double2(nv[1], nv[2]);
         s[3][j] = make_double2(q.qr, q.qi);
-        s[4][j] = make_double2(q.cr, q.ci);
-        s[5][j] = make_double2(q.kqi, q.mkqr);
+        if constexpr (!CHIK) s[4][j] = make_double2(q.cr, q.ci);
+        s[SK][j] = make_double2(q.kqi, q.mkqr);
     }
-    // CHIK: ch = -i k, so ch q equals the (k Im q, -k Re q) slot and slot 4
-    // is not read
-    template <bool CHIK = false>
     __device__ static __forceinline__ void get(const double2 (*s)[kSymBS], int j, double *y, double *nv,
                                                QExt &q) {
-        const double2 v0 = s[0][j], v1 = s[1][j], v2 = s[2][j], v3 = s[3][j], v5 = s[5][j];
+        const double2 v0 = s[0][j], v1 = s[1][j], v2 = s[2][j], v3 = s[3][j], v5 = s[SK][j];
         y[0] = v0.x; y[1] = v0.y; y[2] = v1.x;
         nv[0] = v1.y; nv[1] = v2.x; nv[2] = v2.y;
         q.qr = v3.x; q.qi = v3.y; q.kqi = v5.x; q.mkqr = v5.y;
@@ -179,14 +182,14 @@ struct SymRecF {
 // holds targets t, t + 256 / TPL, ...
 // TRANS: the transposed sum z_j = sum_i K(x_i, x_j) q_i (the kernel matrix
 // transposed, for A^H ubar): the same pairs with the two values exchanged.
-template <int PDE, int DIM, int KIND, bool MIXED, int G, int TPL, bool TRANS = false>
+template <int PDE, int DIM, int KIND, bool MIXED, int G, int TPL, bool TRANS = false, bool CHIK = false>
 __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
     // mixed 3D Helmholtz DL and DL + ch*SL: value and accumulation fused
     // (2.8 % faster measured; in FP64 the larger LDS record cost more than
     // the 5 % fewer VALU instructions saved, so FP64 keeps the plain pair)
     constexpr bool FUSED =
         MIXED && PDE == INTI_HELMHOLTZ && DIM == 3 && (KIND == INTI_DL || KIND == INTI_COMB_SD);
-    using R = std::conditional_t<FUSED, SymRecF, SymRec<PDE, DIM, KIND>>;
+    using R = std::conditional_t<FUSED, SymRecF<CHIK>, SymRec<PDE, DIM, KIND>>;
     constexpr bool CPLX = PDE == INTI_HELMHOLTZ;
     constexpr int BS = kSymBS / TPL;  // threads
     constexpr int NW = BS / 64;       // waves
@@ -231,7 +234,16 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                 sym_node<DIM, CPLX>(A, a * kSymBS + tid + t * BS, xh[t], nh[t], qhr[t], qhi[t]);
         }
         for (int u = 0; u < 4; ++u) {
-            double ajr = 0.0, aji = 0.0;  // running transposed sum of source (lane + s) mod 64
+            // Two independent rotating chains of transposed sums: at step s
+            // lane l pairs its targets with source (l + s) mod 64 (chain A,
+            // running sum aa) and with source (l + s + 32) mod 64 (chain B,
+            // running sum ab), s < 32, and hands both sums to lane l - 1. The
+            // rotation of one chain completes while the other computes, so no
+            // step waits on the crossbar move of the step before. After a
+            // pass A's sum of source j sits in lane j - 32 and B's in lane j;
+            // the two are exchanged (register names only), so at the start of
+            // every pass lane l holds A for source l and B for source l + 32.
+            double aar = 0.0, aai = 0.0, abr = 0.0, abi = 0.0;
 #pragma unroll
             for (int gi = 0; gi < G; ++gi) {
                 const int64_t I = a * G + gi;
@@ -255,7 +267,7 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                         double yj[3], nj[3], qjr, qji;
                         if constexpr (FUSED) {
                             QExt qx;
-                            R::template get<false>(s_src, jl, yj, nj, qx);
+                            R::get(s_src, jl, yj, nj, qx);
                             qjr = qx.qr;
                             qji = qx.qi;
                         } else {
@@ -279,19 +291,16 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                 } else {
                     // off-diagonal tiles of a quadrature with distinct nodes hold
                     // no coincident pairs: the pair values need no mask there
-                    auto pairs = [&](auto mask_tag, auto chik_tag) {
+                    auto pairs = [&](auto mask_tag) {
                         constexpr bool MASK = decltype(mask_tag)::value;
-                        constexpr bool CHIK = decltype(chik_tag)::value;
                         if constexpr (FUSED) {
                             QExt qit[TPL];
 #pragma unroll
                             for (int t = 0; t < TPL; ++t) qit[t] = qext(qir[t], qii[t], p);
-#pragma unroll 2
-                            for (int s = 0; s < 64; ++s) {
-                                const int jl = u * 64 + ((lane + s) & 63);
+                            auto one = [&](int jl, double &ajr, double &aji) {
                                 double yj[3], nj[3];
                                 QExt qj;
-                                R::template get<CHIK>(s_src, jl, yj, nj, qj);
+                                R::get(s_src, jl, yj, nj, qj);
 #pragma unroll
                                 for (int t = 0; t < TPL; ++t) {
                                     double r[3];
@@ -300,13 +309,18 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                                     helm3_sym_fused<MIXED, MASK, TRANS>(r, ni[t], nj, p, qj, qit[t], adr[gi][t],
                                                                         adi[gi][t], ajr, aji);
                                 }
-                                ajr = lane_next(ajr, nxt);
-                                aji = lane_next(aji, nxt);
+                            };
+                            for (int s = 0; s < 32; ++s) {
+                                const int jl = u * 64 + ((lane + s) & 63);
+                                one(jl, aar, aai);
+                                one(jl ^ 32, abr, abi);
+                                aar = lane_next(aar, nxt);
+                                aai = lane_next(aai, nxt);
+                                abr = lane_next(abr, nxt);
+                                abi = lane_next(abi, nxt);
                             }
                         } else {
-#pragma unroll 2
-                            for (int s = 0; s < 64; ++s) {
-                                const int jl = u * 64 + ((lane + s) & 63);
+                            auto one = [&](int jl, double &ajr, double &aji) {
                                 double yj[3], nj[3], qjr, qji;
                                 R::get(s_src, jl, yj, nj, qjr, qji);
 #pragma unroll
@@ -333,21 +347,31 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                                         ajr = __builtin_fma(k2r, qir[t], ajr);
                                     }
                                 }
-                                ajr = lane_next(ajr, nxt);
-                                if constexpr (CPLX) aji = lane_next(aji, nxt);
+                            };
+                            for (int s = 0; s < 32; ++s) {
+                                const int jl = u * 64 + ((lane + s) & 63);
+                                one(jl, aar, aai);
+                                one(jl ^ 32, abr, abi);
+                                aar = lane_next(aar, nxt);
+                                abr = lane_next(abr, nxt);
+                                if constexpr (CPLX) {
+                                    aai = lane_next(aai, nxt);
+                                    abi = lane_next(abi, nxt);
+                                }
                             }
                         }
                     };
-                    if constexpr (FUSED) {
-                        if (A.distinct && A.ch_ik) pairs(std::false_type{}, std::true_type{});
-                        else if (A.distinct) pairs(std::false_type{}, std::false_type{});
-                        else pairs(std::true_type{}, std::false_type{});
-                    } else {
-                        if (A.distinct) pairs(std::false_type{}, std::false_type{});
-                        else pairs(std::true_type{}, std::false_type{});
-                    }
+                    if (A.distinct) pairs(std::false_type{});
+                    else pairs(std::true_type{});
+                    // exchange the chains' roles for the next pass
+                    double t0 = aar; aar = abr; abr = t0;
+                    t0 = aai; aai = abi; abi = t0;
                 }
             }
+            // source l: A's part in this lane, B's in lane l + 32
+            const int far = ((lane + 32) & 63) << 2;
+            double ajr = aar + lane_next(abr, far);
+            double aji = CPLX ? aai + lane_next(abi, far) : 0.0;
             if (trans) {
                 if constexpr (RED_ADD) {
                     double *sr = &s_red[0][u * 64 + lane].x;
@@ -413,6 +437,15 @@ using SymFn = void (*)(const SymArgs &, int64_t, hipStream_t);
 template <int PDE, int DIM, int KIND, bool MIXED, int G, bool TRANS = false>
 static void sym_launcher(const SymArgs &a, int64_t nblocks, hipStream_t s) {
     constexpr int TPL = PDE == INTI_LAPLACE ? 4 : DIM == 2 ? 1 : 2;
+    // the fused mixed combined field with ch = -i k has its own (smaller) record
+    constexpr bool CHIK_KERNEL = MIXED && !TRANS && PDE == INTI_HELMHOLTZ && DIM == 3 && KIND == INTI_COMB_SD;
+    if constexpr (CHIK_KERNEL) {
+        if (a.ch_ik) {
+            hipLaunchKernelGGL((k_apply_sym<PDE, DIM, KIND, MIXED, G, TPL, TRANS, true>), dim3((unsigned)nblocks),
+                               dim3(kSymBS / TPL), 0, s, a);
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_apply_sym<PDE, DIM, KIND, MIXED, G, TPL, TRANS>), dim3((unsigned)nblocks),
                        dim3(kSymBS / TPL), 0, s, a);
 }

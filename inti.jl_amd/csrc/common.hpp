@@ -68,6 +68,11 @@ struct Op {
     double cid[2] = {0, 0};          // identity coefficient (square ops)
     const Quad *trg = nullptr, *src = nullptr;
     Csr *corr = nullptr;  // owned merged correction (may be null)
+    // the correction's transposed pattern (built at the first attach; the
+    // pattern never changes while attached): column j's entries are
+    // corr_trow[p] / corr->val[corr_tperm[p]] for p in [corr_tptr[j],
+    // corr_tptr[j+1]), rows increasing. The adjoint gathers through it.
+    int32_t *corr_tptr = nullptr, *corr_trow = nullptr, *corr_tperm = nullptr;
     // launch configuration
     int block = 256, tpt = 1, split = 0;  // split 0 = automatic
     // symmetric evaluation of square operators on one quadrature: 0 off,
@@ -110,6 +115,7 @@ int refine_solve(Op *op_hi, Op *op_lo, const double *b, double *x, int restart, 
                  double inner_rtol, int max_outer, double rtol, int use_graph, int *outer,
                  int *inner_total, double *relres);
 int launch_assemble(const Op *op, double *out_dev);
+void op_free_correction(Op *op);
 int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_bar, double *ybar,
                double *nbar, double *wbar, double *xbar, const double *dcoefs, double *kbar);
 

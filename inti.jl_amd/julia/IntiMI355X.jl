@@ -3,10 +3,14 @@
 
 Julia binding of libinti_amd (the MI355X backend of the corrected Nyström
 operators). It keeps Inti's API: `IntegralOperator`s are uploaded once and
-applied as `LinearMap`s; `bdim_correction` returns `SparseMatrixCSC`s.
+applied as `LinearMap`s (`GPUOperator`); `bdim_correction` returns
+`SparseMatrixCSC`s; `single_double_layer` / `adj_double_layer_hypersingular`
+return corrected operators (dense matrices with `compression = (method = :none,)`);
+`*(::GPUOperator, σ)` has a ChainRulesCore `rrule`.
 
 Not exercised in this repository (no Julia toolchain in the build image);
-the C-ABI it calls is tested from Python with the same argument conventions.
+the C-ABI it calls is tested from Python with the same argument conventions
+and call sequences (tests/test_gpu_shim_sequences.py).
 """
 module IntiMI355X
 
@@ -14,6 +18,7 @@ using Inti
 using LinearAlgebra
 using LinearMaps
 using SparseArrays
+using ChainRulesCore
 
 const libinti = joinpath(@__DIR__, "..", "libinti_amd.so")
 
@@ -40,21 +45,26 @@ function __init__()
     finalizer(c -> ccall((:inti_ctx_destroy, libinti), Cint, (Ptr{Cvoid},), c.ptr), CTX[])
 end
 
-# QuadratureNode{N,T} (src/quadrature.jl:7-11) -> SoA, as the FMM ext does
+# QuadratureNode{N,T} (src/quadrature.jl:7-11) -> SoA, as the FMM ext does.
+# Nodes without normals (volume quadratures, point targets) upload none.
 function upload(Q::Inti.Quadrature{N}) where {N}
     n = length(Q)
     X = Matrix{Float64}(undef, n, N)        # column d = coordinate d (SoA)
-    Nrm = Matrix{Float64}(undef, n, N)
+    has_normals = n > 0 && Q.qnodes[1].normal !== nothing
+    Nrm = has_normals ? Matrix{Float64}(undef, n, N) : nothing
     w = Vector{Float64}(undef, n)
     for (i, q) in enumerate(Q.qnodes)
         X[i, :] .= Inti.coords(q)
-        Nrm[i, :] .= Inti.normal(q)
+        has_normals && (Nrm[i, :] .= Inti.normal(q))
         w[i] = Inti.weight(q)
     end
     p = Ref{Ptr{Cvoid}}(C_NULL)
-    check(CTX[].ptr, ccall((:inti_quad_upload, libinti), Cint,
-        (Ptr{Cvoid}, Cint, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{Ptr{Cvoid}}),
-        CTX[].ptr, N, n, X, Nrm, w, p))
+    nrm_ptr = has_normals ? pointer(Nrm) : Ptr{Float64}(C_NULL)
+    GC.@preserve Nrm begin
+        check(CTX[].ptr, ccall((:inti_quad_upload, libinti), Cint,
+            (Ptr{Cvoid}, Cint, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{Ptr{Cvoid}}),
+            CTX[].ptr, N, n, X, nrm_ptr, w, p))
+    end
     return p[]
 end
 
@@ -68,88 +78,161 @@ wavenumber(::Inti.Laplace) = 0.0
 wavenumber(op::Inti.Helmholtz) = Float64(op.k)
 
 const QUADS = IdDict{Any,Ptr{Cvoid}}()
-const HANDLES = IdDict{Any,Ptr{Cvoid}}()
+# one device operator per (IntegralOperator, precision): a mixed-precision
+# handle must never be returned for an FP64 request (the adjoint is FP64 only)
+const HANDLES = Dict{Tuple{UInt,Cint},Tuple{Any,Ptr{Cvoid}}}()
 
 quad_handle(Q) = get!(() -> upload(Q), QUADS, Q)
 
 """
     op_handle(iop::Inti.IntegralOperator; precision = FP64)
 
-Device operator of `iop` (created once per operator and cached).
+Device operator of `iop` at `precision` (created once per pair and cached).
 """
 function op_handle(iop::Inti.IntegralOperator; precision = FP64)
-    return get!(HANDLES, iop) do
-        K = Inti.kernel(iop)
-        op = Inti.operator(K)
-        N = Inti.ambient_dimension(iop.source)
-        h = Ref{Ptr{Cvoid}}(C_NULL)
-        check(CTX[].ptr, ccall((:inti_op_create, libinti), Cint,
-            (Ptr{Cvoid}, Cint, Cint, Cint, Cdouble, Ptr{Cdouble}, Ptr{Cvoid}, Ptr{Cvoid}, Cint,
-             Ref{Ptr{Cvoid}}),
-            CTX[].ptr, pde_id(op), kind_id(K), N, wavenumber(op), C_NULL,
-            quad_handle(iop.target), quad_handle(iop.source), precision, h))
-        h[]
-    end
+    key = (objectid(iop), Cint(precision))
+    entry = get(HANDLES, key, nothing)
+    entry !== nothing && entry[1] === iop && return entry[2]
+    K = Inti.kernel(iop)
+    op = Inti.operator(K)
+    N = Inti.ambient_dimension(iop.source)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(CTX[].ptr, ccall((:inti_op_create, libinti), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Cint, Cdouble, Ptr{Cdouble}, Ptr{Cvoid}, Ptr{Cvoid}, Cint,
+         Ref{Ptr{Cvoid}}),
+        CTX[].ptr, pde_id(op), kind_id(K), N, wavenumber(op), C_NULL,
+        quad_handle(iop.target), quad_handle(iop.source), precision, h))
+    HANDLES[key] = (iop, h[])   # the operator is kept alive with its handle
+    return h[]
+end
+
+"""
+    GPUOperator{T}
+
+Matrix-free GPU counterpart of an `IntegralOperator` (with any attached
+DIM correction and identity term) as a `LinearMap`: `mul!(y, A, x, α, β)`
+calls `inti_op_apply`.
+"""
+struct GPUOperator{T} <: LinearMaps.LinearMap{T}
+    handle::Ptr{Cvoid}
+    m::Int
+    n::Int
+    iop::Inti.IntegralOperator
+    precision::Cint
+end
+
+Base.size(A::GPUOperator) = (A.m, A.n)
+
+function LinearMaps._unsafe_mul!(y::AbstractVecOrMat, A::GPUOperator{T}, x::AbstractVecOrMat,
+                                 α::Number = true, β::Number = false) where {T}
+    xs, ys = convert(Array{T}, x), convert(Array{T}, y)
+    a, b = T[α], T[β]
+    check(CTX[].ptr, ccall((:inti_op_apply, libinti), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{T}, Ptr{T}, Ptr{T}, Ptr{T}), A.handle, size(xs, 2), xs, ys, a, b))
+    y === ys || copyto!(y, ys)
+    return y
 end
 
 """
     gpu_operator(iop::Inti.IntegralOperator; precision = FP64)
 
-Matrix-free GPU counterpart of `iop` as a `LinearMap` (mul! calls
-`inti_op_apply`), to be combined with a sparse correction as
-`gpu_operator(Sop) + LinearMap(δS)` (src/api.jl:182-185), or with the
-correction attached on the device by `attach!`.
+Matrix-free GPU counterpart of `iop`, to be combined with a sparse
+correction as `gpu_operator(Sop) + LinearMap(δS)` (src/api.jl:182-185), or
+with the correction attached on the device by `attach!`.
 """
 function gpu_operator(iop::Inti.IntegralOperator; precision = FP64)
     h = op_handle(iop; precision)
-    T = eltype(iop)
     m, n = size(iop)
-    return LinearMap{T}(m, n) do y, x
-        α = T[1]; β = T[0]
-        check(CTX[].ptr, ccall((:inti_op_apply, libinti), Cint,
-            (Ptr{Cvoid}, Cint, Ptr{T}, Ptr{T}, Ptr{T}, Ptr{T}), h, 1, x, y, α, β))
-        return y
-    end
+    return GPUOperator{eltype(iop)}(h, m, n, iop, Cint(precision))
 end
 
 """
-    set_symmetric!(iop, mode)
+    attach!(A::GPUOperator, csr::Ptr{Cvoid}, coef = 1)
+
+Fuse `coef * δA` (a device CSR from the DIM build) into every apply of `A`.
+"""
+function attach!(A::GPUOperator, csr::Ptr{Cvoid}, coef::Number = 1)
+    c = ComplexF64(coef)
+    check(CTX[].ptr, ccall((:inti_op_attach_correction, libinti), Cint,
+        (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble), A.handle, csr, real(c), imag(c)))
+    return A
+end
+
+"""
+    set_symmetric!(A, mode)
 
 Symmetric evaluation of an operator whose target is its source (the same
 `Quadrature` object, hence the same device quadrature): 0 off, 1 automatic
 (the default), 2 always.
 """
-function set_symmetric!(iop::Inti.IntegralOperator, mode::Integer)
+function set_symmetric!(A::GPUOperator, mode::Integer)
     check(CTX[].ptr, ccall((:inti_op_set_symmetric, libinti), Cint, (Ptr{Cvoid}, Cint, Cint),
-        op_handle(iop), mode, 0))
-    return iop
+        A.handle, mode, 0))
+    return A
 end
+set_symmetric!(iop::Inti.IntegralOperator, mode::Integer) = set_symmetric!(gpu_operator(iop), mode)
 
 """
-    pullback(iop, ȳ, σ; dcoefs = nothing)
+    pullback(A, ȳ, σ; dcoefs = nothing, geometry = true)
 
-Reverse mode of `y = iop * σ` on the GPU: the gradients of Re⟨ȳ, y⟩ with
-respect to σ (`iop' * ȳ`), the source nodes, normals and weights, the
-target nodes and k. For an operator on one quadrature (`iop.target ===
-iop.source`) the node gradient is the sum of the source and target parts.
+Reverse mode of `y = A * σ` on the GPU: the gradients of Re⟨ȳ, y⟩ with
+respect to σ (`A' * ȳ`) and, if `geometry`, the source nodes, normals and
+weights, the target nodes and k. For an operator on one quadrature
+(`iop.target === iop.source`) the node gradient is the sum of the source
+and target parts. With a DIM correction attached only σ̄ is available.
 """
-function pullback(iop::Inti.IntegralOperator, ȳ::Vector{T}, σ::Vector{T};
-                  dcoefs = nothing) where {T}
-    m, n = size(iop)
+function pullback(A::GPUOperator, ȳ::Vector{T}, σ::Vector{T}; dcoefs = nothing,
+                  geometry = true) where {T}
+    m, n = size(A)
+    iop = A.iop
     N = Inti.ambient_dimension(iop.source)
     σ̄ = similar(σ)
     # gradients of points are n x N (column d = coordinate d), the layout of
     # the uploaded coordinates
     ys, ns, xs, ws = zeros(n, N), zeros(n, N), zeros(m, N), zeros(n)
     k̄ = Ref(0.0)
-    dc = dcoefs === nothing ? C_NULL : Float64[real(dcoefs[1]), imag(dcoefs[1]), real(dcoefs[2]), imag(dcoefs[2])]
-    check(CTX[].ptr, ccall((:inti_op_adjoint, libinti), Cint,
-        (Ptr{Cvoid}, Ptr{T}, Ptr{T}, Ptr{T}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
-         Ptr{Float64}, Ptr{Float64}, Ref{Float64}),
-        op_handle(iop), ȳ, σ, σ̄, ys, ns, ws, xs, dc, k̄))
+    dc = dcoefs === nothing ? Ptr{Float64}(C_NULL) :
+         Float64[real(dcoefs[1]), imag(dcoefs[1]), real(dcoefs[2]), imag(dcoefs[2])]
+    if geometry
+        check(CTX[].ptr, ccall((:inti_op_adjoint, libinti), Cint,
+            (Ptr{Cvoid}, Ptr{T}, Ptr{T}, Ptr{T}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+             Ptr{Float64}, Ptr{Float64}, Ref{Float64}),
+            A.handle, ȳ, σ, σ̄, ys, ns, ws, xs, dc, k̄))
+    else
+        check(CTX[].ptr, ccall((:inti_op_adjoint, libinti), Cint,
+            (Ptr{Cvoid}, Ptr{T}, Ptr{T}, Ptr{T}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+             Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+            A.handle, ȳ, σ, σ̄, C_NULL, C_NULL, C_NULL, C_NULL, C_NULL, C_NULL))
+    end
     nodes = iop.target === iop.source ? ys .+ xs : ys
     return (; sigma = σ̄, nodes, target_nodes = xs, normals = ns, weights = ws, k = k̄[])
 end
+pullback(iop::Inti.IntegralOperator, ȳ, σ; kw...) = pullback(gpu_operator(iop), ȳ, σ; kw...)
+
+"""
+    rrule(*, A::GPUOperator, σ)
+
+ChainRulesCore rule of the GPU apply: the pullback recomputes kernel values
+(nothing N×N is stored). The operator's tangent carries the gradients of
+its geometry and wavenumber (`nodes`, `normals`, `weights`, `k`); for an
+operator with a DIM correction attached (whose weights depend on the
+geometry through the least squares) only σ̄ is returned.
+"""
+function ChainRulesCore.rrule(::typeof(*), A::GPUOperator{T}, σ::Vector{T}) where {T}
+    y = A * σ
+    function times_pullback(ȳ)
+        ȳv = convert(Vector{T}, unthunk(ȳ))
+        geometry = !has_correction(A)
+        g = pullback(A, ȳv, σ; geometry)
+        dA = geometry ? Tangent{typeof(A)}(; nodes = g.nodes, normals = g.normals,
+                                            weights = g.weights, k = g.k) : NoTangent()
+        return NoTangent(), dA, g.sigma
+    end
+    return y, times_pullback
+end
+
+const CORRECTED = Set{Ptr{Cvoid}}()
+has_correction(A::GPUOperator) = A.handle in CORRECTED
 
 """
     assemble_matrix(iop) -> Matrix   (src/nystrom.jl:103-124 on the GPU)
@@ -179,18 +262,9 @@ function to_csc(csr::Ptr{Cvoid}, ::Type{T}) where {T}
     return sparse(I, colind .+ 1, val, m[], n[])
 end
 
-"""
-    bdim_correction(op, target, source, Sop, Dop; green_multiplier,
-                    parameters = Inti.DimParameters(), derivative = false)
-
-GPU counterpart of `Inti.bdim_correction` (src/bdim.jl:50-61) for
-`target === source` (self pattern) or explicit near lists.
-"""
-function bdim_correction(op, target, source::Inti.Quadrature, Sop, Dop;
-                         green_multiplier::Vector{<:Real},
-                         parameters = Inti.DimParameters(), derivative = false,
-                         maxdist = Inf)
-    T = eltype(Sop)
+# the device CSR pair (δS, δD) of the DIM build
+function bdim_build(op, target, source::Inti.Quadrature, Sop, Dop; green_multiplier,
+                    parameters = Inti.DimParameters(), derivative = false, maxdist = Inf)
     (E, qtags), = collect(source.etype2qtags)          # single element type
     nq, ne = size(qtags)
     qt = Int64.(qtags .- 1)                             # 0-based, column-major
@@ -207,7 +281,87 @@ function bdim_correction(op, target, source::Inti.Quadrature, Sop, Dop;
          Ptr{Cdouble}, Ptr{Cdouble}, Ref{Ptr{Cvoid}}, Ref{Ptr{Cvoid}}, Ptr{Cdouble}),
         op_handle(Sop), op_handle(Dop), derivative, nq, ne, qt, near_ptr, near_idx,
         Float64.(green_multiplier), params, dS, dD, C_NULL))
-    return to_csc(dS[], T), to_csc(dD[], T)
+    return dS[], dD[]
 end
+
+free_csr(c::Ptr{Cvoid}) = ccall((:inti_csr_destroy, libinti), Cint, (Ptr{Cvoid},), c)
+
+"""
+    bdim_correction(op, target, source, Sop, Dop; green_multiplier,
+                    parameters = Inti.DimParameters(), derivative = false)
+
+GPU counterpart of `Inti.bdim_correction` (src/bdim.jl:50-61) for
+`target === source` (self pattern) or explicit near lists.
+"""
+function bdim_correction(op, target, source::Inti.Quadrature, Sop, Dop; kw...)
+    T = eltype(Sop)
+    dS, dD = bdim_build(op, target, source, Sop, Dop; kw...)
+    out = (to_csc(dS, T), to_csc(dD, T))
+    free_csr(dS); free_csr(dD)
+    return out
+end
+
+"""
+    single_double_layer(; op, target, source = target, compression = (method = :gpu,),
+                        correction = (method = :dim,), derivative = false, precision = FP64)
+
+`Inti.single_double_layer` (src/api.jl:65-187) on the GPU. With
+`compression.method == :gpu` the two operators are matrix-free
+`GPUOperator`s with the DIM correction fused into every apply (the
+`LinearMap(Smat) + LinearMap(δS)` of src/api.jl:182-185); with `:none`
+they are dense matrices (`assemble_matrix` + `axpy!` of the correction,
+src/api.jl:171-173). `derivative = true` gives the adjoint double layer and
+hypersingular pair.
+"""
+function single_double_layer(; op, target, source = target, compression = (method = :gpu,),
+                             correction = (method = :dim,), derivative = false,
+                             precision = FP64)
+    compression.method in (:gpu, :none) ||
+        throw(ArgumentError("compression.method must be :gpu or :none"))
+    correction.method in (:dim, :none) ||
+        throw(ArgumentError("correction.method must be :dim or :none"))
+    G = derivative ? Inti.AdjointDoubleLayerKernel(op) : Inti.SingleLayerKernel(op)
+    dG = derivative ? Inti.HyperSingularKernel(op) : Inti.DoubleLayerKernel(op)
+    Sop = Inti.IntegralOperator(G, target, source)
+    Dop = Inti.IntegralOperator(dG, target, source)
+    dS = dD = C_NULL
+    if correction.method == :dim
+        μ = if haskey(correction, :green_multiplier)
+            correction.green_multiplier
+        else
+            loc = target === source ? :on : correction.target_location
+            fill(Inti._green_multiplier(loc), length(target))
+        end
+        dS, dD = bdim_build(op, target, source, Sop, Dop; green_multiplier = μ,
+                            parameters = get(correction, :parameters, Inti.DimParameters()),
+                            derivative, maxdist = get(correction, :maxdist, Inf))
+    end
+    if compression.method == :none
+        S, D = assemble_matrix(Sop), assemble_matrix(Dop)
+        if dS != C_NULL
+            T = eltype(Sop)
+            axpy!(true, to_csc(dS, T), S)
+            axpy!(true, to_csc(dD, T), D)
+            free_csr(dS); free_csr(dD)
+        end
+        return S, D
+    end
+    S = gpu_operator(Sop; precision)
+    D = gpu_operator(Dop; precision)
+    if dS != C_NULL
+        attach!(S, dS); attach!(D, dD)       # the operators keep merged copies
+        push!(CORRECTED, S.handle, D.handle)
+        free_csr(dS); free_csr(dD)
+    end
+    return S, D
+end
+
+"""
+    adj_double_layer_hypersingular(; op, target, source = target, kw...)
+
+`Inti.adj_double_layer_hypersingular` (src/api.jl:197-212) on the GPU.
+"""
+adj_double_layer_hypersingular(; op, target, source = target, kw...) =
+    single_double_layer(; op, target, source, derivative = true, kw...)
 
 end # module

@@ -124,5 +124,33 @@ def weights_backward_error(Q, elems, A, th, tau, rows, rpS, ciS, vS, rpD, ciD, v
     return etas
 
 
+def theta_spread(Q, elems, A, tau, rows, sigma, cS, cD, y_ref, draws=3, seed=0):
+    """How far the corrected rows move when Theta is perturbed at the level
+    an FP64 summation may leave in it. Theta's row t is a sum over all n
+    sources whose terms have magnitudes adding up to tau_t; the rounding
+    errors of such a sum, in any order, grow like a random walk, about
+    sqrt(n) * 2.2e-16 * tau_t (the oracle sums sequentially, the GPU in
+    tiles). Each row gets a random perturbation of that norm, the
+    per-element least squares are re-solved (numpy lstsq) and the change of
+    the rows of cS dS + cD dD applied to sigma (dD = W[1:nq],
+    dS = -W[nq+1:2nq]) is measured. Returns the largest relative change
+    over `draws` draws."""
+    rng = np.random.default_rng(seed)
+    nq = Q.nq
+    worst = 0.0
+    for _ in range(draws):
+        dy = np.zeros((len(rows),) + np.shape(sigma)[1:], dtype=np.complex128)
+        for e, el in enumerate(elems):
+            cols = Q.qtags[:, el]
+            ns = A[e].shape[0]
+            xi = rng.normal(size=(ns, nq)) + (1j * rng.normal(size=(ns, nq)) if np.iscomplexobj(A[e]) else 0)
+            xi *= np.sqrt(len(Q)) * 2.2e-16 * tau[e * nq:(e + 1) * nq][None, :] / np.linalg.norm(xi, axis=0)[None, :]
+            dW = np.linalg.lstsq(A[e], xi, rcond=None)[0]           # 2nq x nq: one column per target
+            comb = cD * dW[:nq] - cS * dW[nq:]                      # weights on the element's nodes
+            dy[e * nq:(e + 1) * nq] = comb.T @ sigma[cols]
+        worst = max(worst, float(np.linalg.norm(dy) / np.linalg.norm(y_ref)))
+    return worst
+
+
 def rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))

@@ -266,6 +266,14 @@ def test_gpu_pullback_of_the_corrected_operator(inti):
     M = 0.5 * np.eye(len(Q)) + Dm + dD.to_scipy().toarray() - 1j * k * (Sm + dS.to_scipy().toarray())
     ref = M.conj().T @ u
     assert np.linalg.norm(g["sigma"] - ref) <= 1e-12 * np.linalg.norm(ref)
+    # the correction's adjoint is a gather (one writer per entry, fixed
+    # order): the same bits on every call, in both summation modes
+    for mode in (0, 2):
+        L.iop.set_symmetric(mode)
+        a = L.iop.pullback(u, s, wrt=("sigma",))["sigma"]
+        for _ in range(3):
+            assert np.array_equal(L.iop.pullback(u, s, wrt=("sigma",))["sigma"], a)
+        assert np.linalg.norm(a - ref) <= 1e-12 * np.linalg.norm(ref)
     with pytest.raises(inti.IntiError):
         L.iop.pullback(u, s, wrt=("source_nodes",))
 

@@ -19,8 +19,10 @@ Bars (each stated where it is asserted):
   * DIM weights: backward error eta <= 1e-13 against the oracle's own
     least-squares data (tests/_dimcheck.py explains why the forward error of
     these ill-conditioned solves is not a usable bar);
-  * corrected rows: <= max(1e-12, 10 x the oracle's own rounding spread,
-    i.e. the oracle built with FMA contraction vs without);
+  * corrected rows: <= max(1e-12, 10 x the oracle's own rounding spread
+    (built with FMA contraction vs without), the spread of the rows under
+    perturbations of Theta at the size of FP64 summation errors of its
+    n-term sums, sqrt(n) eps tau (tests/_dimcheck.py: theta_spread));
   * MIXED: <= 1e-6 against FP64."""
 import sys
 
@@ -59,7 +61,7 @@ def _sampled_dim_check(oracle, pde, dim, k, Q, mu, elems, corr, label):
     say(f"  {label}: weights backward error GPU max {eta_gpu.max():.2e} (oracle {eta_or.max():.2e}), "
         f"{len(rows)} rows of {len(elems)} elements")
     assert eta_gpu.max() <= dc.ETA_BAR, eta_gpu.max()
-    return rows, (rp, ci, vS, vD), (rpf, cif, vSf, vDf), (grpS, gciS, gvS), (grpD, gciD, gvD)
+    return rows, (rp, ci, vS, vD), (rpf, cif, vSf, vDf), (grpS, gciS, gvS), (grpD, gciD, gvD), (A, tau)
 
 
 # ------------------------------------------------------------------- C1
@@ -84,7 +86,7 @@ def test_c1_circle_pipeline(inti, oracle, bench):
     corr = inti.bdim_correction(op, Q, Q, Sop, Dop, green_multiplier=mu)
     # pattern (every row) and weights' backward error (every element)
     elems = np.arange(Q.ne)
-    rows, orc, orf, gS, gD = _sampled_dim_check(oracle, 0, 2, 0.0, Q, mu, elems, corr, "C1")
+    rows, orc, orf, gS, gD, (A_, tau_) = _sampled_dim_check(oracle, 0, 2, 0.0, Q, mu, elems, corr, "C1")
     rp, ci, vS, vD = oracle.bdim(0, 2, 0.0, Q, Q, mu)[:4]
     assert np.array_equal(gD[0], rp) and np.array_equal(gD[1], ci)
     Am = oracle.assemble(0, 2, 1, 0.0, Q, Q).real
@@ -123,7 +125,7 @@ def test_c2_cfie_63480(inti, oracle, bench):
     y64 = L64 @ x
     ymx = Lmx @ x
     elems = dc.sample_elements(Q.ne, 48, 2, must=(0, Q.ne - 1))
-    rows, orc, orf, gS, gD = _sampled_dim_check(oracle, 1, 3, k, Q, mu, elems, corr, "C2")
+    rows, orc, orf, gS, gD, (A_, tau_) = _sampled_dim_check(oracle, 1, 3, k, Q, mu, elems, corr, "C2")
     T = dc.subset(Q, rows)
     direct = oracle.matvec(1, 3, 4, k, T, Q, x, coefs=(-1j * k, 1.0))
     c_gpu = dc.csr_rows_apply(gD[0], gD[1], gD[2] - 1j * k * gS[2], rows, x)
@@ -131,14 +133,16 @@ def test_c2_cfie_63480(inti, oracle, bench):
     y_or = direct + dc.csr_rows_apply(orc[0], orc[1], orc[3] - 1j * k * orc[2], rows, x)
     y_orf = direct + dc.csr_rows_apply(orf[0], orf[1], orf[3] - 1j * k * orf[2], rows, x)
     spread = dc.rel(y_orf, y_or)
+    tspread = dc.theta_spread(Q, elems, A_, tau_, rows, x, -1j * k, 1.0, y_or)
     e_corr = dc.rel(y64[rows], y_or)
     e_mx = dc.rel(ymx[rows], y_or)
     e_mx_full = dc.rel(ymx, y64)
     say(f"  C2: {len(rows)} rows: direct sum {e_direct:.2e}, corrected {e_corr:.2e} (oracle spread "
-        f"{spread:.2e}), mixed vs oracle {e_mx:.2e}, mixed vs FP64 (all rows) {e_mx_full:.2e}")
+        f"{spread:.2e}, Theta spread {tspread:.2e}), mixed vs oracle {e_mx:.2e}, mixed vs FP64 (all rows) "
+        f"{e_mx_full:.2e}")
     assert len(rows) >= 256
     assert e_direct <= 1e-12
-    assert e_corr <= max(1e-12, 10 * spread)
+    assert e_corr <= max(1e-12, 10 * spread, tspread)
     assert e_mx <= 1e-6 and e_mx_full <= 1e-6
 
 
@@ -179,7 +183,7 @@ def test_c3_batch_256(inti, oracle, bench):
         Q, k = qs[p], ks[p]
         mu = np.full(len(Q), -0.5)
         elems = dc.sample_elements(Q.ne, 12, p, must=(0,))
-        rows, orc, orf, gS, gD = _sampled_dim_check(oracle, 1, 2, k, Q, mu, elems, corr[p], f"C3 p={p}")
+        rows, orc, orf, gS, gD, (A_, tau_) = _sampled_dim_check(oracle, 1, 2, k, Q, mu, elems, corr[p], f"C3 p={p}")
         T = dc.subset(Q, rows)
         direct = oracle.matvec(1, 2, 1, k, T, Q, Xs[p])
         c_gpu = dc.csr_rows_apply(gD[0], gD[1], gD[2], rows, Xs[p])
@@ -188,11 +192,13 @@ def test_c3_batch_256(inti, oracle, bench):
         y_or = 0.5 * Xs[p][rows] + direct + dc.csr_rows_apply(orc[0], orc[1], orc[3], rows, Xs[p])
         y_orf = 0.5 * Xs[p][rows] + direct + dc.csr_rows_apply(orf[0], orf[1], orf[3], rows, Xs[p])
         spread = dc.rel(y_orf, y_or)
+        tspread = dc.theta_spread(Q, elems, A_, tau_, rows, Xs[p], 0.0, 1.0, y_or)
         e_corr = dc.rel(Ys[p][rows], y_or)
         say(f"  C3 p={p}: N={len(Q)}, k={k:.2f}, kd={kd[p]:.1f}: direct sum {e_direct:.2e} (mixed "
-            f"{e_mx_direct:.2e}), corrected {e_corr:.2e} (oracle spread {spread:.2e})")
+            f"{e_mx_direct:.2e}), corrected {e_corr:.2e} (oracle spread {spread:.2e}, Theta spread "
+            f"{tspread:.2e})")
         assert e_direct <= 1e-12 and e_mx_direct <= 1e-6
-        assert e_corr <= max(1e-12, 10 * spread)
+        assert e_corr <= max(1e-12, 10 * spread, tspread)
     assert len(pick) >= 8
 
 
@@ -217,7 +223,7 @@ def test_c4_bumpy_491520(inti, oracle, bench):
     y32 = S32 @ rhs
     say("  C4: applies done")
     elems = dc.sample_elements(Q.ne, 48, 4, must=(0, Q.ne - 1))
-    rows, orc, orf, gS, gD = _sampled_dim_check(oracle, 0, 3, 0.0, Q, mu, elems, corr, "C4")
+    rows, orc, orf, gS, gD, (A_, tau_) = _sampled_dim_check(oracle, 0, 3, 0.0, Q, mu, elems, corr, "C4")
     T = dc.subset(Q, rows)
     direct = oracle.matvec(0, 3, 0, 0.0, T, Q, rhs).real
     c_gpu = dc.csr_rows_apply(gS[0], gS[1], gS[2], rows, rhs)
@@ -225,12 +231,14 @@ def test_c4_bumpy_491520(inti, oracle, bench):
     y_or = direct + dc.csr_rows_apply(orc[0], orc[1], orc[2], rows, rhs).real
     y_orf = direct + dc.csr_rows_apply(orf[0], orf[1], orf[2], rows, rhs).real
     spread = dc.rel(y_orf, y_or)
+    tspread = dc.theta_spread(Q, elems, A_, tau_, rows, rhs, 1.0, 0.0, y_or)
     e_corr = dc.rel(y64[rows], y_or)
     e_mx, e_mx_full = dc.rel(y32[rows], y_or), dc.rel(y32, y64)
     say(f"  C4: {len(rows)} rows: direct sum {e_direct:.2e}, corrected {e_corr:.2e} (oracle spread "
-        f"{spread:.2e}), mixed vs oracle {e_mx:.2e}, mixed vs FP64 (all rows) {e_mx_full:.2e}")
+        f"{spread:.2e}, Theta spread {tspread:.2e}), mixed vs oracle {e_mx:.2e}, mixed vs FP64 (all rows) "
+        f"{e_mx_full:.2e}")
     assert e_direct <= 1e-12
-    assert e_corr <= max(1e-12, 10 * spread)
+    assert e_corr <= max(1e-12, 10 * spread, tspread)
     assert e_mx <= 1e-6 and e_mx_full <= 1e-6
 
 
