@@ -140,6 +140,24 @@ int inti_op_apply(inti_op *op, int nrhs, const double *x, double *y,
 int inti_op_apply_device(inti_op *op, int nrhs, const double *x, int64_t ldx,
                          double *y, int64_t ldy, const double *alpha,
                          const double *beta);
+/* inti_op_apply_device without any device allocation. No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc). The
+ * one-direction direct sum runs (never the symmetric one) with its partial
+ * sums in a static buffer of INTI_NOALLOC_SCRATCH_BYTES that the library
+ * holds on every GPU it runs on until the process ends; the source split is
+ * capped so that the partials of a block of columns (16, 4 or 1) fit it.
+ * Shapes whose partials do not fit even unsplit (m * 16 columns * 16 bytes
+ * for complex operators with 16 or more right-hand sides, i.e. more than
+ * 262,144 targets; 4,194,304 targets for one complex column) return
+ * INTI_ERR_SIZE. The correction must already be attached (attaching
+ * allocates). Every call on one GPU shares that buffer: calls that overlap
+ * on one GPU (several contexts or host threads on the same device, or
+ * streams running concurrently) race; calls on one context are serialised. */
+#define INTI_NOALLOC_SCRATCH_BYTES (64u << 20)
+int inti_op_apply_device_noalloc(inti_op *op, int nrhs, const double *x,
+                                 int64_t ldx, double *y, int64_t ldy,
+                                 const double *alpha, const double *beta);
 /* Dense column-major m x n matrix of the operator, written to host memory
  * (assemble_matrix, src/nystrom.jl:103-124). */
 int inti_op_assemble_dense(inti_op *op, double *out);

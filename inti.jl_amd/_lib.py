@@ -54,6 +54,7 @@ SIGNATURES = {
     "inti_op_set_symmetric": (C.c_int, [_vp, C.c_int, C.c_int]),
     "inti_op_apply": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
     "inti_op_apply_device": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64, _dp, _dp]),
+    "inti_op_apply_device_noalloc": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64, _dp, _dp]),
     "inti_op_assemble_dense": (C.c_int, [_vp, _dp]),
     "inti_op_adjoint": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "inti_op_assemble_dense_device": (C.c_int, [_vp, _vp, C.c_int64]),

@@ -12,6 +12,14 @@ namespace inti {
 
 static thread_local std::string g_create_err;
 
+// Partial sums of the allocation-free apply (inti_op_apply_device_noalloc):
+// a static buffer that every GPU the library runs on holds from load until
+// the process ends. Host code never uses this name as an address: it is a
+// host-side stand-in; the device address is looked up once per context with
+// hipGetSymbolAddress (Ctx::noalloc_scratch).
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
+static __device__ __align__(256) double scratch[kNoAllocBytes / sizeof(double)];
+
 int set_error(Ctx *ctx, int code, const std::string &msg) {
     if (ctx) ctx->err = msg;
     else g_create_err = msg;
@@ -106,6 +114,18 @@ int inti_ctx_create(int device, inti_ctx **out) {
         delete c;
         return set_error(nullptr, INTI_ERR_HIP, "stream/event creation failed");
     }
+    // this device's copy of the static partial-sum buffer (the device is
+    // current here)
+    void *sp = nullptr;
+    const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (se != hipSuccess || sp == nullptr) {
+        (void)hipEventDestroy(c->ev0);
+        (void)hipEventDestroy(c->ev1);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return set_error(nullptr, INTI_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se));
+    }
+    c->noalloc_scratch = (double *)sp;
     *out = c;
     return INTI_OK;
 }
@@ -403,6 +423,29 @@ int inti_op_apply_device(inti_op *op, int nrhs, const double *x, int64_t ldx, do
     REQUIRE(ctx, ldx >= op->src->n && ldy >= op->trg->n, "apply: leading dimension too small");
     LOCK(ctx);
     return apply_device_locked(op, nrhs, x, ldx, y, ldy, alpha, beta);
+}
+
+int inti_op_apply_device_noalloc(inti_op *op, int nrhs, const double *x, int64_t ldx, double *y,
+                                 int64_t ldy, const double *alpha, const double *beta) {
+    if (!op) return INTI_ERR_ARG;
+    Ctx *ctx = op->ctx;
+    REQUIRE(ctx, nrhs >= 0, "apply_noalloc: nrhs < 0");
+    REQUIRE(ctx, nrhs == 0 || (x && y), "apply_noalloc: null operand");
+    REQUIRE(ctx, ldx >= op->src->n && ldy >= op->trg->n, "apply_noalloc: leading dimension too small");
+    if (!ctx->noalloc_scratch) return set_error(ctx, INTI_ERR_HIP, "apply_noalloc: no scratch address");
+    LOCK(ctx);
+    DirectLaunch L;
+    L.op = op; L.nrhs = nrhs; L.x = x; L.ldx = ldx; L.y = y; L.ldy = ldy;
+    L.alpha[0] = alpha ? alpha[0] : 1.0; L.alpha[1] = alpha ? alpha[1] : 0.0;
+    L.beta[0] = beta ? beta[0] : 0.0; L.beta[1] = beta ? beta[1] : 0.0;
+    if (op->pde == INTI_LAPLACE && (L.alpha[1] != 0.0 || L.beta[1] != 0.0))
+        return set_error(ctx, INTI_ERR_UNSUPPORTED, "apply_noalloc: Laplace operators take real alpha/beta");
+    // every partial slot the reduction reads is written by the sum of the
+    // same call (each block writes its own chunk's partial of every row of
+    // its tile): nothing needs clearing between calls
+    L.part = ctx->noalloc_scratch;
+    L.part_bytes = kNoAllocBytes;
+    return launch_apply(L);
 }
 
 int inti_op_apply(inti_op *op, int nrhs, const double *x, double *y, const double *alpha,

@@ -34,6 +34,9 @@ struct Ctx {
     // 1-2 staging of host operands, 3 correction-build temporaries
     void *scratch[4] = {nullptr, nullptr, nullptr, nullptr};
     size_t scratch_bytes[4] = {0, 0, 0, 0};
+    // device address of the library's static partial-sum buffer on this
+    // context's device (the allocation-free apply)
+    double *noalloc_scratch = nullptr;
 };
 
 struct Quad {
@@ -105,6 +108,11 @@ struct DirectLaunch {
     double *y;        // device, m x nrhs
     int64_t ldy;
     double alpha[2], beta[2];
+    // caller-provided partial-sum buffer: when set, the one-direction sum
+    // runs with its source split capped so that nr x split x m partials fit
+    // part_bytes, and nothing is allocated
+    double *part = nullptr;
+    size_t part_bytes = 0;
 };
 int launch_apply(const DirectLaunch &L);
 // allocate the scratch an apply of nrhs columns needs (before graph capture)

@@ -249,16 +249,21 @@ int launch_apply(const DirectLaunch &L) {
     if (m == 0 || L.nrhs == 0) return INTI_OK;
     int kind = op->kind;
     const KParams kp = op_params(op, &kind);
+    const bool given = L.part != nullptr;
     void *scr = nullptr;
-    int st = apply_reserve(op, L.nrhs);
-    if (st) return st;
-    st = ctx_scratch(ctx, kScrPartial, 0, &scr);
-    if (st) return st;
-    double *part = (double *)scr;
+    double *part = L.part;
+    if (!given) {
+        int st = apply_reserve(op, L.nrhs);
+        if (st) return st;
+        st = ctx_scratch(ctx, kScrPartial, 0, &scr);
+        if (st) return st;
+        part = (double *)scr;
+    }
+    int st = INTI_OK;
     const int64_t ntiles = (m + kApplyBS - 1) / kApplyBS;
     int symG = 0;
     int64_t symT = 0, symS = 0;
-    const bool sym = plan_sym(op, kind, 1, symG, symT, symS);
+    const bool sym = !given && plan_sym(op, kind, 1, symG, symT, symS);
     if (sym) {
         const size_t bytes = sizeof(double) * cw * (size_t)symS * (size_t)n;
         st = ctx_scratch(ctx, kScrPartial, bytes, &scr);
@@ -274,6 +279,19 @@ int launch_apply(const DirectLaunch &L) {
         int split = 1;
         int64_t chunk = n;
         if (fn) plan_split(op, m, n, fn, nr, split, chunk);
+        if (given && n > 0) {
+            // the largest split whose partials fit the caller's buffer, in
+            // chunks of whole 256-source tiles
+            const size_t per = sizeof(double) * cw * (size_t)m * nr;
+            const int64_t cap = (int64_t)(L.part_bytes / per);
+            if (cap < 1)
+                return set_error(ctx, INTI_ERR_SIZE, "apply: the partial sums of this shape exceed the buffer");
+            if (split > cap) {
+                const int64_t stiles = (n + kApplyBS - 1) / kApplyBS;
+                chunk = (stiles + cap - 1) / cap * kApplyBS;
+                split = (int)((n + chunk - 1) / chunk);
+            }
+        }
         int nsplit = split;
         if (n > 0 && nr == 1 && sym) {
             SymFn sfn = pick_sym(op->pde, op->dim, kind, op->precision == INTI_MIXED, symG);
