@@ -126,9 +126,64 @@ def cpu_baseline(oracle, Q, x, k, target_seconds=15.0):
         oracle.matvec(1, 3, 4, k, Q, Q, x, coefs=coefs, rows=(0, rows), nthreads=threads)
         t = time.perf_counter() - t0
     rate = rows * n / t
+    # one thread on a sample of about 4 s
+    rows1 = max(1, min(n, int(rows * 4.0 / max(t, 1e-3) / max(threads, 1))))
+    t0 = time.perf_counter()
+    oracle.matvec(1, 3, 4, k, Q, Q, x, coefs=coefs, rows=(0, rows1), nthreads=1)
+    t1 = time.perf_counter() - t0
     return {"value": rate / 1e9, "unit": "Gpair-evals/s", "cores": threads, "kind": "port",
             "sample": f"oracle FP64 direct sum of the same combined-field operator, rows 0..{rows} "
-                      f"of {n} against all {n} sources ({rows * n:.3g} pairs, {t:.1f} s)"}
+                      f"of {n} against all {n} sources ({rows * n:.3g} pairs, {t:.1f} s)",
+            "cpu_model": _cpu_model(), "host_cpus_visible": os.cpu_count(),
+            "single_thread": {"value": rows1 * n / t1 / 1e9, "unit": "Gpair-evals/s",
+                              "sample": f"rows 0..{rows1} ({rows1 * n:.3g} pairs, {t1:.1f} s)"}}
+
+
+def _cpu_model():
+    """The host CPU's model name (the `Model name` line of lscpu)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def sym_group(n, value_bytes):
+    """Tiles of 256 nodes per work group of the symmetric sum, by the
+    library's rule: the smallest of 1, 2, 4 whose ceil(T / G) x n partial
+    sums (T = ceil(n / 256) tiles) fit 4 GiB."""
+    T = -(-n // 256)
+    for g in (1, 2, 4):
+        if -(-T // g) * n * value_bytes <= 4 << 30:
+            return g
+    return None
+
+
+MICRO_OPS = ["v_fma_f64", "v_fma_f32", "v_rsq_f32", "v_sin_f32", "v_cos_f32", "v_log_f32", "v_rcp_f32", "v_rsq_f64"]
+
+
+def measured_roofs(ctx, lib, flops_per_pair, sfu_per_pair):
+    """Issue rates of the FMA and transcendental instructions over the whole
+    GPU (inti_microbench: lane-operations per second) and the two pair-rate
+    roofs of a kernel with the given canonical FP64 flops and FP32
+    transcendentals per pair; the smaller one binds."""
+    rates = {}
+    for op, name in enumerate(MICRO_OPS):
+        r = C.c_double()
+        ctx.check(lib.inti_microbench(ctx.handle, op, C.byref(r)))
+        rates[name] = r.value
+    valu_flops = 2.0 * rates["v_fma_f64"]
+    sfu = min(rates["v_rsq_f32"], rates["v_sin_f32"], rates["v_cos_f32"])
+    valu_roof = valu_flops / flops_per_pair
+    sfu_roof = sfu / sfu_per_pair
+    return {"lane_ops_per_s": {k: round(v / 1e12, 3) for k, v in rates.items()}, "unit": "T lane-ops/s",
+            "fp64_fma_TFLOPs": round(valu_flops / 1e12, 2),
+            "valu_roof_Gpair_s": round(valu_roof / 1e9, 1), "sfu_roof_Gpair_s": round(sfu_roof / 1e9, 1),
+            "binding": "valu" if valu_roof <= sfu_roof else "sfu",
+            "per_pair": {"flops": flops_per_pair, "f32_transcendentals": sfu_per_pair}}
 
 
 def calibrate_copy(ctx, lib, nbytes):
@@ -231,13 +286,18 @@ def run_cfie(inti, lib, ctx, args):
     err_mx_inf = float(np.abs(ymx - y64).max() / np.abs(y64).max())
     log(f"bench: fp64 {t64 * 1e3:.3f} ms/apply (kernel {k64:.3f} ms), "
         f"mixed {tmx * 1e3:.3f} ms/apply (kernel {kmx:.3f} ms), rel err {err_mx:.2e}")
-    # the plain FP64 kernel (one direction per pair, no symmetric evaluation)
-    t64p = None
+    # the same matvec by the one-direction kernels (each pair evaluated once
+    # per direction) in both precisions: the mixed/FP64 ratio of that algorithm
+    t64p = tmxp = None
     if args.sym != 0:
-        L64.iop.set_symmetric(0)
+        mode = args.sym if args.sym is not None else 1
+        for L in (L64, Lmx):
+            L.iop.set_symmetric(0)
         t64p, _ = time_apply(ctx, lib, L64.iop.handle, dx, dy, n, max(3, args.steps // 4), 1)
-        L64.iop.set_symmetric(args.sym if args.sym is not None else 1)
-        log(f"bench: plain fp64 (one direction per pair) {t64p * 1e3:.3f} ms/apply")
+        tmxp, _ = time_apply(ctx, lib, Lmx.iop.handle, dx, dy, n, max(3, args.steps // 4), 1)
+        for L in (L64, Lmx):
+            L.iop.set_symmetric(mode)
+        log(f"bench: one-direction kernels: fp64 {t64p * 1e3:.3f} ms/apply, mixed {tmxp * 1e3:.3f} ms/apply")
 
     cpu = None
     err64 = None
@@ -257,6 +317,8 @@ def run_cfie(inti, lib, ctx, args):
         cpu = cpu_baseline(oracle, Q, x, k)
         log(f"bench: cpu baseline {cpu['value']:.4f} Gpair/s on {cpu['cores']} threads")
 
+    roofs = measured_roofs(ctx, lib, FLOPS_PER_PAIR, 3)
+    log(f"bench: measured roofs {roofs}")
     achieved = FLOPS_PER_PAIR * pairs / (kmx * 1e-3) / 1e12
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "pmc_cfie_mixed.json")
@@ -285,10 +347,11 @@ def run_cfie(inti, lib, ctx, args):
             "correction_nnz": corr[1].nnz, "precision": "mixed",
         },
         "fp64_value": round(pairs / t64 / 1e9, 3),
+        # the same algorithm (the symmetric sum) in both precisions
         "mixed_over_fp64": round(t64 / tmx, 3),
-        # the same ratio against the plain FP64 kernel (one direction per pair)
-        "fp64_one_direction_value": round(pairs / t64p / 1e9, 3) if t64p else None,
-        "mixed_over_fp64_one_direction": round(t64p / tmx, 3) if t64p else None,
+        "one_direction_kernels": ({"mixed_value": round(pairs / tmxp / 1e9, 3),
+                                   "fp64_value": round(pairs / t64p / 1e9, 3),
+                                   "mixed_over_fp64": round(t64p / tmxp, 3)} if t64p else None),
         "rel_err_mixed_vs_fp64": err_mx,
         "relinf_err_mixed_vs_fp64": err_mx_inf,
         "rel_err_fp64_vs_oracle_rows": err64,
@@ -297,9 +360,9 @@ def run_cfie(inti, lib, ctx, args):
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_VALU_TFLOPS, 4),
                      "traffic": traffic,
                      "kernel": ("k_apply<HELMHOLTZ,3,COMB_SD,MIXED>" if args.sym == 0 else
-                                "k_apply_sym<HELMHOLTZ,3,COMB_SD,MIXED,G=1,TPL=2>"),
+                                f"k_apply_sym<HELMHOLTZ,3,COMB_SD,MIXED,G={sym_group(n, 16)},TPL=2>"),
                      "kernel_ms": round(kmx, 4), "flops_per_pair": FLOPS_PER_PAIR,
-                     "symmetric": args.sym != 0},
+                     "symmetric": args.sym != 0, "measured_roofs": roofs},
         "cpu_baseline": cpu,
     }
     ctx.check(lib.inti_device_free(ctx.handle, dx))
@@ -578,8 +641,23 @@ def run_bumpy(inti, lib, ctx, args):
     log(f"bench bumpy: fp64 {res['fp64'][0] * 1e3:.1f} ms, mixed {res['mixed'][0] * 1e3:.1f} ms, err {err:.2e}")
     if args.only:  # apply legs only (profiling and launch comparisons), no solves, no JSON
         return None
-    # bounded refined solve on the bumpy sphere (one outer step of 300 inner iterations)
-    xb, ib, tb, _, _, _ = _refined(inti, S64, S32, rhs, 1, 300, "bumpy")
+    # refined solve on the bumpy sphere to a 1e-10 FP64 residual: FP64
+    # residuals, mixed GMRES corrections with no restart inside an outer step
+    # (restarted GMRES(300) stagnates near 1e-5 on this indefinite, non-normal
+    # discretisation; unrestarted GMRES does not), one outer step per call so
+    # that progress is logged
+    t1 = time.perf_counter()
+    xb = np.zeros(n)
+    ib = {"outer": 0, "inner_iterations": 0, "relres": 1.0, "restart": 1000, "rtol": 1e-10}
+    for _ in range(6):
+        xb, step = inti.refine_solve(S64, S32, rhs, x0=xb, restart=1000, maxiter_inner=1000, inner_rtol=1e-6,
+                                     max_outer=1, rtol=1e-10)
+        ib.update(outer=ib["outer"] + step["outer"], inner_iterations=ib["inner_iterations"] + step["inner_iterations"],
+                  relres=step["relres"])
+        log(f"bench bumpy: refinement {ib} at {time.perf_counter() - t1:.1f} s")
+        if step["relres"] <= 1e-10 or step["outer"] == 0:
+            break
+    tb = time.perf_counter() - t1
     del S64, S32
     # the full refinement on the unit sphere of the same size
     Qs = inti.sphere_quadrature(freq)
@@ -603,7 +681,7 @@ def run_bumpy(inti, lib, ctx, args):
         "mixed_over_fp64": round(res["fp64"][0] / res["mixed"][0], 3),
         "rel_err_mixed_vs_fp64": err, "kernel_ms": {k: round(v[1], 3) for k, v in res.items()},
         "dim_build_s": round(t_dim, 2),
-        "bumpy_bounded_refinement": {"seconds": round(tb, 2), **ib},
+        "bumpy_refined_solve": {"seconds": round(tb, 2), **ib},
         "unit_sphere_491520": {"dim_build_s": round(t_dim_s, 2),
                                "refined_solve": {"seconds": round(t_ref, 2), **ir, "rtol": 1e-12},
                                "fp64_gmres": {"seconds": round(t_64, 2), **i64},
@@ -612,125 +690,183 @@ def run_bumpy(inti, lib, ctx, args):
 
 
 def run_shape(inti, lib, ctx, args):
-    """Shape-gradient objective: sound-soft scattering of e^{ikz} by the
-    sphere r = 1 + 0.1 sin(3 theta) cos(2 phi), k = 10 (icosphere frequency
-    29: 100,920 nodes), sigma from the corrected CFIE by GMRES;
-    J = sum over 64 Fibonacci directions of |u_inf|^2. The adjoint gives
-    dJ/d(sigma, nodes, normals, weights, k); checked by central differences
-    (h = 1e-6) along 20 seeded random directions; the adjoint of the full
+    """Shape gradient of sound-soft scattering: e^{ikz} on the sphere
+    r = 1 + 0.1 sin(3 theta) cos(2 phi), k = 10 (icosphere frequency 29:
+    100,920 nodes); sigma from the DIM-corrected CFIE by GMRES;
+    J = sum over 64 Fibonacci directions of |u_inf|^2. The adjoint state
+    (shape_gradient) gives dJ/d(nodes, normals, weights, k) through the
+    solve; checked by central differences (h = 1e-6) of the whole solve along
+    20 seeded random directions, with the DIM correction frozen (what the
+    adjoint differentiates) and rebuilt (the full solve). The pullback of the
     N x N combined-field direct sum is timed against its forward."""
     k = 10.0
     Q = shape_quadrature(inti)
     n = len(Q)
     op = inti.Helmholtz(k=k, dim=3)
+    dirs = fibonacci_directions(64)
     if args.only:  # profiling runs: a random density instead of the solve, kernels only, no JSON
         rng = np.random.default_rng(1)
         sig = rng.normal(size=n) + 1j * rng.normal(size=n)
         A = inti.IntegralOperator(inti.CombinedFieldKernel(op, -1j * k, 1.0), Q)
         for _ in range(3):
             A.apply(sig)
-            A.pullback(sig, sig, wrt=("sigma", "source_nodes", "source_normals", "source_weights", "k"))
-            A.pullback(sig, sig, wrt=("sigma", "source_nodes", "target_nodes"))
+            A.pullback(sig, sig, wrt=("sigma", "nodes", "source_normals", "source_weights", "k"))
+            A.pullback(sig, sig, wrt=("sigma", "source_nodes", "source_normals", "source_weights",
+                                      "target_nodes", "k"))
         ctx.synchronize()
         log("bench shape: profiling legs done")
         return None
-    L, _ = inti.combined_field(op=op, quad=Q, c_single=-1j * k, c_double=1.0, c_identity=0.5)
-    b = -np.exp(1j * k * Q.coords[2])
     t0 = time.perf_counter()
-    sig, info = inti.gmres(L, b, restart=100, maxiter=300, rtol=1e-8)
-    t_solve = time.perf_counter() - t0
-    P = inti.points(fibonacci_directions(64))
-
-    def quad(y, nv, w):
-        return inti.Quadrature(y, nv, w, np.arange(n, dtype=np.int64).reshape(1, -1))
-
-    def J(y, nv, w, kk, s):
-        o = inti.Helmholtz(k=kk, dim=3)
-        F = inti.IntegralOperator(inti.CombinedFieldKernel(o, -1j * kk, 1.0, "FF"), P, quad(y, nv, w))
-        u = F.apply(s)
-        return float(np.sum(np.abs(u) ** 2)), u
-
-    base = (Q.coords, Q.normals, Q.weights, k, sig)
-    J0, u0 = J(*base)
-    F = inti.IntegralOperator(inti.CombinedFieldKernel(op, -1j * k, 1.0, "FF"), P, quad(*base[:3]))
-    wrt = ("sigma", "source_nodes", "source_normals", "source_weights", "k")
-    F.pullback(2 * u0, sig, wrt=wrt, dcoefs=(-1j, 0.0))
-    ctx.synchronize()
-    reps = 10
-    ka, kf = [], []
+    J0, grad, info = shape_gradient(inti, Q.coords, Q.normals, Q.weights, Q.qtags, k, dirs)
+    t_grad = time.perf_counter() - t0
+    log(f"bench shape: nodes {n}, J {J0:.6e}, solve {info['solve']}, adjoint solve {info['adjoint_solve']}, "
+        f"objective + gradient {t_grad:.2f} s")
     t0 = time.perf_counter()
-    for _ in range(reps):
-        g = F.pullback(2 * u0, sig, wrt=wrt, dcoefs=(-1j, 0.0))
-        ka.append(ctx.last_kernel_ms())  # source pass + its reduction
-    t_adj_ff = (time.perf_counter() - t0) / reps
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        F.apply(sig)
-        kf.append(ctx.last_kernel_ms())  # direct sum kernel
-    t_fwd_ff = (time.perf_counter() - t0) / reps
-    k_adj_ff, k_fwd_ff = float(np.mean(ka)), float(np.mean(kf))
-    rng = np.random.default_rng(20261202)
-    worst = 0.0
-    checks = []
-    for t in range(20):
-        which = t % 5
-        v = rng.normal(size=np.shape(base[which])) if which != 3 else 1.0
-        if which == 4:
-            v = v + 1j * rng.normal(size=n)
-            pred = float(np.real(np.vdot(g["sigma"], v)))
-        elif which == 3:
-            pred = float(g["k"])
-        else:
-            pred = float(np.sum(g[("source_nodes", "source_normals", "source_weights")[which]] * v))
-        h = 1e-6
-        ap, am = list(base), list(base)
-        ap[which] = base[which] + h * v
-        am[which] = base[which] - h * v
-        fd = (J(*ap)[0] - J(*am)[0]) / (2 * h)
-        rel = abs(fd - pred) / max(abs(pred), 1e-12)
-        worst = max(worst, rel)
-        checks.append(rel)
-    log(f"bench shape: nodes {n}, gmres {info}, J {J0:.6e}, FD worst rel {worst:.2e}")
-    # adjoint of the N x N direct sum (plain combined field) vs its forward
+    checks = shape_fd_check(inti, Q.coords, Q.normals, Q.weights, Q.qtags, k, dirs, grad, info["corrections"],
+                            20, SEED, sigma0=info["sigma"])
+    t_fd = time.perf_counter() - t0
+    e_frozen = [abs(a - fz) / max(abs(fz), 1e-300) for _, a, fz, _ in checks]
+    e_full = [abs(a - fl) / max(abs(fl), 1e-300) for _, a, _, fl in checks]
+    for (w, a, fz, fl), ez, el in zip(checks, e_frozen, e_full):
+        log(f"bench shape: {w:8s} adjoint {a:+.9e} fd(frozen DIM) {fz:+.9e} fd(full) {fl:+.9e} "
+            f"rel {ez:.1e} / {el:.1e}")
+    # the pullback of the N x N combined-field direct sum vs its forward
     A = inti.IntegralOperator(inti.CombinedFieldKernel(op, -1j * k, 1.0), Q)
-    ubar = np.conj(u0.mean()) * np.ones(n) + 0.0j
+    sig = info["sigma"] if "sigma" in info else np.exp(1j * k * Q.coords[0])
+    ubar = np.conj(np.mean(sig)) * np.ones(n) + 0.0j
     nxn = {}
+    legs = (("sigma_only", ("sigma",)),
+            ("all_nodes_combined", ("sigma", "nodes", "source_normals", "source_weights", "k")),
+            ("all_source_and_target_parts", ("sigma", "source_nodes", "source_normals", "source_weights",
+                                             "target_nodes", "k")))
     for label, mode in (("symmetric", 1), ("plain", 0)):
-        # symmetric: one pass over unordered pairs (forward and pullback);
-        # plain: one direction per pair, source-major + target-major passes
         A.set_symmetric(mode)
         A.apply(sig)
-        A.pullback(ubar, sig, wrt=("sigma",))
         ctx.synchronize()
-        A.apply(sig)
-        t_fwd = ctx.last_kernel_ms() * 1e-3
-        A.pullback(ubar, sig, wrt=("sigma",))
-        t_adj_sig = ctx.last_kernel_ms() * 1e-3
-        A.pullback(ubar, sig, wrt=("sigma", "source_nodes", "source_normals", "source_weights", "k"))
-        t_adj_src = ctx.last_kernel_ms() * 1e-3
-        A.pullback(ubar, sig, wrt=("sigma", "source_nodes", "source_normals", "source_weights", "target_nodes", "k"))
-        t_adj_all = ctx.last_kernel_ms() * 1e-3
-        log(f"bench shape: N x N {label}: forward {t_fwd * 1e3:.1f} ms, adjoint (sigma only) "
-            f"{t_adj_sig * 1e3:.1f} ms, (source side) {t_adj_src * 1e3:.1f} ms, (both sides) {t_adj_all * 1e3:.1f} ms")
-        nxn[label] = {"forward": round(t_fwd * 1e3, 2), "adjoint_sigma_only": round(t_adj_sig * 1e3, 2),
-                      "adjoint_source_side": round(t_adj_src * 1e3, 2),
-                      "adjoint_both_sides": round(t_adj_all * 1e3, 2),
-                      "ratio_source_side": round(t_adj_src / t_fwd, 3),
-                      "ratio_both_sides": round(t_adj_all / t_fwd, 3)}
+        fw = []
+        for _ in range(3):
+            A.apply(sig)
+            fw.append(ctx.last_kernel_ms())
+        t_fwd = float(np.median(fw))
+        row = {"forward": round(t_fwd, 3)}
+        for leg, wrt in legs:
+            A.pullback(ubar, sig, wrt=wrt, dcoefs=(-1j, 0.0))
+            ts = []
+            for _ in range(3):
+                A.pullback(ubar, sig, wrt=wrt, dcoefs=(-1j, 0.0))
+                ts.append(ctx.last_kernel_ms())
+            row[leg] = round(float(np.median(ts)), 3)
+            row[leg + "_over_forward"] = round(float(np.median(ts)) / t_fwd, 3)
+        log(f"bench shape: N x N {label} kernels (ms): {row}")
+        nxn[label] = row
+    sym = nxn["symmetric"]
     return {
-        "metric": "adjoint / forward time of the far-field objective's pullback (shape gradient, k=10, ~1e5 nodes)",
-        "value": round(k_adj_ff / k_fwd_ff, 3), "unit": "ratio", "n_gpus": 1, "steps": reps, "warmup": 1,
-        "ms_per_step": round(k_adj_ff, 4), "higher_is_better": False, "scaling": "weak",
+        "metric": "adjoint / forward kernel time of the N x N combined-field pullback (all gradients: sigma, nodes, "
+                  "normals, weights, k), shape gradient workload, 100,920 nodes",
+        "value": sym["all_nodes_combined_over_forward"], "unit": "ratio", "n_gpus": 1, "steps": 3, "warmup": 1,
+        "ms_per_step": sym["all_nodes_combined"], "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"sound-soft 3D Helmholtz k=10, sphere r = 1 + 0.1 sin(3 theta) cos(2 phi), "
                                f"icosphere freq 29: {Q.ne} P2 triangles x VR4N6 = {n} nodes, plane wave +z, "
                                f"J = sum over 64 Fibonacci directions |u_inf|^2", "nodes": n},
-        "solve": {"seconds": round(t_solve, 2), **info},
-        "objective": J0, "fd_checks": {"directions": 20, "h": 1e-6, "worst_rel_err": worst},
-        "farfield_kernel_ms": {"forward": round(k_fwd_ff, 4), "adjoint": round(k_adj_ff, 4)},
-        "farfield_call_ms": {"forward": round(t_fwd_ff * 1e3, 3), "adjoint": round(t_adj_ff * 1e3, 3)},
+        "objective": J0, "solve": info["solve"], "adjoint_solve": info["adjoint_solve"],
+        "objective_and_gradient_s": round(t_grad, 2),
+        "fd_checks": {"directions": 20, "h": 1e-6, "seconds": round(t_fd, 1),
+                      "worst_rel_err_vs_fd_frozen_dim": max(e_frozen),
+                      "worst_rel_err_vs_fd_full_solve": max(e_full),
+                      "median_rel_err_vs_fd_full_solve": float(np.median(e_full)),
+                      "by_direction": [{"wrt": w, "adjoint": a, "fd_frozen_dim": fz, "fd_full": fl}
+                                       for w, a, fz, fl in checks]},
         "direct_sum_NxN_kernel_ms": nxn,
     }
+
+
+def shape_problem(inti, y, nv, w, qtags, k, dirs, frozen=None, rtol=1e-12, x0=None):
+    """J = sum over the directions of |u_inf|^2 for the sound-soft scattering
+    of e^{i k z} by the surface with quadrature (y, nv, w, qtags): sigma from
+    the DIM-corrected combined field (I/2 + D - i k S) sigma = -e^{i k z},
+    u_inf = F sigma with F the far field of D - i k S. frozen = (dS, dD)
+    keeps those corrections instead of rebuilding them on this geometry; x0
+    is GMRES's initial guess. Returns (J, u_inf, sigma, L, F, corrections,
+    gmres info)."""
+    Q = inti.Quadrature(y, nv, w, qtags)
+    op = inti.Helmholtz(k=k, dim=3)
+    if frozen is None:
+        L, corr = inti.combined_field(op=op, quad=Q, c_single=-1j * k, c_double=1.0, c_identity=0.5)
+    else:
+        A = inti.IntegralOperator(inti.CombinedFieldKernel(op, -1j * k, 1.0), Q)
+        A.attach(frozen[0], -1j * k)
+        A.attach(frozen[1], 1.0)
+        A.set_identity(0.5)
+        L, corr = A, frozen
+    b = -np.exp(1j * k * y[2])
+    sig, info = inti.gmres(L, b, x0=x0, restart=200, maxiter=600, rtol=rtol)
+    F = inti.IntegralOperator(inti.CombinedFieldKernel(op, -1j * k, 1.0, "FF"), inti.points(dirs),
+                              inti.Quadrature(y, nv, w, np.arange(len(w), dtype=np.int64).reshape(1, -1)))
+    u = F.apply(sig)
+    return float(np.sum(np.abs(u) ** 2)), u, sig, L, F, corr, info
+
+
+def shape_gradient(inti, y, nv, w, qtags, k, dirs):
+    """dJ/d(nodes, normals, weights, k) of shape_problem by the adjoint state:
+    with L sigma = b, L^H lam = F^H ubar (ubar = 2 u_inf),
+        dJ = Re<ubar, dF sigma> + Re<lam, db> - Re<lam, dL sigma>,
+    dF sigma and dL sigma from the pullbacks of the far-field operator and of
+    the combined-field direct sum (node gradient: source and target parts in
+    one pass), db of b = -e^{i k z}, and the k-derivative of the correction's
+    coefficient -i k. The DIM correction matrices themselves are held fixed:
+    their dependence on the geometry (through the per-element least squares
+    and Theta) is not differentiated. Returns (J, grad dict, info)."""
+    n = len(w)
+    J, u, sig, L, F, (dS, dD), sinfo = shape_problem(inti, y, nv, w, qtags, k, dirs)
+    ubar = 2.0 * u
+    gF = F.pullback(ubar, sig, wrt=("sigma", "source_nodes", "source_normals", "source_weights", "k"),
+                    dcoefs=(-1j, 0.0))
+    lam, ainfo = inti.gmres_adjoint(L, gF["sigma"], restart=100, maxiter=600, rtol=1e-12)
+    Q = inti.Quadrature(y, nv, w, qtags)
+    op = inti.Helmholtz(k=k, dim=3)
+    P = inti.IntegralOperator(inti.CombinedFieldKernel(op, -1j * k, 1.0), Q)
+    gL = P.pullback(lam, sig, wrt=("nodes", "source_normals", "source_weights", "k"), dcoefs=(-1j, 0.0))
+    e = np.exp(1j * k * y[2])
+    lc = np.conj(lam)
+    gnodes = gF["source_nodes"] - gL["nodes"]
+    gnodes[2] += np.real(lc * (-1j * k * e))
+    dSs = dS.to_scipy() @ sig
+    gk = gF["k"] - gL["k"] + float(np.real(np.sum(lc * (-1j * y[2] * e)))) - float(np.real(np.vdot(lam, -1j * dSs)))
+    grad = {"nodes": gnodes, "normals": gF["source_normals"] - gL["source_normals"],
+            "weights": gF["source_weights"] - gL["source_weights"], "k": gk}
+    return J, grad, {"solve": sinfo, "adjoint_solve": ainfo, "corrections": (dS, dD), "sigma": sig}
+
+
+def shape_fd_check(inti, y, nv, w, qtags, k, dirs, grad, corr, ndir, seed, h=1e-6, sigma0=None,
+                   rtol=1e-14):
+    """Central differences of J along ndir seeded random directions (nodes,
+    normals, weights and k in turn), with the DIM correction frozen at the
+    base geometry (what the adjoint-state gradient differentiates) and
+    rebuilt on each perturbed geometry (the full corrected solve). The
+    perturbed solves start from the base solution sigma0 and run to rtol:
+    GMRES's stopping error divided by 2h is the FD noise floor (1e-12 / 2e-6
+    would be 5e-7 of J). Returns a list of (which, adjoint, fd_frozen,
+    fd_full)."""
+    rng = np.random.default_rng(seed)
+    base = [y, nv, w, k]
+    names = ("nodes", "normals", "weights", "k")
+    out = []
+    for t in range(ndir):
+        which = t % 4
+        v = rng.normal(size=np.shape(base[which])) if which != 3 else 1.0
+        pred = float(np.sum(grad[names[which]] * v)) if which != 3 else float(grad["k"])
+        fds = []
+        for frozen in (corr, None):
+            vals = []
+            for sgn in (1.0, -1.0):
+                q = list(base)
+                q[which] = base[which] + sgn * h * v
+                vals.append(shape_problem(inti, q[0], q[1], q[2], qtags, q[3], dirs, frozen=frozen,
+                                          rtol=rtol, x0=sigma0)[0])
+            fds.append((vals[0] - vals[1]) / (2 * h))
+        out.append((names[which], pred, fds[0], fds[1]))
+    return out
 
 
 def run_circle(inti, lib, ctx, args):

@@ -97,6 +97,14 @@ int inti_memcpy_d2h(inti_ctx *ctx, void *dst, const void *src, size_t bytes);
  * known HBM traffic, used to calibrate the profiler's byte counters. */
 int inti_device_copy(inti_ctx *ctx, void *dst, const void *src, size_t bytes);
 
+/* Issue rate of one instruction over the whole GPU, in lane-operations per
+ * second (8 independent chains per lane, 32 waves per CU): op 0 v_fma_f64,
+ * 1 v_fma_f32 (non-packed), 2 v_rsq_f32, 3 v_sin_f32, 4 v_cos_f32,
+ * 5 v_log_f32 (with one v_add_f32 per log), 6 v_rcp_f32, 7 v_rsq_f64. No
+ * Inti.jl counterpart: the measured VALU and transcendental roofs of the
+ * direct-sum kernels' roofline. */
+int inti_microbench(inti_ctx *ctx, int op, double *rate);
+
 /* ------------------------------------------------------------- quadrature */
 /* Upload a quadrature (replaces handing Inti.Quadrature's QuadratureNode
  * vector, src/quadrature.jl:7-11,65-70, to native code as the FMM ext does,
@@ -173,7 +181,10 @@ int inti_op_assemble_dense(inti_op *op, double *out);
  *   k_bar: dJ/dk. dcoefs = {Re, Im dc0/dk, Re, Im dc1/dk} for combined kinds
  *     whose coefficients depend on k (e.g. -i k), NULL for constants.
  * For an operator on one surface the gradient of a node is the sum of its
- * source and target parts. Kinds: SL, DL, COMB_SD and the far fields, FP64.
+ * source and target parts; passing the same pointer as src_xyz_bar and
+ * trg_xyz_bar (target == source) asks for that sum directly (fewer sums per
+ * pair in the one-pass pullback). Kinds: SL, DL, COMB_SD and the far fields,
+ * FP64.
  * Geometry and k gradients are of the direct sum: they are refused while a
  * correction is attached (its weights depend on the geometry).
  * No counterpart in the reference, which differentiates with ForwardDiff. */
@@ -269,6 +280,21 @@ int inti_bdim_build(inti_op *opS, inti_op *opD, int derivative, int nq,
                     const int64_t *near_idx, const double *mu,
                     const double *params, inti_csr **dS, inti_csr **dD,
                     double *diag);
+/* bdim_correction over a quadrature with several element types
+ * (src/bdim.jl:79-80 sums nq over the types for the monopole count,
+ * src/bdim.jl:138-185 runs the least squares type by type). Type t has nq[t]
+ * nodes per element and ne[t] elements; qtags holds the nq[t] x ne[t]
+ * column-major blocks one after another; near_ptr holds ne[t] + 1 entries per
+ * type (each block starting at 0) and near_idx the blocks of targets they
+ * index, in the same type order (or NULL/NULL for the self pattern, where the
+ * types must cover every source node once). The other arguments are those of
+ * inti_bdim_build; the CSR entries are summed over the types. */
+int inti_bdim_build_types(inti_op *opS, inti_op *opD, int derivative,
+                          int ntypes, const int *nq, const int64_t *ne,
+                          const int64_t *qtags, const int64_t *near_ptr,
+                          const int64_t *near_idx, const double *mu,
+                          const double *params, inti_csr **dS,
+                          inti_csr **dD, double *diag);
 
 /* Batched per-element least-squares correction: the solve-and-scatter step
  * of vdim_correction (src/vdim.jl:84-127). Element e has a real
@@ -288,6 +314,17 @@ int inti_lsq_build(inti_ctx *ctx, int is_complex, int nrows, int ncols,
                    int64_t ne, const double *A, const int64_t *cols, int64_t m,
                    int64_t n, const int64_t *near_ptr, const int64_t *near_idx,
                    const double *rhs, inti_csr **out, double *diag);
+/* inti_lsq_build over ntypes element types (vdim.jl:76-121 runs the least
+ * squares type by type): type t has ncols[t] nodes per element and ne[t]
+ * elements; A holds the nrows x ncols[t] matrices of its elements after those
+ * of the earlier types, cols the ncols[t] x ne[t] column-major blocks one
+ * after another, near_ptr ne[t] + 1 entries per type (each block from 0) and
+ * near_idx the target blocks in the same order (NULL/NULL: self pattern). */
+int inti_lsq_build_types(inti_ctx *ctx, int is_complex, int nrows, int ntypes,
+                         const int *ncols, const int64_t *ne, const double *A,
+                         const int64_t *cols, int64_t m, int64_t n,
+                         const int64_t *near_ptr, const int64_t *near_idx,
+                         const double *rhs, inti_csr **out, double *diag);
 
 /* ----------------------------------------------------------------- solvers */
 /* Restarted GMRES(restart) for the square operator op (with its attached

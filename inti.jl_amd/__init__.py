@@ -19,11 +19,11 @@ from .kernels import (AdjointDoubleLayerKernel, CombinedFieldKernel, DoubleLayer
                       default_kernel_eltype)
 from .nystrom import (DoubleLayerPotential, IntegralOperator, IntegralPotential,
                       SingleLayerPotential, SparseCorrection, _green_multiplier, assemble_matrix,
-                      gmres, refine_solve, rrule, set_default_symmetric)
-from .quadrature import (Quadrature, boundary_quadrature, circle_quadrature, curve_quadrature,
+                      gmres, gmres_adjoint, refine_solve, rrule, set_default_symmetric)
+from .quadrature import (Quadrature, area_quadrature, boundary_quadrature, box_quadratures, circle_quadrature, curve_quadrature,
                          disk_quadratures, ellipse_disk_quadratures, ellipse_quadrature,
-                         gauss_legendre, gauss_legendre_order, points, sphere_quadrature,
-                         vioreanu_rokhlin_triangle)
+                         gauss_legendre, gauss_legendre_order, join_quadratures, points, sphere_quadrature, surface_quadrature,
+                         vioreanu_rokhlin_tetrahedron, vioreanu_rokhlin_triangle)
 from . import gmsh, polynomials
 from .vdim import vdim_auxiliary_quantities, vdim_correction, volume_potential
 
@@ -37,13 +37,15 @@ __all__ = [
     "IntegralOperator", "IntiError", "set_default_symmetric", "Laplace", "Quadrature", "SingleLayerKernel",
     "SparseCorrection", "_green_multiplier", "adj_double_layer_hypersingular",
     "assemble_matrix", "bdim_correction", "circle_quadrature", "combined_field",
-    "disk_quadratures", "ellipse_disk_quadratures", "boundary_quadrature", "curve_quadrature",
+    "area_quadrature", "disk_quadratures", "ellipse_disk_quadratures", "boundary_quadrature", "curve_quadrature",
     "polynomials", "gmsh", "vdim_auxiliary_quantities", "vdim_correction",
     "volume_potential", "IntegralPotential", "SingleLayerPotential", "DoubleLayerPotential",
     "single_double_layer_potential", "FarFieldSingleLayerKernel", "FarFieldDoubleLayerKernel",
     "rrule", "BatchOperator", "DenseOperator",
     "default_context", "default_density_eltype", "default_kernel_eltype",
-    "ellipse_quadrature", "etype_to_nearest_points", "gauss_legendre", "gauss_legendre_order", "gmres",
+    "ellipse_quadrature", "etype_to_nearest_points", "gauss_legendre", "gauss_legendre_order", "gmres", "join_quadratures",
+    "gmres_adjoint",
     "refine_solve",
-    "points", "single_double_layer", "sphere_quadrature", "vioreanu_rokhlin_triangle",
+    "points", "single_double_layer", "sphere_quadrature", "surface_quadrature", "vioreanu_rokhlin_triangle",
+    "vioreanu_rokhlin_tetrahedron", "box_quadratures",
 ]

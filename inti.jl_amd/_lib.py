@@ -54,6 +54,7 @@ SIGNATURES = {
     "inti_op_set_symmetric": (C.c_int, [_vp, C.c_int, C.c_int]),
     "inti_op_apply": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
     "inti_op_apply_device": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64, _dp, _dp]),
+    "inti_microbench": (C.c_int, [_vp, C.c_int, _dp]),
     "inti_op_apply_device_noalloc": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64, _dp, _dp]),
     "inti_op_assemble_dense": (C.c_int, [_vp, _dp]),
     "inti_op_adjoint": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -78,8 +79,12 @@ SIGNATURES = {
     "inti_op_detach_correction": (C.c_int, [_vp]),
     "inti_bdim_build": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int64, _i64p, _i64p, _i64p,
                                   _dp, _dp, C.POINTER(_vp), C.POINTER(_vp), _dp]),
+    "inti_bdim_build_types": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(C.c_int), _i64p, _i64p, _i64p,
+                                        _i64p, _dp, _dp, C.POINTER(_vp), C.POINTER(_vp), _dp]),
     "inti_lsq_build": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int64, _dp, _i64p, C.c_int64,
                                  C.c_int64, _i64p, _i64p, _dp, C.POINTER(_vp), _dp]),
+    "inti_lsq_build_types": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), _i64p, _dp, _i64p,
+                                       C.c_int64, C.c_int64, _i64p, _i64p, _dp, C.POINTER(_vp), _dp]),
     "inti_gmres": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_double, C.c_double,
                              C.c_int, C.POINTER(C.c_int), _dp]),
     "inti_refine_solve": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_double,

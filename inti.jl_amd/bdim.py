@@ -19,7 +19,8 @@ class DimParameters:
 
 
 def etype_to_nearest_points(target, source, maxdist=np.inf):
-    """Element -> target lists (quadrature.jl:249-286), as CSR over elements.
+    """Element -> target lists (quadrature.jl:249-286), as CSR over elements,
+    one (near_ptr, near_idx) pair per element type.
 
     target is source: every element lists its own nodes (returns None, the
     self pattern). Otherwise every target goes to the element owning its
@@ -36,13 +37,17 @@ def etype_to_nearest_points(target, source, maxdist=np.inf):
     for i, j in enumerate(nearest):
         if j >= 0:
             by_node[j].append(i)
-    near_ptr = np.zeros(source.ne + 1, dtype=np.int64)
-    near_idx = []
-    for e in range(source.ne):
-        for j in source.qtags[:, e]:
-            near_idx.extend(by_node[j])
-        near_ptr[e + 1] = len(near_idx)
-    return near_ptr, np.asarray(near_idx, dtype=np.int64)
+    out = []
+    for qtags in source.etype_qtags:
+        ne = qtags.shape[1]
+        near_ptr = np.zeros(ne + 1, dtype=np.int64)
+        near_idx = []
+        for e in range(ne):
+            for j in qtags[:, e]:
+                near_idx.extend(by_node[j])
+            near_ptr[e + 1] = len(near_idx)
+        out.append((near_ptr, np.asarray(near_idx, dtype=np.int64)))
+    return out[0] if len(out) == 1 else out
 
 
 def bdim_correction(op, target, source, Sop, Dop, *, green_multiplier, parameters=None,
@@ -50,7 +55,9 @@ def bdim_correction(op, target, source, Sop, Dop, *, green_multiplier, parameter
     """Corrections (dS, dD) such that S + dS and D + dD are accurate
     (bdim.jl:50-61). Sop/Dop are the GPU IntegralOperators of SL/DL (or
     ADL/HS if derivative) over (target, source); Theta = mu*G + S*B1 - D*B0 is
-    computed matrix-free on the GPU. Returns two SparseCorrection objects
+    computed matrix-free on the GPU. A source with several element types
+    takes one near list per type (etype_to_nearest_points returns them).
+    Returns two SparseCorrection objects
     (CSR on the device; .to_scipy() for host copies)."""
     if not isinstance(Sop, IntegralOperator) or not isinstance(Dop, IntegralOperator):
         raise TypeError("Sop and Dop must be GPU IntegralOperators over (target, source)")
@@ -66,13 +73,22 @@ def bdim_correction(op, target, source, Sop, Dop, *, green_multiplier, parameter
     pv = np.array([params.sources_oversample_factor, params.sources_radius_multiplier])
     if near is None:
         near = etype_to_nearest_points(target, source, maxdist)
-    near_ptr, near_idx = (None, None) if near is None else near
+    ntypes = len(source.etype_qtags)
+    if near is None:
+        near_ptr = near_idx = None
+    else:
+        lists = [near] if ntypes == 1 else list(near)
+        if len(lists) != ntypes:
+            raise ValueError("near: one (near_ptr, near_idx) pair per element type")
+        near_ptr = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.int64) for p, _ in lists]))
+        near_idx = np.ascontiguousarray(np.concatenate([np.asarray(i, dtype=np.int64) for _, i in lists]))
+    nqs, nes, tags = source.etype_flat()
     ctx = Sop.ctx
     hS, hD = C.c_void_p(), C.c_void_p()
     diag = np.zeros(4)
-    ctx.check(ctx.lib.inti_bdim_build(
-        Sop.handle, Dop.handle, 1 if derivative else 0, source.nq, source.ne,
-        _lib.i64ptr(np.ascontiguousarray(source.qtags.T.reshape(-1))),  # nq x ne column-major
+    ctx.check(ctx.lib.inti_bdim_build_types(
+        Sop.handle, Dop.handle, 1 if derivative else 0, ntypes,
+        nqs.ctypes.data_as(C.POINTER(C.c_int)), _lib.i64ptr(nes), _lib.i64ptr(tags),
         _lib.i64ptr(near_ptr), _lib.i64ptr(near_idx), _lib.dptr(mu), _lib.dptr(pv),
         C.byref(hS), C.byref(hD), _lib.dptr(diag)))
     if diagnostics is not None:

@@ -462,6 +462,7 @@ struct AdjSymArgs {
     double *kpart;  // one per block
     int64_t ntiles;
     int distinct;   // no two nodes within twice the coincidence tolerance
+    int comb;       // one node gradient, source and target parts summed
     AdjParams p;
 };
 
@@ -483,39 +484,48 @@ __device__ __forceinline__ void adj_sym_node(const AdjSymArgs &A, int64_t j, dou
     }
 }
 
-// one ordered pair (target t, source s): adds to the source's sums S0, Y, N
-// and the target's X; returns Re(sc gk)
+// one ordered pair (target t, source s): the source's term of S0 (c_t K),
+// the node-gradient term g = alpha v + beta n_s (Y_s += g, X_t -= g) and
+// gamma (N_s += gamma v); returns Re(sc gk)
 template <int PDE, int DIM, int KIND, bool MASK>
-__device__ __forceinline__ double adj_sym_pair(const double *v, const double *ns, const AdjRadial &R,
-                                               const AdjParams &p, double2 ct, double2 ss,
-                                               double *src, double *trg) {
+__device__ __forceinline__ double adj_sym_terms(const double *v, const double *ns, const AdjRadial &R,
+                                                const AdjParams &p, double2 ct, double2 ss, double2 &cK,
+                                                double *g, double &ga) {
     double2 K, gk, a_r, a_n, b_r;
     adj_combine<PDE, DIM, KIND, MASK>(v, ns, R, p, K, gk, a_r, a_n, b_r);
     const double2 sc = cm(ss, ct);
-    src[0] = __builtin_fma(ct.x, K.x, __builtin_fma(-ct.y, K.y, src[0]));
-    src[1] = __builtin_fma(ct.x, K.y, __builtin_fma(ct.y, K.x, src[1]));
+    cK = make_double2(__builtin_fma(ct.x, K.x, -ct.y * K.y), __builtin_fma(ct.x, K.y, ct.y * K.x));
     const double al = sc.x * a_r.x - sc.y * a_r.y;
     const double be = sc.x * a_n.x - sc.y * a_n.y;
-    const double ga = sc.x * b_r.x - sc.y * b_r.y;
+    ga = sc.x * b_r.x - sc.y * b_r.y;
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        const double gd = __builtin_fma(al, v[d], be * ns[d]);
-        src[2 + d] += gd;
-        trg[2 + DIM + d] -= gd;
-        src[2 + 2 * DIM + d] = __builtin_fma(ga, v[d], src[2 + 2 * DIM + d]);
-    }
+    for (int d = 0; d < DIM; ++d) g[d] = __builtin_fma(al, v[d], be * ns[d]);
     return sc.x * gk.x - sc.y * gk.y;
 }
 
-template <int PDE, int DIM, int KIND>
-__global__ __launch_bounds__(kSymBS) void k_adj_sym(AdjSymArgs A) {
-    constexpr int NR = 3 * DIM + 2;
+// LDS add of one double (ds_add_f64): each lane of a wave adds to its own
+// address, in program order
+__device__ __forceinline__ void lds_add(double *a, double v) {
+    __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// COMB: the node gradient is the sum of the source and target parts (Y + X,
+// asked for by passing the same output for both): NR = 2 DIM + 2 sums
+// [S0 (2), G (DIM), N (DIM)] instead of 3 DIM + 2, and each pair adds its
+// difference g1 - g2 once to each node.
+template <int PDE, int DIM, int KIND, bool COMB>
+__global__ __launch_bounds__(kSymBS) __attribute__((amdgpu_waves_per_eu(COMB ? 3 : 1))) void k_adj_sym(AdjSymArgs A) {
+    constexpr int NR = COMB ? 2 * DIM + 2 : 3 * DIM + 2;
+    constexpr int NW = kSymBS / 64;
+    constexpr int iY = 2, iX = COMB ? 2 : 2 + DIM, iN = COMB ? 2 + DIM : 2 + 2 * DIM;
     __shared__ double s_x[DIM][kSymBS], s_n[DIM][kSymBS];
     __shared__ double2 s_c[kSymBS], s_s[kSymBS];
-    __shared__ double s_red[NR][kSymBS];
+    // per-wave sums of the current 64-source sub-tile: at step s lane l adds
+    // to source (l + s) mod 64, so the 64 lanes of a wave add to 64 different
+    // addresses and each address receives its terms in step order
+    __shared__ double s_acc[NW][NR][64];
     __shared__ double s_k[kSymBS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nxt = ((lane + 1) & 63) << 2;
     int64_t a, b;
     sym_item(blockIdx.x, A.ntiles, a, b);
     const bool diag = a == b;
@@ -533,20 +543,20 @@ __global__ __launch_bounds__(kSymBS) void k_adj_sym(AdjSymArgs A) {
         s_c[tid] = cj;
         s_s[tid] = sj;
     }
-    __syncthreads();
     double own[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) own[r] = 0.0;
     double kb = 0.0;
     for (int u = 0; u < 4; ++u) {
-        double rot[NR];  // sums of source (lane + s) mod 64 of sub-tile u
+        __syncthreads();  // the previous sub-tile's sums have been read; s_x.. are written
+        double *acc = &s_acc[wave][0][0];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) rot[r] = 0.0;
+        for (int r = 0; r < NR; ++r) acc[r * 64 + lane] = 0.0;
         // off-diagonal tiles of distinct nodes hold no coincident pairs
         auto pairs = [&](auto mask_tag) {
             constexpr bool MASK = decltype(mask_tag)::value;
             for (int s = 0; s < 64; ++s) {
-                const int jl = u * 64 + ((lane + s) & 63);
+                const int js = (lane + s) & 63, jl = u * 64 + js;
                 double xj[DIM], nj[DIM], v[DIM], vn[DIM];
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
@@ -557,31 +567,67 @@ __global__ __launch_bounds__(kSymBS) void k_adj_sym(AdjSymArgs A) {
                 }
                 const double2 cj = s_c[jl], sj = s_s[jl];
                 const AdjRadial R = adj_radial<PDE, DIM, KIND, MASK>(dotv<DIM>(v, v), p);
-                kb += adj_sym_pair<PDE, DIM, KIND, MASK>(v, nj, R, p, ci, sj, rot, own);  // i target, j source
-                if (!diag) kb += adj_sym_pair<PDE, DIM, KIND, MASK>(vn, ni, R, p, cj, si, own, rot);  // j target, i source
+                // i target, j source
+                double2 cK1;
+                double g1[DIM], ga1;
+                kb += adj_sym_terms<PDE, DIM, KIND, MASK>(v, nj, R, p, ci, sj, cK1, g1, ga1);
+                double *aj = acc + js;
+                lds_add(aj + 0 * 64, cK1.x);
+                lds_add(aj + 1 * 64, cK1.y);
 #pragma unroll
-                for (int r = 0; r < NR; ++r) rot[r] = lane_next(rot[r], nxt);
+                for (int d = 0; d < DIM; ++d) lds_add(aj + (iN + d) * 64, ga1 * v[d]);
+                if (diag) {
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        lds_add(aj + (iY + d) * 64, g1[d]);
+                        own[iX + d] -= g1[d];
+                    }
+                } else {
+                    // j target, i source
+                    double2 cK2;
+                    double g2[DIM], ga2;
+                    kb += adj_sym_terms<PDE, DIM, KIND, MASK>(vn, ni, R, p, cj, si, cK2, g2, ga2);
+                    own[0] += cK2.x;
+                    own[1] += cK2.y;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        own[iN + d] = __builtin_fma(ga2, vn[d], own[iN + d]);
+                        if constexpr (COMB) {
+                            const double df = g1[d] - g2[d];
+                            lds_add(aj + (iY + d) * 64, df);
+                            own[iY + d] -= df;
+                        } else {
+                            lds_add(aj + (iY + d) * 64, g1[d]);
+                            lds_add(aj + (iX + d) * 64, -g2[d]);
+                            own[iY + d] += g2[d];
+                            own[iX + d] -= g1[d];
+                        }
+                    }
+                }
             }
         };
         if (A.distinct && !diag) pairs(std::false_type{});
         else pairs(std::true_type{});
-        // add the waves' sums for sub-tile u in wave order
-        for (int w = 0; w < kSymBS / 64; ++w) {
-            if (wave == w) {
+        __syncthreads();
+        // wave u adds the NW waves' sums of sub-tile u in wave order; its
+        // lane l owns tile row u * 64 + l (= tid), also the row of its own
+        // sums below
+        if (wave == u) {
+            const int64_t j = b * kSymBS + tid;
+            if (j < n) {
 #pragma unroll
-                for (int r = 0; r < NR; ++r)
-                    s_red[r][u * 64 + lane] = (w == 0 ? 0.0 : s_red[r][u * 64 + lane]) + rot[r];
+                for (int r = 0; r < NR; ++r) {
+                    double v = s_acc[0][r][lane];
+#pragma unroll
+                    for (int w = 1; w < NW; ++w) v += s_acc[w][r][lane];
+                    A.part[(a * NR + r) * n + j] = v;
+                }
             }
-            __syncthreads();
         }
     }
-    // source tile b's sums go to slot a; target tile a's to slot b (in a
-    // diagonal block both are the same rows: this thread adds its own)
-    const int64_t j = b * kSymBS + tid;
-    if (j < n) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) A.part[(a * NR + r) * n + j] = s_red[r][tid];
-    }
+    // source tile b's sums went to slot a; target tile a's go to slot b (in
+    // a diagonal block both are the same rows, and this thread wrote its
+    // row's source sums above)
     const int64_t i = a * kSymBS + tid;
     if (i < n) {
         const int64_t slot = diag ? a : b;
@@ -602,9 +648,10 @@ __global__ __launch_bounds__(kSymBS) void k_adj_sym(AdjSymArgs A) {
 }
 
 // per node: sum the S slots and form the outputs
-template <int DIM>
+template <int DIM, bool COMB>
 __global__ __launch_bounds__(256) void k_adj_sym_reduce(AdjOut o, int64_t S) {
-    constexpr int NR = 3 * DIM + 2;
+    constexpr int NR = COMB ? 2 * DIM + 2 : 3 * DIM + 2;
+    constexpr int iX = COMB ? 2 : 2 + DIM, iN = COMB ? 2 + DIM : 2 + 2 * DIM;
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= o.n) return;
     double R[NR];
@@ -628,8 +675,8 @@ __global__ __launch_bounds__(256) void k_adj_sym_reduce(AdjOut o, int64_t S) {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         if (o.ybar) o.ybar[d * o.n + j] = R[2 + d];
-        if (o.xbar) o.xbar[d * o.n + j] = R[2 + DIM + d];
-        if (o.nbar) o.nbar[d * o.n + j] = R[2 + 2 * DIM + d];
+        if (!COMB && o.xbar) o.xbar[d * o.n + j] = R[iX + d];
+        if (o.nbar) o.nbar[d * o.n + j] = R[iN + d];
     }
 }
 
@@ -696,7 +743,8 @@ using SymAdjFn = void (*)(const AdjSymArgs &, int64_t, hipStream_t);
 
 template <int PDE, int DIM, int KIND>
 void adj_sym_launch(const AdjSymArgs &a, int64_t nblocks, hipStream_t s) {
-    hipLaunchKernelGGL((k_adj_sym<PDE, DIM, KIND>), dim3((unsigned)nblocks), dim3(kSymBS), 0, s, a);
+    if (a.comb) hipLaunchKernelGGL((k_adj_sym<PDE, DIM, KIND, true>), dim3((unsigned)nblocks), dim3(kSymBS), 0, s, a);
+    else hipLaunchKernelGGL((k_adj_sym<PDE, DIM, KIND, false>), dim3((unsigned)nblocks), dim3(kSymBS), 0, s, a);
 }
 
 template <int PDE, int DIM>
@@ -907,6 +955,9 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
     const int dim = op->dim;
     const int cplx = op_is_complex(op) ? 1 : 0;
     const int64_t m = op->trg->n, n = op->src->n;
+    // the same output for the source and the target node gradients of an
+    // operator on one quadrature: their sum, the gradient of the nodes
+    const bool comb = ybar && ybar == xbar && op->trg == op->src;
     if (m == 0 || n == 0) {
         if (sigma_bar) std::fill(sigma_bar, sigma_bar + (cplx ? 2 : 1) * n, 0.0);
         if (ybar) std::fill(ybar, ybar + dim * n, 0.0);
@@ -950,7 +1001,7 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
     hipStream_t s = ctx->stream;
     // symmetric pullback: an operator on one quadrature, every output in one
     // pass over the unordered pairs
-    const int NRs = 3 * dim + 2;
+    const int NRs = comb ? 2 * dim + 2 : 3 * dim + 2;
     const int64_t symT = (n + kSymBS - 1) / kSymBS;
     SymAdjFn fsym = nullptr;
     const bool sym_fills = op->sym == 2 || symT * (symT + 1) / 2 >= 2LL * ctx->num_cu;
@@ -1003,14 +1054,17 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
         sa.part = (double *)dpart.p; sa.kpart = (double *)dkp.p;
         sa.ntiles = symT;
         sa.distinct = op->src->distinct ? 1 : 0;
+        sa.comb = comb ? 1 : 0;
         sa.p = p;
         if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev0, s));
         fsym(sa, symB, s);
         INTI_HIP(ctx, hipGetLastError());
         if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev1, s));
         const unsigned nb = (unsigned)((n + 255) / 256);
-        if (dim == 2) hipLaunchKernelGGL(k_adj_sym_reduce<2>, dim3(nb), dim3(256), 0, s, o, symT);
-        else hipLaunchKernelGGL(k_adj_sym_reduce<3>, dim3(nb), dim3(256), 0, s, o, symT);
+        if (dim == 2 && comb) hipLaunchKernelGGL((k_adj_sym_reduce<2, true>), dim3(nb), dim3(256), 0, s, o, symT);
+        else if (dim == 2) hipLaunchKernelGGL((k_adj_sym_reduce<2, false>), dim3(nb), dim3(256), 0, s, o, symT);
+        else if (comb) hipLaunchKernelGGL((k_adj_sym_reduce<3, true>), dim3(nb), dim3(256), 0, s, o, symT);
+        else hipLaunchKernelGGL((k_adj_sym_reduce<3, false>), dim3(nb), dim3(256), 0, s, o, symT);
         INTI_HIP(ctx, hipGetLastError());
     }
     // source-major pass
@@ -1048,9 +1102,14 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
     if (ybar) INTI_HIP(ctx, hipMemcpyAsync(ybar, dyb.p, sizeof(double) * dim * n, hipMemcpyDeviceToHost, s));
     if (nbar) INTI_HIP(ctx, hipMemcpyAsync(nbar, dnb.p, sizeof(double) * dim * n, hipMemcpyDeviceToHost, s));
     if (wbar) INTI_HIP(ctx, hipMemcpyAsync(wbar, dwb.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    if (xbar) INTI_HIP(ctx, hipMemcpyAsync(xbar, dxb.p, sizeof(double) * dim * m, hipMemcpyDeviceToHost, s));
+    // combined node gradient: the symmetric pass summed the two parts; the
+    // two-pass path adds the target part on the host
+    std::vector<double> xsep(comb && !fsym ? (size_t)dim * m : 0);
+    if (xbar && !(comb && fsym))
+        INTI_HIP(ctx, hipMemcpyAsync(comb ? xsep.data() : xbar, dxb.p, sizeof(double) * dim * m, hipMemcpyDeviceToHost, s));
     if (kbar) INTI_HIP(ctx, hipMemcpyAsync(kp.data(), dkp.p, sizeof(double) * nsb, hipMemcpyDeviceToHost, s));
     INTI_HIP(ctx, hipStreamSynchronize(s));
+    for (size_t q = 0; q < xsep.size(); ++q) ybar[q] += xsep[q];
     if (sigma_bar && m == n) adjoint_identity(op, cplx, n, ubar, sigma_bar);
     if (kbar) {
         double sum = 0.0;

@@ -82,6 +82,12 @@ void op_free_correction(Op *op) {
 int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *qtags,
                const int64_t *near_ptr, const int64_t *near_idx, const double *mu,
                const double *params, Csr **outS, Csr **outD, double *diag_out);
+int bdim_build_types(Op *opS, Op *opD, int deriv, int ntypes, const int *nq, const int64_t *ne,
+                     const int64_t *qtags, const int64_t *near_ptr, const int64_t *near_idx, const double *mu,
+                     const double *params, Csr **outS, Csr **outD, double *diag_out);
+int lsq_build_types(Ctx *ctx, int cplx, int nrows, int ntypes, const int *ncols, const int64_t *ne, const double *A,
+                    const int64_t *cols, int64_t m, int64_t n, const int64_t *near_ptr, const int64_t *near_idx,
+                    const double *rhs, Csr **out, double *diag_out);
 int lsq_build(Ctx *ctx, int cplx, int nrows, int ncols, int64_t ne, const double *A, const int64_t *cols,
               int64_t m, int64_t n, const int64_t *near_ptr, const int64_t *near_idx, const double *rhs,
               Csr **out, double *diag_out);
@@ -145,6 +151,13 @@ int inti_ctx_destroy(inti_ctx *ctx) {
 
 const char *inti_last_error(const inti_ctx *ctx) {
     return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+int inti_microbench(inti_ctx *ctx, int op, double *rate) {
+    if (!ctx) return INTI_ERR_ARG;
+    REQUIRE(ctx, rate != nullptr, "microbench: rate is null");
+    LOCK(ctx);
+    return microbench(ctx, op, rate);
 }
 
 int inti_ctx_synchronize(inti_ctx *ctx) {
@@ -687,6 +700,39 @@ int inti_bdim_build(inti_op *opS, inti_op *opD, int derivative, int nq, int64_t 
     return INTI_OK;
 }
 
+int inti_bdim_build_types(inti_op *opS, inti_op *opD, int derivative, int ntypes, const int *nq,
+                          const int64_t *ne, const int64_t *qtags, const int64_t *near_ptr,
+                          const int64_t *near_idx, const double *mu, const double *params, inti_csr **dS,
+                          inti_csr **dD, double *diag) {
+    if (!opS) return INTI_ERR_ARG;
+    Ctx *ctx = opS->ctx;
+    REQUIRE(ctx, opD && nq && ne && qtags && mu && dS && dD, "bdim: null argument");
+    REQUIRE(ctx, opD->ctx == ctx, "bdim: operators on different contexts");
+    REQUIRE(ctx, opS->trg == opD->trg && opS->src == opD->src, "bdim: S and D must share target and source");
+    REQUIRE(ctx, opS->pde == opD->pde && opS->dim == opD->dim && opS->k == opD->k,
+            "bdim: S and D must be the same PDE");
+    const int kS = derivative ? INTI_ADL : INTI_SL, kD = derivative ? INTI_HS : INTI_DL;
+    REQUIRE(ctx, opS->kind == kS && opD->kind == kD,
+            "bdim: expects (SL, DL) operators, or (ADL, HS) with derivative = 1");
+    REQUIRE(ctx, ntypes > 0, "bdim: no element type");
+    int64_t total = 0;
+    for (int t = 0; t < ntypes; ++t) {
+        REQUIRE(ctx, nq[t] > 0 && ne[t] >= 0, "bdim: bad element type sizes");
+        total += ne[t];
+    }
+    REQUIRE(ctx, total > 0, "bdim: empty element set");
+    REQUIRE(ctx, (near_ptr == nullptr) == (near_idx == nullptr), "bdim: near_ptr and near_idx go together");
+    REQUIRE(ctx, params == nullptr || (params[0] > 0 && params[1] > 0), "bdim: bad parameters");
+    LOCK(ctx);
+    Csr *s = nullptr, *d = nullptr;
+    int st = bdim_build_types(opS, opD, derivative, ntypes, nq, ne, qtags, near_ptr, near_idx, mu, params, &s,
+                              &d, diag);
+    if (st) return st;
+    *dS = static_cast<inti_csr *>(s);
+    *dD = static_cast<inti_csr *>(d);
+    return INTI_OK;
+}
+
 int inti_lsq_build(inti_ctx *ctx, int is_complex, int nrows, int ncols, int64_t ne, const double *A,
                    const int64_t *cols, int64_t m, int64_t n, const int64_t *near_ptr,
                    const int64_t *near_idx, const double *rhs, inti_csr **out, double *diag) {
@@ -697,6 +743,28 @@ int inti_lsq_build(inti_ctx *ctx, int is_complex, int nrows, int ncols, int64_t 
     LOCK(ctx);
     Csr *c = nullptr;
     int st = lsq_build(ctx, is_complex ? 1 : 0, nrows, ncols, ne, A, cols, m, n, near_ptr, near_idx, rhs, &c, diag);
+    if (st) return st;
+    *out = static_cast<inti_csr *>(c);
+    return INTI_OK;
+}
+
+int inti_lsq_build_types(inti_ctx *ctx, int is_complex, int nrows, int ntypes, const int *ncols, const int64_t *ne,
+                         const double *A, const int64_t *cols, int64_t m, int64_t n, const int64_t *near_ptr,
+                         const int64_t *near_idx, const double *rhs, inti_csr **out, double *diag) {
+    if (!ctx) return INTI_ERR_ARG;
+    REQUIRE(ctx, ncols && ne && A && cols && rhs && out, "lsq: null argument");
+    REQUIRE(ctx, ntypes > 0 && m > 0 && n > 0, "lsq: empty problem");
+    int64_t total = 0;
+    for (int t = 0; t < ntypes; ++t) {
+        REQUIRE(ctx, ne[t] >= 0, "lsq: negative element count");
+        total += ne[t];
+    }
+    REQUIRE(ctx, total > 0, "lsq: empty problem");
+    REQUIRE(ctx, (near_ptr == nullptr) == (near_idx == nullptr), "lsq: near_ptr and near_idx go together");
+    LOCK(ctx);
+    Csr *c = nullptr;
+    int st = lsq_build_types(ctx, is_complex ? 1 : 0, nrows, ntypes, ncols, ne, A, cols, m, n, near_ptr, near_idx,
+                             rhs, &c, diag);
     if (st) return st;
     *out = static_cast<inti_csr *>(c);
     return INTI_OK;

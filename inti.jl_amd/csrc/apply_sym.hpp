@@ -90,6 +90,9 @@ __device__ __forceinline__ double lane_next(double v, int addr) {
     const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(v));
     return __hiloint2double(hi, lo);
 }
+__device__ __forceinline__ float lane_next(float v, int addr) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
 
 // LDS record of a source node: its coordinates, its normal when the kernel
 // reads one (every kind but SL), and q (1 or 2 doubles), packed in double2
@@ -142,37 +145,36 @@ struct SymRec {
     }
 };
 
-// LDS record of a source node for the fused 3D Helmholtz DL / DL + ch*SL
-// pairs (helm3_sym_fused): coordinates, normal and the staged density terms
-// q, ch q, k Im q, -k Re q, in six double2 slots. CHIK: ch = -i k, so ch q
-// equals the (k Im q, -k Re q) slot; the record has five slots (20 KB of LDS
-// per tile instead of 24: one more block per CU).
+// LDS record of a source node for the fused mixed 3D Helmholtz DL / DL +
+// ch*SL pairs (helm3_sym_fused32): coordinates and normal in FP64 (three
+// double2 slots) and the staged density terms in FP32: (q, k Im q, -k Re q)
+// in one 16-byte slot and ch q in another. CHIK: ch = -i k, so ch q equals
+// (k Im q, -k Re q) and the record has four slots (16 KB of LDS per tile).
 template <bool CHIK>
 struct SymRecF {
-    static constexpr int NS = CHIK ? 5 : 6;
-    static constexpr int SK = CHIK ? 4 : 5;  // slot of (k Im q, -k Re q)
+    static constexpr int NS = CHIK ? 4 : 5;
     __device__ static __forceinline__ void put(double2 (*s)[kSymBS], int j, const double *y, const double *nv,
-                                               const QExt &q) {
+                                               const QExt32 &q) {
         s[0][j] = make_double2(y[0], y[1]);
         s[1][j] = make_double2(y[2], nv[0]);
         s[2][j] = make_double2(nv[1], nv[2]);
-        s[3][j] = make_double2(q.qr, q.qi);
-        if constexpr (!CHIK) s[4][j] = make_double2(q.cr, q.ci);
-        s[SK][j] = make_double2(q.kqi, q.mkqr);
+        *reinterpret_cast<float4 *>(&s[3][j]) = make_float4(q.qr, q.qi, q.kqi, q.mkqr);
+        if constexpr (!CHIK) *reinterpret_cast<float4 *>(&s[4][j]) = make_float4(q.cr, q.ci, 0.0f, 0.0f);
     }
     __device__ static __forceinline__ void get(const double2 (*s)[kSymBS], int j, double *y, double *nv,
-                                               QExt &q) {
-        const double2 v0 = s[0][j], v1 = s[1][j], v2 = s[2][j], v3 = s[3][j], v5 = s[SK][j];
+                                               QExt32 &q) {
+        const double2 v0 = s[0][j], v1 = s[1][j], v2 = s[2][j];
+        const float4 f = *reinterpret_cast<const float4 *>(&s[3][j]);
         y[0] = v0.x; y[1] = v0.y; y[2] = v1.x;
         nv[0] = v1.y; nv[1] = v2.x; nv[2] = v2.y;
-        q.qr = v3.x; q.qi = v3.y; q.kqi = v5.x; q.mkqr = v5.y;
+        q.qr = f.x; q.qi = f.y; q.kqi = f.z; q.mkqr = f.w;
         if constexpr (CHIK) {
-            q.cr = v5.x;
-            q.ci = v5.y;
+            q.cr = f.z;
+            q.ci = f.w;
         } else {
-            const double2 v4 = s[4][j];
-            q.cr = v4.x;
-            q.ci = v4.y;
+            const float4 g = *reinterpret_cast<const float4 *>(&s[4][j]);
+            q.cr = g.x;
+            q.ci = g.y;
         }
     }
 };
@@ -184,9 +186,11 @@ struct SymRecF {
 // transposed, for A^H ubar): the same pairs with the two values exchanged.
 template <int PDE, int DIM, int KIND, bool MIXED, int G, int TPL, bool TRANS = false, bool CHIK = false>
 __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
-    // mixed 3D Helmholtz DL and DL + ch*SL: value and accumulation fused
-    // (2.8 % faster measured; in FP64 the larger LDS record cost more than
-    // the 5 % fewer VALU instructions saved, so FP64 keeps the plain pair)
+    // mixed 3D Helmholtz DL and DL + ch*SL: value and accumulation fused,
+    // the products and the 64-term running sums in FP32 (added into FP64
+    // after every 32-step pass); in FP64 the fused form's larger LDS record
+    // cost more than the 5 % fewer VALU instructions saved, so FP64 keeps
+    // the plain pair
     constexpr bool FUSED =
         MIXED && PDE == INTI_HELMHOLTZ && DIM == 3 && (KIND == INTI_DL || KIND == INTI_COMB_SD);
     using R = std::conditional_t<FUSED, SymRecF<CHIK>, SymRec<PDE, DIM, KIND>>;
@@ -201,6 +205,7 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
     __shared__ double2 s_red[RED_ADD ? 1 : NW][kSymBS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nxt = ((lane + 1) & 63) << 2;  // ds_bpermute address of lane l+1
+    const int far = ((lane + 32) & 63) << 2;  // and of lane l+32
     int64_t a, b;
     sym_item(blockIdx.x, A.ngroups, a, b);
     const bool diag = a == b;
@@ -219,7 +224,7 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
         for (int t = 0; t < TPL; ++t) {
             double y[3], nv[3], qr, qi;
             sym_node<DIM, CPLX>(A, J * kSymBS + tid + t * BS, y, nv, qr, qi);
-            if constexpr (FUSED) R::put(s_src, tid + t * BS, y, nv, qext(qr, qi, p));
+            if constexpr (FUSED) R::put(s_src, tid + t * BS, y, nv, qext32(qr, qi, p));
             else R::put(s_src, tid + t * BS, y, nv, qr, qi);
             if constexpr (RED_ADD) s_red[0][tid + t * BS] = make_double2(0.0, 0.0);
         }
@@ -266,7 +271,7 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                         const int jl = u * 64 + ((lane + s) & 63);
                         double yj[3], nj[3], qjr, qji;
                         if constexpr (FUSED) {
-                            QExt qx;
+                            QExt32 qx;
                             R::get(s_src, jl, yj, nj, qx);
                             qjr = qx.qr;
                             qji = qx.qi;
@@ -294,31 +299,51 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                     auto pairs = [&](auto mask_tag) {
                         constexpr bool MASK = decltype(mask_tag)::value;
                         if constexpr (FUSED) {
-                            QExt qit[TPL];
+                            // FP32 sums of this pass: s1 for the lane's
+                            // targets, the chains c for the rotating sources
+                            QExt32 qit[TPL];
+                            float s1r[TPL], s1i[TPL];
 #pragma unroll
-                            for (int t = 0; t < TPL; ++t) qit[t] = qext(qir[t], qii[t], p);
-                            auto one = [&](int jl, double &ajr, double &aji) {
+                            for (int t = 0; t < TPL; ++t) {
+                                qit[t] = qext32(qir[t], qii[t], p);
+                                s1r[t] = s1i[t] = 0.0f;
+                            }
+                            float car = 0.0f, cai = 0.0f, cbr = 0.0f, cbi = 0.0f;
+                            auto one = [&](int jl, float &ajr, float &aji) {
                                 double yj[3], nj[3];
-                                QExt qj;
+                                QExt32 qj;
                                 R::get(s_src, jl, yj, nj, qj);
 #pragma unroll
                                 for (int t = 0; t < TPL; ++t) {
                                     double r[3];
 #pragma unroll
                                     for (int d = 0; d < 3; ++d) r[d] = xi[t][d] - yj[d];
-                                    helm3_sym_fused<MIXED, MASK, TRANS>(r, ni[t], nj, p, qj, qit[t], adr[gi][t],
-                                                                        adi[gi][t], ajr, aji);
+                                    helm3_sym_fused32<MASK, TRANS>(r, ni[t], nj, p, qj, qit[t], s1r[t], s1i[t],
+                                                                   ajr, aji);
                                 }
                             };
                             for (int s = 0; s < 32; ++s) {
                                 const int jl = u * 64 + ((lane + s) & 63);
-                                one(jl, aar, aai);
-                                one(jl ^ 32, abr, abi);
-                                aar = lane_next(aar, nxt);
-                                aai = lane_next(aai, nxt);
-                                abr = lane_next(abr, nxt);
-                                abi = lane_next(abi, nxt);
+                                one(jl, car, cai);
+                                one(jl ^ 32, cbr, cbi);
+                                car = lane_next(car, nxt);
+                                cai = lane_next(cai, nxt);
+                                cbr = lane_next(cbr, nxt);
+                                cbi = lane_next(cbi, nxt);
                             }
+#pragma unroll
+                            for (int t = 0; t < TPL; ++t) {
+                                adr[gi][t] += (double)s1r[t];
+                                adi[gi][t] += (double)s1i[t];
+                            }
+                            // the FP64 chain sums do not move: lane l keeps
+                            // A's sum of source l and B's of source l + 32.
+                            // After the pass the FP32 chains hold A's part of
+                            // source l + 32 and B's of source l in lane l.
+                            aar += (double)lane_next(car, far);
+                            aai += (double)lane_next(cai, far);
+                            abr += (double)lane_next(cbr, far);
+                            abi += (double)lane_next(cbi, far);
                         } else {
                             auto one = [&](int jl, double &ajr, double &aji) {
                                 double yj[3], nj[3], qjr, qji;
@@ -363,13 +388,14 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                     };
                     if (A.distinct) pairs(std::false_type{});
                     else pairs(std::true_type{});
-                    // exchange the chains' roles for the next pass
-                    double t0 = aar; aar = abr; abr = t0;
-                    t0 = aai; aai = abi; abi = t0;
+                    if constexpr (!FUSED) {
+                        // exchange the chains' roles for the next pass
+                        double t0 = aar; aar = abr; abr = t0;
+                        t0 = aai; aai = abi; abi = t0;
+                    }
                 }
             }
             // source l: A's part in this lane, B's in lane l + 32
-            const int far = ((lane + 32) & 63) << 2;
             double ajr = aar + lane_next(abr, far);
             double aji = CPLX ? aai + lane_next(abi, far) : 0.0;
             if (trans) {

@@ -19,8 +19,6 @@
 
 namespace inti {
 
-constexpr int kThetaTT = 32;   // sources per LDS tile of the Theta kernel
-constexpr int kThetaBS = 256;  // targets per block
 
 // ----------------------------------------------------------------- traces
 struct TraceArgs {
@@ -54,28 +52,115 @@ __global__ __launch_bounds__(256) void k_traces(TraceArgs a) {
 }
 
 // ------------------------------------------------------------------ Theta
+// Theta[l, k] = mu_i G(x_i, z_k) + sum_j S(x_i, y_j) B1[j, k] - D(x_i, y_j) B0[j, k]
+// for the corrected targets i = glob[l]: an ns-right-hand-side direct sum
+// with two kernels per pair. Every kernel value feeds 2 ns complex
+// multiply-adds (8 ns FP64 FMAs), so the sum is FMA-bound on the VALU; the
+// right-hand sides are the same for all targets of a wavefront, so they are
+// read as wave-uniform scalar loads (SGPR operands of the FMAs: no LDS, no
+// VGPRs for them) from a packed per-source record:
+//   rec[j] = { y_j (DIM), n_j (DIM), then per monopole k:
+//              post_S w_j B1[j,k], -post_D w_j B0[j,k] (complex: 4 doubles,
+//              real for Laplace: 2) }
+// so that the pair loop accumulates raw_S * rec + raw_D * rec only.
+// A block is 64 targets (one per lane) x 4 wavefronts, each wavefront a
+// quarter of the block's sources; the quarters are added in LDS in wave
+// order (deterministic). With one source chunk (split = 1) the block writes
+// Theta itself, mu G included; otherwise partial sums go to `part` and
+// k_theta_reduce adds them.
+constexpr int kThetaWaves = 4;
+constexpr int kThetaRed = 8;  // doubles per lane per LDS reduction round
+
+typedef const __attribute__((address_space(4))) double uniform_double;
+
+template <int PDE>
+constexpr int theta_per_k() { return PDE == INTI_HELMHOLTZ ? 4 : 2; }
+
+struct ThetaPackArgs {
+    int64_t n;
+    int ns, nsp, nst, rl;  // monopoles, trace columns, record columns, record length
+    const double *sx, *sn, *sw;
+    const double2 *B0, *B1;  // n x nsp, column-major
+    double *rec;             // n x rl
+    double psr, psi, pdr, pdi;  // post factors of the S and D kernels
+};
+
+template <int PDE, int DIM>
+__global__ __launch_bounds__(256) void k_theta_pack(ThetaPackArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= a.n * (a.nst + 1)) return;
+    const int64_t j = e % a.n;
+    const int kk = (int)(e / a.n);
+    double *R = a.rec + j * a.rl;
+    if (kk == a.nst) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            R[d] = a.sx[d * a.n + j];
+            R[DIM + d] = a.sn[d * a.n + j];
+        }
+        return;
+    }
+    double2 b1 = make_double2(0.0, 0.0), b0 = b1;
+    if (kk < a.ns) {
+        b1 = a.B1[j + a.n * kk];
+        b0 = a.B0[j + a.n * kk];
+    }
+    const double w = a.sw[j];
+    double *q = R + 2 * DIM + theta_per_k<PDE>() * kk;
+    if constexpr (PDE == INTI_LAPLACE) {
+        q[0] = a.psr * (w * b1.x);
+        q[1] = -a.pdr * (w * b0.x);
+    } else {
+        const double s1r = w * b1.x, s1i = w * b1.y, s0r = w * b0.x, s0i = w * b0.y;
+        q[0] = a.psr * s1r - a.psi * s1i;
+        q[1] = a.psr * s1i + a.psi * s1r;
+        q[2] = -(a.pdr * s0r - a.pdi * s0i);
+        q[3] = -(a.pdr * s0i + a.pdi * s0r);
+    }
+}
+
 struct ThetaArgs {
     int64_t m, n, nloc;
-    int nsp, split;
-    int64_t chunk;
+    int ns, rl, split;
+    int64_t chunk;  // sources per block
     const int32_t *glob;  // local row -> global target
-    const double *tx, *tn, *sx, *sn, *sw;
-    const double2 *B0, *B1;
-    double2 *part;  // split x nloc x nsp
+    const double *tx, *tn;
+    const double *rec;
+    const double *zs, *mu;  // monopoles (DIM x ns), green multiplier per target
+    double2 *part;          // split x NS x nloc partial sums (split > 1)
+    double2 *theta;         // nloc x ns, column-major (split == 1)
     KParams p;
 };
 
+// raw values of the S and D kernels of one pair, sharing the radial parts
+template <int PDE, int DIM, int KS, int KD>
+__device__ __forceinline__ void theta_pair(const double *r, const double *nx, const double *ny,
+                                           const KParams &p, double &sr, double &si, double &dr,
+                                           double &di) {
+    if constexpr (PDE == INTI_HELMHOLTZ && DIM == 2) {
+        const double d2 = dotv<2>(r, r);
+        const bool ok = d2 > kSamePointTol2;
+        const Helm2Radial R = helm2_radial<false>(fmax(d2, kClampD2), p);
+        helm2_combine<KS, false>(r, ok, nx, ny, p, R, sr, si);
+        helm2_combine<KD, false>(r, ok, nx, ny, p, R, dr, di);
+    } else {
+        pair_raw<PDE, DIM, KS, false>(r, nx, ny, p, sr, si);
+        pair_raw<PDE, DIM, KD, false>(r, nx, ny, p, dr, di);
+    }
+}
+
 template <int PDE, int DIM, int DERIV, int NS>
-__global__ __launch_bounds__(kThetaBS) void k_theta(ThetaArgs a) {
+__global__ __launch_bounds__(64 * kThetaWaves) void k_theta(ThetaArgs a) {
     constexpr int KS = DERIV ? INTI_ADL : INTI_SL;
     constexpr int KD = DERIV ? INTI_HS : INTI_DL;
-    __shared__ double s_geo[kThetaTT][2 * DIM + 1];
-    __shared__ double2 s_b0[kThetaTT][NS];
-    __shared__ double2 s_b1[kThetaTT][NS];
-    const int tid = threadIdx.x;
-    const int64_t b = blockIdx.x;
-    const int64_t chunk = b % a.split, tile = b / a.split;
-    const int64_t l = tile * kThetaBS + tid;
+    constexpr int PK = theta_per_k<PDE>();
+    constexpr int NA = PDE == INTI_HELMHOLTZ ? 2 * NS : NS;  // accumulators per lane
+    constexpr int NR = (NA + kThetaRed - 1) / kThetaRed;
+    __shared__ double s_red[kThetaWaves - 1][kThetaRed][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t chunk = blockIdx.x % a.split, tile = blockIdx.x / a.split;
+    const int64_t l = tile * 64 + lane;
     const int64_t lc = l < a.nloc ? l : a.nloc - 1;
     const int64_t i = a.glob[lc];
     double xt[DIM], nt[DIM];
@@ -84,79 +169,107 @@ __global__ __launch_bounds__(kThetaBS) void k_theta(ThetaArgs a) {
         xt[d] = a.tx[d * a.m + i];
         nt[d] = a.tn ? a.tn[d * a.m + i] : 0.0;
     }
-    double accr[NS], acci[NS];
+    double acc[NA];
 #pragma unroll
-    for (int k = 0; k < NS; ++k) { accr[k] = 0.0; acci[k] = 0.0; }
-    const int64_t j0 = chunk * a.chunk;
-    const int64_t j1 = j0 + a.chunk < a.n ? j0 + a.chunk : a.n;
-    for (int64_t s0 = j0; s0 < j1; s0 += kThetaTT) {
-        for (int q = tid; q < kThetaTT * (2 * DIM + 1); q += kThetaBS) {
-            const int jj = q / (2 * DIM + 1), f = q % (2 * DIM + 1);
-            const int64_t j = s0 + jj;
-            double v = 0.0;
-            if (j < j1) {
-                if (f < DIM) v = a.sx[f * a.n + j];
-                else if (f < 2 * DIM) v = a.sn[(f - DIM) * a.n + j];
-                else v = a.sw[j];
-            } else if (f < DIM) {
-                v = 1.0e3;  // padding source far away, weight 0
-            }
-            s_geo[jj][f] = v;
+    for (int q = 0; q < NA; ++q) acc[q] = 0.0;
+    // this wavefront's sources: a quarter of the block's chunk
+    const int64_t b0 = chunk * a.chunk < a.n ? chunk * a.chunk : a.n;
+    const int64_t b1 = b0 + a.chunk < a.n ? b0 + a.chunk : a.n;
+    const int64_t quarter = (b1 - b0 + kThetaWaves - 1) / kThetaWaves;
+    const int64_t j0 = b0 + wave * quarter;
+    const int64_t j1 = j0 + quarter < b1 ? j0 + quarter : b1;
+    const uniform_double *rec = (const uniform_double *)a.rec;
+    for (int64_t j = j0; j < j1; ++j) {
+        const uniform_double *R = rec + j * a.rl;
+        double r[DIM], ny[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            r[d] = xt[d] - R[d];
+            ny[d] = R[DIM + d];
         }
-        for (int q = tid; q < kThetaTT * NS; q += kThetaBS) {
-            const int jj = q % kThetaTT, k = q / kThetaTT;
-            const int64_t j = s0 + jj;
-            double2 v0 = make_double2(0.0, 0.0), v1 = v0;
-            if (j < j1) {
-                v0 = a.B0[j + a.n * k];
-                v1 = a.B1[j + a.n * k];
+        double sr, si, dr, di;
+        theta_pair<PDE, DIM, KS, KD>(r, nt, ny, a.p, sr, si, dr, di);
+        const uniform_double *Q = R + 2 * DIM;
+        if constexpr (PDE == INTI_LAPLACE) {
+#pragma unroll
+            for (int k = 0; k < NS; ++k)
+                acc[k] = __builtin_fma(sr, Q[PK * k], __builtin_fma(dr, Q[PK * k + 1], acc[k]));
+        } else {
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                const double q1r = Q[PK * k], q1i = Q[PK * k + 1], q0r = Q[PK * k + 2], q0i = Q[PK * k + 3];
+                double ar = acc[2 * k], ai = acc[2 * k + 1];
+                ar = __builtin_fma(sr, q1r, ar);
+                ai = __builtin_fma(sr, q1i, ai);
+                ar = __builtin_fma(-si, q1i, ar);
+                ai = __builtin_fma(si, q1r, ai);
+                ar = __builtin_fma(dr, q0r, ar);
+                ai = __builtin_fma(dr, q0i, ai);
+                ar = __builtin_fma(-di, q0i, ar);
+                ai = __builtin_fma(di, q0r, ai);
+                acc[2 * k] = ar;
+                acc[2 * k + 1] = ai;
             }
-            s_b0[jj][k] = v0;
-            s_b1[jj][k] = v1;
+        }
+    }
+    // add the quarters in wave order, kThetaRed accumulators per round
+#pragma unroll
+    for (int rd = 0; rd < NR; ++rd) {
+        if (wave > 0) {
+#pragma unroll
+            for (int q = 0; q < kThetaRed; ++q)
+                if (rd * kThetaRed + q < NA) s_red[wave - 1][q][lane] = acc[rd * kThetaRed + q];
         }
         __syncthreads();
-        for (int jj = 0; jj < kThetaTT; ++jj) {
-            double r[DIM], ny[DIM];
+        if (wave == 0) {
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                r[d] = xt[d] - s_geo[jj][d];
-                ny[d] = s_geo[jj][DIM + d];
-            }
-            const double w = s_geo[jj][2 * DIM];
-            double sr, si, dr, di;
-            pair_value<PDE, DIM, KS>(r, nt, ny, a.p, sr, si);
-            pair_value<PDE, DIM, KD>(r, nt, ny, a.p, dr, di);
-            sr *= w; si *= w; dr *= w; di *= w;
-            if constexpr (PDE == INTI_LAPLACE) {
-                // real kernels and traces: two FMAs per monopole
+            for (int q = 0; q < kThetaRed; ++q)
+                if (rd * kThetaRed + q < NA) {
+                    double v = acc[rd * kThetaRed + q];
 #pragma unroll
-                for (int k = 0; k < NS; ++k)
-                    accr[k] = __builtin_fma(sr, s_b1[jj][k].x, __builtin_fma(-dr, s_b0[jj][k].x, accr[k]));
-            } else {
-#pragma unroll
-                for (int k = 0; k < NS; ++k) {
-                    const double2 b1 = s_b1[jj][k], b0 = s_b0[jj][k];
-                    accr[k] += sr * b1.x - si * b1.y - (dr * b0.x - di * b0.y);
-                    acci[k] += sr * b1.y + si * b1.x - (dr * b0.y + di * b0.x);
+                    for (int w = 0; w < kThetaWaves - 1; ++w) v += s_red[w][q][lane];
+                    acc[rd * kThetaRed + q] = v;
                 }
-            }
         }
         __syncthreads();
     }
-    if (l < a.nloc) {
-        double2 *pp = a.part + (chunk * a.nloc + l) * a.nsp;
+    if (wave != 0 || l >= a.nloc) return;
+    if (a.split == 1) {
+        const double mu = a.mu[i];
+        for (int k = 0; k < a.ns; ++k) {
+            double r[DIM], dummy[DIM];
 #pragma unroll
-        for (int k = 0; k < NS; ++k) pp[k] = make_double2(accr[k], acci[k]);
+            for (int d = 0; d < DIM; ++d) {
+                r[d] = xt[d] - a.zs[d * a.ns + k];
+                dummy[d] = 0.0;
+            }
+            double gr, gi;
+            pair_value<PDE, DIM, DERIV ? INTI_ADL : INTI_SL>(r, nt, dummy, a.p, gr, gi);
+            double vr = 0.0, vi = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < NS; ++kk)
+                if (kk == k) {
+                    vr = PDE == INTI_HELMHOLTZ ? acc[2 * kk] : acc[kk];
+                    vi = PDE == INTI_HELMHOLTZ ? acc[2 * kk + 1] : 0.0;
+                }
+            a.theta[l + a.nloc * k] = make_double2(mu * gr + vr, mu * gi + vi);
+        }
+    } else {
+        double2 *pp = a.part + chunk * NS * a.nloc + l;
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            pp[k * a.nloc] = make_double2(PDE == INTI_HELMHOLTZ ? acc[2 * k] : acc[k],
+                                          PDE == INTI_HELMHOLTZ ? acc[2 * k + 1] : 0.0);
     }
 }
 
 struct ThetaRedArgs {
     int64_t m, nloc;
-    int ns, nsp, split;
+    int ns, nst, split;
     const int32_t *glob;
     const double *tx, *tn, *zs, *mu;
-    const double2 *part;
-    double2 *theta;  // nloc x ns, column-major
+    const double2 *part;  // split x nst x nloc
+    double2 *theta;       // nloc x ns, column-major
     KParams p;
 };
 
@@ -177,13 +290,12 @@ __global__ __launch_bounds__(256) void k_theta_reduce(ThetaRedArgs a) {
     double gr, gi;
     pair_value<PDE, DIM, DERIV ? INTI_ADL : INTI_SL>(r, nx, dummy, a.p, gr, gi);
     const double mu = a.mu[i];
-    double tr = mu * gr, ti = mu * gi;
     double sr = 0.0, si = 0.0;
     for (int c = 0; c < a.split; ++c) {
-        const double2 v = a.part[((int64_t)c * a.nloc + l) * a.nsp + kk];
+        const double2 v = a.part[((int64_t)c * a.nst + kk) * a.nloc + l];
         sr += v.x; si += v.y;
     }
-    a.theta[l + a.nloc * kk] = make_double2(tr + sr, ti + si);
+    a.theta[l + a.nloc * kk] = make_double2(mu * gr + sr, mu * gi + si);
 }
 
 // ------------------------------------------------------- batched QR / LS
@@ -258,46 +370,63 @@ __global__ __launch_bounds__(64) void k_ls(LsArgs a) {
 }
 
 // --------------------------------------------------------------- dispatch
+// Theta kernels are instantiated for these monopole counts (multiples of 3:
+// ns = ceil(3 nq) with the default oversampling); a build takes the smallest
+// one >= ns, the extra record columns being zero.
+constexpr int kThetaNs[] = {6, 9, 12, 15, 18, 21, 24, 27, 30, 32};
+
 using ThetaFn = void (*)(const ThetaArgs &, dim3, hipStream_t);
 template <int PDE, int DIM, int DERIV, int NS>
 static void theta_launcher(const ThetaArgs &a, dim3 g, hipStream_t s) {
-    hipLaunchKernelGGL((k_theta<PDE, DIM, DERIV, NS>), g, dim3(kThetaBS), 0, s, a);
+    hipLaunchKernelGGL((k_theta<PDE, DIM, DERIV, NS>), g, dim3(64 * kThetaWaves), 0, s, a);
+}
+template <int PDE, int DIM, int DERIV, int NS>
+static int theta_occupancy() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_theta<PDE, DIM, DERIV, NS>, 64 * kThetaWaves, 0) !=
+        hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
 }
 template <int PDE, int DIM, int DERIV>
-static ThetaFn theta_ns(int nsp) {
-    switch (nsp) {
-        case 8: return theta_launcher<PDE, DIM, DERIV, 8>;
-        case 12: return theta_launcher<PDE, DIM, DERIV, 12>;
-        case 16: return theta_launcher<PDE, DIM, DERIV, 16>;
-        case 20: return theta_launcher<PDE, DIM, DERIV, 20>;
-        case 24: return theta_launcher<PDE, DIM, DERIV, 24>;
-        case 32: return theta_launcher<PDE, DIM, DERIV, 32>;
+static ThetaFn theta_ns(int nst, int *occ) {
+#define INTI_T(N) \
+    case N: *occ = theta_occupancy<PDE, DIM, DERIV, N>(); return theta_launcher<PDE, DIM, DERIV, N>;
+    switch (nst) {
+        INTI_T(6) INTI_T(9) INTI_T(12) INTI_T(15) INTI_T(18) INTI_T(21) INTI_T(24) INTI_T(27) INTI_T(30)
+        INTI_T(32)
     }
+#undef INTI_T
     return nullptr;
 }
 
 struct BdimFns {
     void (*traces)(const TraceArgs &, dim3, hipStream_t);
+    void (*pack)(const ThetaPackArgs &, dim3, hipStream_t);
     ThetaFn theta;
+    int theta_occ;  // resident Theta blocks per CU
     void (*reduce)(const ThetaRedArgs &, dim3, hipStream_t);
 };
 
 template <int PDE, int DIM, int DERIV>
-static BdimFns bdim_fns(int nsp) {
+static BdimFns bdim_fns(int nst) {
     BdimFns f;
     f.traces = [](const TraceArgs &a, dim3 g, hipStream_t s) {
         hipLaunchKernelGGL((k_traces<PDE, DIM>), g, dim3(256), 0, s, a);
     };
-    f.theta = theta_ns<PDE, DIM, DERIV>(nsp);
+    f.pack = [](const ThetaPackArgs &a, dim3 g, hipStream_t s) {
+        hipLaunchKernelGGL((k_theta_pack<PDE, DIM>), g, dim3(256), 0, s, a);
+    };
+    f.theta = theta_ns<PDE, DIM, DERIV>(nst, &f.theta_occ);
     f.reduce = [](const ThetaRedArgs &a, dim3 g, hipStream_t s) {
         hipLaunchKernelGGL((k_theta_reduce<PDE, DIM, DERIV>), g, dim3(256), 0, s, a);
     };
     return f;
 }
 
-static bool pick_bdim(int pde, int dim, int deriv, int nsp, BdimFns &f) {
+static bool pick_bdim(int pde, int dim, int deriv, int nst, BdimFns &f) {
 #define INTI_B(P, D, R) \
-    if (pde == P && dim == D && deriv == R) { f = bdim_fns<P, D, R>(nsp); return f.theta != nullptr; }
+    if (pde == P && dim == D && deriv == R) { f = bdim_fns<P, D, R>(nst); return f.theta != nullptr; }
     INTI_B(INTI_LAPLACE, 2, 0) INTI_B(INTI_LAPLACE, 2, 1)
     INTI_B(INTI_LAPLACE, 3, 0) INTI_B(INTI_LAPLACE, 3, 1)
     INTI_B(INTI_HELMHOLTZ, 2, 0) INTI_B(INTI_HELMHOLTZ, 2, 1)
@@ -358,9 +487,16 @@ struct DevBuf {
 int csr_alloc(Ctx *ctx, int64_t m, int64_t n, int64_t nnz, int cplx, Csr **out);
 void csr_free(Csr *c);
 
-int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *qtags,
-               const int64_t *near_ptr, const int64_t *near_idx, const double *mu,
-               const double *params, Csr **outS, Csr **outD, double *diag_out) {
+// Element types: type t has nq[t] nodes per element and ne[t] elements; its
+// node tags are the nq[t] x ne[t] column-major block of qtags at
+// sum_{u<t} nq[u] ne[u], and (target != source) its near lists are the
+// ne[t] + 1 entries of near_ptr at sum_{u<t} (ne[u] + 1) (each starting at 0)
+// indexing the block of near_idx at sum_{u<t} near_ptr_u[ne[u]]. The monopole
+// count uses qmax = sum_t nq[t] (bdim.jl:79-80); the least squares run type
+// by type and the triplets are taken in that order (bdim.jl:138-185).
+int bdim_build_types(Op *opS, Op *opD, int deriv, int ntypes, const int *nq, const int64_t *ne,
+                     const int64_t *qtags, const int64_t *near_ptr, const int64_t *near_idx, const double *mu,
+                     const double *params, Csr **outS, Csr **outD, double *diag_out) {
     Ctx *ctx = opS->ctx;
     const Quad *trg = opS->trg, *src = opS->src;
     const int dim = src->dim;
@@ -368,36 +504,68 @@ int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *q
     const bool self = near_ptr == nullptr;
     const double oversample = params ? params[0] : 3.0;
     const double radius_mult = params ? params[1] : 1.5;
-    const int ns = (int)std::ceil(oversample * nq);
-    const int nc = 2 * nq;
-    if (ns > kMaxNs || nc > kMaxNc || ns < nc)
+    if (ntypes < 1) return set_error(ctx, INTI_ERR_ARG, "bdim: no element type");
+    int qmax = 0, nqmax = 0;
+    int64_t ntags = 0;
+    for (int t = 0; t < ntypes; ++t) {
+        if (nq[t] < 1 || ne[t] < 0) return set_error(ctx, INTI_ERR_ARG, "bdim: bad element type sizes");
+        qmax += nq[t];
+        nqmax = std::max(nqmax, nq[t]);
+        ntags += (int64_t)nq[t] * ne[t];
+    }
+    const int ns = (int)std::ceil(oversample * qmax);
+    if (ns > kMaxNs || 2 * nqmax > kMaxNc || ns < 2 * nqmax)
         return set_error(ctx, INTI_ERR_UNSUPPORTED,
-                         "bdim: need 2*nq <= ns <= 32 and 2*nq <= 24 (ns = ceil(oversample*nq))");
+                         "bdim: need 2*nq <= ns <= 32 and 2*nq <= 24 (ns = ceil(oversample * sum of nq over types))");
     if (self && (trg != src))
         return set_error(ctx, INTI_ERR_ARG, "bdim: the self pattern needs target == source");
     if (!src->normals) return set_error(ctx, INTI_ERR_ARG, "bdim: source quadrature needs normals");
     if (deriv && !trg->normals)
         return set_error(ctx, INTI_ERR_ARG, "bdim: derivative correction needs target normals");
-    if (n == 0 || ne == 0) return set_error(ctx, INTI_ERR_ARG, "bdim: empty source");
-    if ((int64_t)nq * ne != n && self)
-        return set_error(ctx, INTI_ERR_SIZE, "bdim: nq*ne must equal the number of sources");
-    for (int64_t q = 0; q < (int64_t)nq * ne; ++q)
+    if (n == 0 || ntags == 0) return set_error(ctx, INTI_ERR_ARG, "bdim: empty source");
+    if (ntags != n && self)
+        return set_error(ctx, INTI_ERR_SIZE, "bdim: the element types must cover every source node once");
+    for (int64_t q = 0; q < ntags; ++q)
         if (qtags[q] < 0 || qtags[q] >= n) return set_error(ctx, INTI_ERR_ARG, "bdim: qtags out of range");
     const int nsp = ns <= 8 ? 8 : ns <= 12 ? 12 : ns <= 16 ? 16 : ns <= 20 ? 20 : ns <= 24 ? 24 : 32;
+    int nst = 32;
+    for (int c : kThetaNs)
+        if (c >= ns) { nst = c; break; }
     BdimFns fns;
-    if (!pick_bdim(opS->pde, dim, deriv, nsp, fns))
+    if (!pick_bdim(opS->pde, dim, deriv, nst, fns))
         return set_error(ctx, INTI_ERR_UNSUPPORTED, "bdim: no kernel for this operator");
 
-    // ---- host-side lists (element -> targets) and the CSR pattern
-    std::vector<int32_t> pair_ptr(ne + 1, 0), pair_tgt;
-    for (int64_t e = 0; e < ne; ++e) {
-        const int64_t cnt = self ? nq : near_ptr[e + 1] - near_ptr[e];
-        for (int64_t t = 0; t < cnt; ++t) {
-            const int64_t i = self ? qtags[t + (int64_t)nq * e] : near_idx[near_ptr[e] + t];
-            if (i < 0 || i >= m) return set_error(ctx, INTI_ERR_ARG, "bdim: near target out of range");
-            pair_tgt.push_back((int32_t)i);
+    // ---- host-side lists per type (element -> targets), concatenated
+    std::vector<int64_t> tag_off(ntypes + 1, 0), el_off(ntypes + 1, 0), pair_off(ntypes + 1, 0),
+        tri_off(ntypes + 1, 0), elems_off(ntypes + 1, 0);
+    std::vector<int32_t> pair_ptr, pair_tgt, elems;  // pair_ptr: ne[t] + 1 per type, local to the type
+    int64_t nidx_off = 0, nptr_off = 0;
+    for (int t = 0; t < ntypes; ++t) {
+        tag_off[t + 1] = tag_off[t] + (int64_t)nq[t] * ne[t];
+        const int64_t *qt = qtags + tag_off[t];
+        const int64_t *np = self ? nullptr : near_ptr + nptr_off;
+        const int64_t *ni = self ? nullptr : near_idx + nidx_off;
+        const int64_t p0 = (int64_t)pair_tgt.size();
+        pair_ptr.push_back(0);
+        for (int64_t e = 0; e < ne[t]; ++e) {
+            const int64_t cnt = self ? nq[t] : np[e + 1] - np[e];
+            if (cnt < 0) return set_error(ctx, INTI_ERR_ARG, "bdim: near lists must be non-decreasing");
+            for (int64_t q = 0; q < cnt; ++q) {
+                const int64_t i = self ? qt[q + (int64_t)nq[t] * e] : ni[np[e] + q];
+                if (i < 0 || i >= m) return set_error(ctx, INTI_ERR_ARG, "bdim: near target out of range");
+                pair_tgt.push_back((int32_t)i);
+            }
+            pair_ptr.push_back((int32_t)((int64_t)pair_tgt.size() - p0));
+            if (cnt > 0) elems.push_back((int32_t)e);
         }
-        pair_ptr[e + 1] = (int32_t)pair_tgt.size();
+        el_off[t + 1] = el_off[t] + ne[t] + 1;
+        pair_off[t + 1] = (int64_t)pair_tgt.size();
+        tri_off[t + 1] = tri_off[t] + (pair_off[t + 1] - pair_off[t]) * nq[t];
+        elems_off[t + 1] = (int64_t)elems.size();
+        if (!self) {
+            nidx_off += near_ptr[nptr_off + ne[t]];
+            nptr_off += ne[t] + 1;
+        }
     }
     const int64_t npairs = (int64_t)pair_tgt.size();
     std::vector<int32_t> loc(m, -1), glob;
@@ -408,19 +576,21 @@ int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *q
     std::vector<int32_t> pair_loc(npairs);
     for (int64_t p = 0; p < npairs; ++p) pair_loc[p] = loc[pair_tgt[p]];
     const int64_t nloc = (int64_t)glob.size();
-    std::vector<int32_t> elems;
-    for (int64_t e = 0; e < ne; ++e)
-        if (pair_ptr[e + 1] > pair_ptr[e]) elems.push_back((int32_t)e);
-    // triplets (pair*nq + r) -> (row, col), CSR pattern
-    const int64_t ntri = npairs * nq;
+    // triplets (type, element, target, node) -> (row, col), CSR pattern
+    const int64_t ntri = tri_off[ntypes];
     if (ntri >= (int64_t)INT32_MAX) return set_error(ctx, INTI_ERR_SIZE, "bdim: too many triplets");
     std::vector<int32_t> tri_row(ntri), tri_col(ntri);
-    for (int64_t e = 0; e < ne; ++e)
-        for (int32_t p = pair_ptr[e]; p < pair_ptr[e + 1]; ++p)
-            for (int r = 0; r < nq; ++r) {
-                tri_row[(int64_t)p * nq + r] = pair_tgt[p];
-                tri_col[(int64_t)p * nq + r] = (int32_t)qtags[r + (int64_t)nq * e];
-            }
+    for (int t = 0; t < ntypes; ++t) {
+        const int64_t *qt = qtags + tag_off[t];
+        const int32_t *pp = pair_ptr.data() + el_off[t];
+        for (int64_t e = 0; e < ne[t]; ++e)
+            for (int32_t p = pp[e]; p < pp[e + 1]; ++p)
+                for (int r = 0; r < nq[t]; ++r) {
+                    const int64_t q = tri_off[t] + (int64_t)p * nq[t] + r;
+                    tri_row[q] = pair_tgt[pair_off[t] + p];
+                    tri_col[q] = (int32_t)qt[r + (int64_t)nq[t] * e];
+                }
+    }
     TripletPattern pat;
     build_pattern(m, tri_row, tri_col, pat);
     const std::vector<int32_t> &rowptr = pat.rowptr, &colind = pat.colind, &seg = pat.seg, &perm = pat.perm;
@@ -429,9 +599,9 @@ int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *q
     // ---- device buffers
     std::vector<double> zs;
     monopoles(src, ns, radius_mult, zs);
-    std::vector<int32_t> qt32((size_t)nq * ne);
+    std::vector<int32_t> qt32((size_t)ntags);
     for (size_t q = 0; q < qt32.size(); ++q) qt32[q] = (int32_t)qtags[q];
-    DevBuf dzs, dqt, dglob, dpp, dploc, delems, dB0, dB1, dpart, dth, dtriS, dtriD, ddiag, dseg, dperm, dmu;
+    DevBuf dzs, dqt, dglob, dpp, dploc, delems, dB0, dB1, dpart, drec, dth, dtriS, dtriD, ddiag, dseg, dperm, dmu;
     const size_t cb = sizeof(double2);
     INTI_ALLOC(ctx, dzs, zs.size() * sizeof(double));
     INTI_ALLOC(ctx, dqt, qt32.size() * sizeof(int32_t));
@@ -474,53 +644,76 @@ int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *q
         fns.traces(a, dim3((unsigned)((n * nsp + 255) / 256)), st);
         INTI_HIP(ctx, hipGetLastError());
     }
-    // Theta partial sums
-    int split = 1;
-    int64_t chunk = n;
+    // Theta: pack the per-source records, then the sum (and the reduction of
+    // its partial sums when the sources are split over several blocks)
     if (nloc > 0) {
-        const int64_t ntiles = (nloc + kThetaBS - 1) / kThetaBS;
-        const int64_t stiles = (n + kThetaTT - 1) / kThetaTT;
-        int64_t s = (8LL * ctx->num_cu + ntiles - 1) / ntiles;
-        if (s >= 8) s = (s + 7) / 8 * 8;
-        if (s > stiles) s = stiles;
-        if (s < 1) s = 1;
-        chunk = (stiles + s - 1) / s * kThetaTT;
-        split = (int)((n + chunk - 1) / chunk);
-        INTI_ALLOC(ctx, dpart, cb * split * nloc * nsp);
+        const int pk = opS->pde == INTI_HELMHOLTZ ? 4 : 2;
+        const int rl = 2 * dim + pk * nst;
+        INTI_ALLOC(ctx, drec, sizeof(double) * n * rl);
+        ThetaPackArgs pa;
+        pa.n = n; pa.ns = ns; pa.nsp = nsp; pa.nst = nst; pa.rl = rl;
+        pa.sx = src->coords; pa.sn = src->normals; pa.sw = src->weights;
+        pa.B0 = (const double2 *)dB0.p; pa.B1 = (const double2 *)dB1.p;
+        pa.rec = (double *)drec.p;
+        basic_post(opS->pde, dim, deriv ? INTI_ADL : INTI_SL, opS->k, pa.psr, pa.psi);
+        basic_post(opS->pde, dim, deriv ? INTI_HS : INTI_DL, opS->k, pa.pdr, pa.pdi);
+        fns.pack(pa, dim3((unsigned)((n * (nst + 1) + 255) / 256)), st);
+        INTI_HIP(ctx, hipGetLastError());
+        // source split: the fewest rounds of resident blocks per unit of work,
+        // the smaller split on ties (every split adds nloc x nst partials)
+        const int64_t ntiles = (nloc + 63) / 64;
+        const int64_t slots = (int64_t)fns.theta_occ * ctx->num_cu;
+        const int64_t max_split = std::max<int64_t>(1, std::min<int64_t>(64, n / (64 * kThetaWaves)));
+        int64_t best = 1;
+        double best_cost = 1e300;
+        for (int64_t sp = 1; sp <= max_split; ++sp) {
+            const double rounds = (double)((ntiles * sp + slots - 1) / slots);
+            const double cost = rounds / (double)sp;
+            if (cost < best_cost * (1 - 1e-9)) { best_cost = cost; best = sp; }
+        }
+        const int split = (int)best;
+        const int64_t chunk = (n + split - 1) / split;
+        if (split > 1) INTI_ALLOC(ctx, dpart, sizeof(double2) * split * nst * nloc);
         ThetaArgs a;
-        a.m = m; a.n = n; a.nloc = nloc; a.nsp = nsp; a.split = split; a.chunk = chunk;
+        a.m = m; a.n = n; a.nloc = nloc;
+        a.ns = ns; a.rl = rl; a.split = split; a.chunk = chunk;
         a.glob = (const int32_t *)dglob.p;
         a.tx = trg->coords; a.tn = trg->normals;
-        a.sx = src->coords; a.sn = src->normals; a.sw = src->weights;
-        a.B0 = (const double2 *)dB0.p; a.B1 = (const double2 *)dB1.p;
+        a.rec = (const double *)drec.p;
+        a.zs = (const double *)dzs.p; a.mu = (const double *)dmu.p;
         a.part = (double2 *)dpart.p;
+        a.theta = (double2 *)dth.p;
         a.p = kp;
         fns.theta(a, dim3((unsigned)(ntiles * split)), st);
         INTI_HIP(ctx, hipGetLastError());
-        ThetaRedArgs r;
-        r.m = m; r.nloc = nloc; r.ns = ns; r.nsp = nsp; r.split = split;
-        r.glob = (const int32_t *)dglob.p;
-        r.tx = trg->coords; r.tn = trg->normals; r.zs = (const double *)dzs.p;
-        r.mu = (const double *)dmu.p;
-        r.part = (const double2 *)dpart.p;
-        r.theta = (double2 *)dth.p;
-        r.p = kp;
-        fns.reduce(r, dim3((unsigned)((nloc * ns + 255) / 256)), st);
-        INTI_HIP(ctx, hipGetLastError());
+        if (split > 1) {
+            ThetaRedArgs r;
+            r.m = m; r.nloc = nloc; r.ns = ns; r.nst = nst; r.split = split;
+            r.glob = (const int32_t *)dglob.p;
+            r.tx = trg->coords; r.tn = trg->normals; r.zs = (const double *)dzs.p;
+            r.mu = (const double *)dmu.p;
+            r.part = (const double2 *)dpart.p;
+            r.theta = (double2 *)dth.p;
+            r.p = kp;
+            fns.reduce(r, dim3((unsigned)((nloc * ns + 255) / 256)), st);
+            INTI_HIP(ctx, hipGetLastError());
+        }
     }
-    // batched QR + LS
-    if (!elems.empty()) {
+    // batched QR + LS, one launch per element type
+    for (int t = 0; t < ntypes; ++t) {
+        const int64_t nel = elems_off[t + 1] - elems_off[t];
+        if (nel == 0) continue;
         LsArgs a;
-        a.n = n; a.nloc = nloc; a.nq = nq; a.ns = ns;
-        a.elems = (const int32_t *)delems.p;
-        a.qtags = (const int32_t *)dqt.p;
-        a.pair_ptr = (const int32_t *)dpp.p;
-        a.pair_loc = (const int32_t *)dploc.p;
+        a.n = n; a.nloc = nloc; a.nq = nq[t]; a.ns = ns;
+        a.elems = (const int32_t *)delems.p + elems_off[t];
+        a.qtags = (const int32_t *)dqt.p + tag_off[t];
+        a.pair_ptr = (const int32_t *)dpp.p + el_off[t];
+        a.pair_loc = (const int32_t *)dploc.p + pair_off[t];
         a.B0 = (const double2 *)dB0.p; a.B1 = (const double2 *)dB1.p; a.ldb = n;
         a.theta = (const double2 *)dth.p;
-        a.triS = (double2 *)dtriS.p; a.triD = (double2 *)dtriD.p;
-        a.diag = (double *)ddiag.p;
-        hipLaunchKernelGGL(k_ls, dim3((unsigned)elems.size()), dim3(64), 0, st, a);
+        a.triS = (double2 *)dtriS.p + tri_off[t]; a.triD = (double2 *)dtriD.p + tri_off[t];
+        a.diag = (double *)ddiag.p + 3 * elems_off[t];
+        hipLaunchKernelGGL(k_ls, dim3((unsigned)nel), dim3(64), 0, st, a);
         INTI_HIP(ctx, hipGetLastError());
     }
     // CSR
@@ -565,6 +758,13 @@ int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *q
     *outS = cS;
     *outD = cD;
     return INTI_OK;
+}
+
+int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *qtags,
+               const int64_t *near_ptr, const int64_t *near_idx, const double *mu,
+               const double *params, Csr **outS, Csr **outD, double *diag_out) {
+    return bdim_build_types(opS, opD, deriv, 1, &nq, &ne, qtags, near_ptr, near_idx, mu, params, outS, outD,
+                            diag_out);
 }
 
 }  // namespace inti

@@ -128,6 +128,7 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
                double *nbar, double *wbar, double *xbar, const double *dcoefs, double *kbar);
 
 bool op_is_complex(const Op *op);
+int microbench(Ctx *ctx, int op, double *rate);
 int dense_gemm(Ctx *ctx, int cplx, int64_t m, int64_t n, int nrhs, const double *A, int64_t lda, const double *X,
                int64_t ldx, double *Y, int64_t ldy, const double alpha[2], const double beta[2]);
 

@@ -135,37 +135,66 @@ struct HipBuf {
     }
 };
 
-int lsq_build(Ctx *ctx, int cplx, int nrows, int ncols, int64_t ne, const double *A, const int64_t *cols,
-              int64_t m, int64_t n, const int64_t *near_ptr, const int64_t *near_idx, const double *rhs,
-              Csr **out, double *diag_out) {
-    if (nrows < 1 || ncols < 1 || std::max(nrows, ncols) > kMaxNs || std::min(nrows, ncols) > kMaxNc)
-        return set_error(ctx, INTI_ERR_UNSUPPORTED, "lsq: need 1 <= min(nrows, ncols) <= 24 and max(nrows, ncols) <= 32");
+// Element types: type t has ncols[t] columns (nodes) and ne[t] elements; A
+// holds the nrows x ncols[t] matrices of its elements after the earlier
+// types', cols the ncols[t] x ne[t] column-major block after theirs, and
+// (near_ptr non-NULL) near_ptr ne[t] + 1 entries per type (each block from
+// 0) indexing the block of near_idx after the earlier types' targets. One
+// k_lsq launch per type (vdim.jl:76-121 loops over the element types).
+int lsq_build_types(Ctx *ctx, int cplx, int nrows, int ntypes, const int *ncols, const int64_t *ne, const double *A,
+                    const int64_t *cols, int64_t m, int64_t n, const int64_t *near_ptr, const int64_t *near_idx,
+                    const double *rhs, Csr **out, double *diag_out) {
+    if (ntypes < 1) return set_error(ctx, INTI_ERR_ARG, "lsq: no element type");
+    for (int t = 0; t < ntypes; ++t)
+        if (nrows < 1 || ncols[t] < 1 || ne[t] < 0 || std::max(nrows, ncols[t]) > kMaxNs ||
+            std::min(nrows, ncols[t]) > kMaxNc)
+            return set_error(ctx, INTI_ERR_UNSUPPORTED,
+                             "lsq: need 1 <= min(nrows, ncols) <= 24 and max(nrows, ncols) <= 32");
     const bool self = near_ptr == nullptr;
     if (self && m != n) return set_error(ctx, INTI_ERR_ARG, "lsq: the self pattern needs m == n");
-    for (int64_t q = 0; q < (int64_t)ncols * ne; ++q)
-        if (cols[q] < 0 || cols[q] >= n) return set_error(ctx, INTI_ERR_ARG, "lsq: column index out of range");
-    // element -> target pairs and the triplet pattern
-    std::vector<int32_t> pair_ptr(ne + 1, 0), pair_tgt;
-    for (int64_t e = 0; e < ne; ++e) {
-        const int64_t cnt = self ? ncols : near_ptr[e + 1] - near_ptr[e];
-        for (int64_t t = 0; t < cnt; ++t) {
-            const int64_t i = self ? cols[t + (int64_t)ncols * e] : near_idx[near_ptr[e] + t];
-            if (i < 0 || i >= m) return set_error(ctx, INTI_ERR_ARG, "lsq: target out of range");
-            pair_tgt.push_back((int32_t)i);
-        }
-        pair_ptr[e + 1] = (int32_t)pair_tgt.size();
-    }
-    const int64_t npairs = (int64_t)pair_tgt.size();
-    if (npairs * ncols >= (int64_t)INT32_MAX) return set_error(ctx, INTI_ERR_SIZE, "lsq: too many triplets");
-    std::vector<int32_t> elems, tri_row(npairs * ncols), tri_col(npairs * ncols);
-    for (int64_t e = 0; e < ne; ++e) {
-        if (pair_ptr[e + 1] > pair_ptr[e]) elems.push_back((int32_t)e);
-        for (int32_t p = pair_ptr[e]; p < pair_ptr[e + 1]; ++p)
-            for (int c = 0; c < ncols; ++c) {
-                tri_row[(int64_t)p * ncols + c] = pair_tgt[p];
-                tri_col[(int64_t)p * ncols + c] = (int32_t)cols[c + (int64_t)ncols * e];
+    // per-type offsets
+    std::vector<int64_t> a_off(ntypes + 1, 0), c_off(ntypes + 1, 0), pp_off(ntypes + 1, 0), pt_off(ntypes + 1, 0),
+        tri_off(ntypes + 1, 0), el_off(ntypes + 1, 0);
+    std::vector<int32_t> pair_ptr, pair_tgt, elems, tri_row, tri_col;
+    int64_t nptr = 0, nidx = 0;
+    for (int t = 0; t < ntypes; ++t) {
+        const int nc = ncols[t];
+        const int64_t *ct = cols + c_off[t];
+        a_off[t + 1] = a_off[t] + ne[t] * nrows * nc;
+        c_off[t + 1] = c_off[t] + (int64_t)nc * ne[t];
+        for (int64_t q = 0; q < (int64_t)nc * ne[t]; ++q)
+            if (ct[q] < 0 || ct[q] >= n) return set_error(ctx, INTI_ERR_ARG, "lsq: column index out of range");
+        const int64_t *np = self ? nullptr : near_ptr + nptr;
+        const int64_t *ni = self ? nullptr : near_idx + nidx;
+        const int64_t p0 = (int64_t)pair_tgt.size();
+        pair_ptr.push_back(0);
+        for (int64_t e = 0; e < ne[t]; ++e) {
+            const int64_t cnt = self ? nc : np[e + 1] - np[e];
+            if (cnt < 0) return set_error(ctx, INTI_ERR_ARG, "lsq: near lists must be non-decreasing");
+            for (int64_t q = 0; q < cnt; ++q) {
+                const int64_t i = self ? ct[q + (int64_t)nc * e] : ni[np[e] + q];
+                if (i < 0 || i >= m) return set_error(ctx, INTI_ERR_ARG, "lsq: target out of range");
+                pair_tgt.push_back((int32_t)i);
             }
+            pair_ptr.push_back((int32_t)((int64_t)pair_tgt.size() - p0));
+            if (cnt > 0) elems.push_back((int32_t)e);
+            for (int64_t q = 0; q < cnt; ++q)
+                for (int c = 0; c < nc; ++c) {
+                    tri_row.push_back(pair_tgt[p0 + pair_ptr[pp_off[t] + e] + q]);
+                    tri_col.push_back((int32_t)ct[c + (int64_t)nc * e]);
+                }
+        }
+        pp_off[t + 1] = pp_off[t] + ne[t] + 1;
+        pt_off[t + 1] = (int64_t)pair_tgt.size();
+        tri_off[t + 1] = (int64_t)tri_row.size();
+        el_off[t + 1] = (int64_t)elems.size();
+        if (tri_off[t + 1] >= (int64_t)INT32_MAX) return set_error(ctx, INTI_ERR_SIZE, "lsq: too many triplets");
+        if (!self) {
+            nidx += np[ne[t]];
+            nptr += ne[t] + 1;
+        }
     }
+    const int64_t npairs = (int64_t)pair_tgt.size(), ntri = tri_off[ntypes], nel = (int64_t)elems.size();
     TripletPattern P;
     build_pattern(m, tri_row, tri_col, P);
     const int64_t nnz = (int64_t)P.colind.size();
@@ -173,16 +202,16 @@ int lsq_build(Ctx *ctx, int cplx, int nrows, int ncols, int64_t ne, const double
     hipStream_t s = ctx->stream;
     HipBuf dA, dpp, dpt, del, drhs, dtri, ddiag, dseg, dperm;
     int rc;
-    if ((rc = dA.alloc(ctx, sizeof(double) * ne * nrows * ncols)) || (rc = dpp.alloc(ctx, sizeof(int32_t) * (ne + 1))) ||
-        (rc = dpt.alloc(ctx, sizeof(int32_t) * npairs)) || (rc = del.alloc(ctx, sizeof(int32_t) * elems.size())) ||
-        (rc = drhs.alloc(ctx, sizeof(double2) * m * nrows)) || (rc = dtri.alloc(ctx, sizeof(double2) * npairs * ncols)) ||
-        (rc = ddiag.alloc(ctx, sizeof(double) * 2 * elems.size())) || (rc = dseg.alloc(ctx, sizeof(int32_t) * P.seg.size())) ||
+    if ((rc = dA.alloc(ctx, sizeof(double) * a_off[ntypes])) || (rc = dpp.alloc(ctx, sizeof(int32_t) * pair_ptr.size())) ||
+        (rc = dpt.alloc(ctx, sizeof(int32_t) * npairs)) || (rc = del.alloc(ctx, sizeof(int32_t) * nel)) ||
+        (rc = drhs.alloc(ctx, sizeof(double2) * m * nrows)) || (rc = dtri.alloc(ctx, sizeof(double2) * ntri)) ||
+        (rc = ddiag.alloc(ctx, sizeof(double) * 2 * nel)) || (rc = dseg.alloc(ctx, sizeof(int32_t) * P.seg.size())) ||
         (rc = dperm.alloc(ctx, sizeof(int32_t) * P.perm.size())))
         return rc;
-    INTI_HIP(ctx, hipMemcpyAsync(dA.p, A, sizeof(double) * ne * nrows * ncols, hipMemcpyHostToDevice, s));
-    INTI_HIP(ctx, hipMemcpyAsync(dpp.p, pair_ptr.data(), sizeof(int32_t) * (ne + 1), hipMemcpyHostToDevice, s));
+    if (a_off[ntypes]) INTI_HIP(ctx, hipMemcpyAsync(dA.p, A, sizeof(double) * a_off[ntypes], hipMemcpyHostToDevice, s));
+    INTI_HIP(ctx, hipMemcpyAsync(dpp.p, pair_ptr.data(), sizeof(int32_t) * pair_ptr.size(), hipMemcpyHostToDevice, s));
     if (npairs) INTI_HIP(ctx, hipMemcpyAsync(dpt.p, pair_tgt.data(), sizeof(int32_t) * npairs, hipMemcpyHostToDevice, s));
-    if (!elems.empty()) INTI_HIP(ctx, hipMemcpyAsync(del.p, elems.data(), sizeof(int32_t) * elems.size(), hipMemcpyHostToDevice, s));
+    if (nel) INTI_HIP(ctx, hipMemcpyAsync(del.p, elems.data(), sizeof(int32_t) * nel, hipMemcpyHostToDevice, s));
     INTI_HIP(ctx, hipMemcpyAsync(dseg.p, P.seg.data(), sizeof(int32_t) * P.seg.size(), hipMemcpyHostToDevice, s));
     if (!P.perm.empty()) INTI_HIP(ctx, hipMemcpyAsync(dperm.p, P.perm.data(), sizeof(int32_t) * P.perm.size(), hipMemcpyHostToDevice, s));
     // right-hand sides as complex (m x nrows)
@@ -192,17 +221,19 @@ int lsq_build(Ctx *ctx, int cplx, int nrows, int ncols, int64_t ne, const double
         rc2[2 * q + 1] = cplx ? rhs[2 * q + 1] : 0.0;
     }
     INTI_HIP(ctx, hipMemcpyAsync(drhs.p, rc2.data(), sizeof(double) * rc2.size(), hipMemcpyHostToDevice, s));
-    if (!elems.empty()) {
+    for (int t = 0; t < ntypes; ++t) {
+        const int64_t cnt = el_off[t + 1] - el_off[t];
+        if (cnt == 0) continue;
         LsqArgs a;
-        a.nrows = nrows; a.ncols = ncols; a.m = m;
-        a.A = (const double *)dA.p;
-        a.elems = (const int32_t *)del.p;
-        a.pair_ptr = (const int32_t *)dpp.p;
-        a.pair_tgt = (const int32_t *)dpt.p;
+        a.nrows = nrows; a.ncols = ncols[t]; a.m = m;
+        a.A = (const double *)dA.p + a_off[t];
+        a.elems = (const int32_t *)del.p + el_off[t];
+        a.pair_ptr = (const int32_t *)dpp.p + pp_off[t];
+        a.pair_tgt = (const int32_t *)dpt.p + pt_off[t];
         a.rhs = (const double2 *)drhs.p;
-        a.tri = (double2 *)dtri.p;
-        a.diag = (double *)ddiag.p;
-        hipLaunchKernelGGL(k_lsq, dim3((unsigned)elems.size()), dim3(64), 0, s, a);
+        a.tri = (double2 *)dtri.p + tri_off[t];
+        a.diag = (double *)ddiag.p + 2 * el_off[t];
+        hipLaunchKernelGGL(k_lsq, dim3((unsigned)cnt), dim3(64), 0, s, a);
         INTI_HIP(ctx, hipGetLastError());
     }
     Csr *c = nullptr;
@@ -217,7 +248,7 @@ int lsq_build(Ctx *ctx, int cplx, int nrows, int ncols, int64_t ne, const double
     g.nnz = nnz; g.seg = (const int32_t *)dseg.p; g.perm = (const int32_t *)dperm.p;
     g.tri = (const double2 *)dtri.p; g.val = c->val; g.cplx = cplx; g.sign = 1.0;
     launch_gather1(g, s);
-    std::vector<double> dg(2 * elems.size());
+    std::vector<double> dg(2 * nel);
     if ((!dg.empty() && hipMemcpyAsync(dg.data(), ddiag.p, sizeof(double) * dg.size(), hipMemcpyDeviceToHost, s) != hipSuccess) ||
         hipStreamSynchronize(s) != hipSuccess) {
         csr_free(c);
@@ -225,11 +256,17 @@ int lsq_build(Ctx *ctx, int cplx, int nrows, int ncols, int64_t ne, const double
     }
     if (diag_out) {
         double r = 0, w = 0;
-        for (size_t q = 0; q < elems.size(); ++q) { r = std::max(r, dg[2 * q]); w = std::max(w, dg[2 * q + 1]); }
+        for (int64_t q = 0; q < nel; ++q) { r = std::max(r, dg[2 * q]); w = std::max(w, dg[2 * q + 1]); }
         diag_out[0] = r; diag_out[1] = w;
     }
     *out = c;
     return INTI_OK;
+}
+
+int lsq_build(Ctx *ctx, int cplx, int nrows, int ncols, int64_t ne, const double *A, const int64_t *cols,
+              int64_t m, int64_t n, const int64_t *near_ptr, const int64_t *near_idx, const double *rhs,
+              Csr **out, double *diag_out) {
+    return lsq_build_types(ctx, cplx, nrows, 1, &ncols, &ne, A, cols, m, n, near_ptr, near_idx, rhs, out, diag_out);
 }
 
 }  // namespace inti

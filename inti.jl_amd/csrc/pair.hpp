@@ -115,8 +115,13 @@ __device__ __forceinline__ void helm3_raw(const double *r, double d2s, bool ok, 
         f = mask(ok, id2);
     }
     double cf = c * f, sf = s * f;
-    kr = cf * br - sf * bi;
-    ki = cf * bi + sf * br;
+    if constexpr (KIND == INTI_SL) {  // B = 1: no products with 1 and 0
+        kr = cf;
+        ki = sf;
+    } else {
+        kr = cf * br - sf * bi;
+        ki = cf * bi + sf * br;
+    }
 }
 
 // MIXED 3D Helmholtz SL, DL, ADL and DL + ch*SL with the fewest VALU
@@ -338,6 +343,7 @@ __device__ __forceinline__ void pair_raw_sym(const double *r, const double *ni, 
 // direction (i <- j) adds to (a1r, a1i) with the source normal nj and q_j,
 // the direction (j <- i) to (a2r, a2i) with ni and q_i. TRANS accumulates
 // the transposed sums (K(x_j, x_i) q_j at i, K(x_i, x_j) q_i at j).
+// QExt / qext serve the one-direction form below.
 struct QExt {
     double qr, qi, cr, ci, kqi, mkqr;  // q, ch q, k Im q, -k Re q
 };
@@ -350,45 +356,52 @@ __device__ __forceinline__ QExt qext(double qr, double qi, const KParams &p) {
     q.mkqr = -p.k * qr;
     return q;
 }
-template <bool MIXED, bool MASK, bool TRANS>
-__device__ __forceinline__ void helm3_sym_fused(const double *r, const double *ni, const double *nj,
-                                                const KParams &p, const QExt &qj, const QExt &qi,
-                                                double &a1r, double &a1i, double &a2r, double &a2i) {
+// The mixed symmetric form: the distance, 1/d and the phase as in
+// helm3_raw_mixed (FP64 geometry and phase, FP32 transcendentals); the
+// staged density terms, both directions' products and their running sums in
+// FP32 (non-packed FP32 arithmetic issues at twice the FP64 rate). The
+// caller adds the FP32 sums into FP64 every 64 sources, so an FP32 sum never
+// holds more than 64 terms.
+struct QExt32 {
+    float qr, qi, cr, ci, kqi, mkqr;  // q, ch q, k Im q, -k Re q
+};
+__device__ __forceinline__ QExt32 qext32(double qr, double qi, const KParams &p) {
+    QExt32 q;
+    q.qr = (float)qr;
+    q.qi = (float)qi;
+    q.cr = (float)(p.chr * qr - p.chi * qi);
+    q.ci = (float)(p.chr * qi + p.chi * qr);
+    q.kqi = (float)(p.k * qi);
+    q.mkqr = (float)(-p.k * qr);
+    return q;
+}
+template <bool MASK, bool TRANS>
+__device__ __forceinline__ void helm3_sym_fused32(const double *r, const double *ni, const double *nj,
+                                                  const KParams &p, const QExt32 &qj, const QExt32 &qi,
+                                                  float &a1r, float &a1i, float &a2r, float &a2i) {
     const double d2 = dotv<3>(r, r);
-    double id, cf, sf;
-    if constexpr (MIXED) {
-        const bool ok = !MASK || d2 > kSamePointTol2;
-        const float y32 = __builtin_amdgcn_rsqf(MASK ? fmaxf((float)d2, 1.0e-30f) : (float)d2);
-        id = (double)y32;
-        const double e = d2 * id;
-        const double t = (p.kappa_half * e) * __builtin_fma(-e, id, 3.0);  // kappa d, one Newton step
-        const float u = (float)__builtin_amdgcn_fract(t);
-        const float ym = ok ? y32 : 0.0f;
-        cf = (double)(__builtin_amdgcn_cosf(u) * ym);
-        sf = (double)(__builtin_amdgcn_sinf(u) * ym);
-    } else {
-        const bool ok = !MASK || d2 > kSamePointTol2;
-        const double d2s = MASK ? fmax(d2, kClampD2) : d2;
-        id = rsqp<false>(d2s);
-        double s, c;
-        sincos_rev<false>(p.kappa * (d2s * id), s, c);
-        const double f = MASK ? mask(ok, id) : id;
-        cf = c * f;
-        sf = s * f;
-    }
-    const double aj = dotv<3>(r, nj) * id;    // K(x_i, x_j): source normal nj
-    const double ai = -dotv<3>(r, ni) * id;   // K(x_j, x_i): source normal ni, r -> -r
-    const double b1 = TRANS ? ai : aj, b2 = TRANS ? aj : ai;
+    const bool ok = !MASK || d2 > kSamePointTol2;
+    const float y = __builtin_amdgcn_rsqf(MASK ? fmaxf((float)d2, 1.0e-30f) : (float)d2);
+    const double id = (double)y;
+    const double e = d2 * id;
+    const double t = (p.kappa_half * e) * __builtin_fma(-e, id, 3.0);  // kappa d, one Newton step
+    const float u = (float)__builtin_amdgcn_fract(t);
+    const float ym = ok ? y : 0.0f;
+    const float cf = __builtin_amdgcn_cosf(u) * ym;
+    const float sf = __builtin_amdgcn_sinf(u) * ym;
+    const float aj = (float)dotv<3>(r, nj) * y;    // K(x_i, x_j): source normal nj
+    const float ai = -(float)dotv<3>(r, ni) * y;   // K(x_j, x_i): source normal ni, r -> -r
+    const float b1 = TRANS ? ai : aj, b2 = TRANS ? aj : ai;
     // (i <- j): q_j
-    const double w1r = __builtin_fma(id, qj.qr, qj.kqi), w1i = __builtin_fma(id, qj.qi, qj.mkqr);
-    const double z1r = __builtin_fma(b1, w1r, qj.cr), z1i = __builtin_fma(b1, w1i, qj.ci);
-    a1r = __builtin_fma(cf, z1r, __builtin_fma(-sf, z1i, a1r));
-    a1i = __builtin_fma(cf, z1i, __builtin_fma(sf, z1r, a1i));
+    const float w1r = __builtin_fmaf(y, qj.qr, qj.kqi), w1i = __builtin_fmaf(y, qj.qi, qj.mkqr);
+    const float z1r = __builtin_fmaf(b1, w1r, qj.cr), z1i = __builtin_fmaf(b1, w1i, qj.ci);
+    a1r = __builtin_fmaf(cf, z1r, __builtin_fmaf(-sf, z1i, a1r));
+    a1i = __builtin_fmaf(cf, z1i, __builtin_fmaf(sf, z1r, a1i));
     // (j <- i): q_i
-    const double w2r = __builtin_fma(id, qi.qr, qi.kqi), w2i = __builtin_fma(id, qi.qi, qi.mkqr);
-    const double z2r = __builtin_fma(b2, w2r, qi.cr), z2i = __builtin_fma(b2, w2i, qi.ci);
-    a2r = __builtin_fma(cf, z2r, __builtin_fma(-sf, z2i, a2r));
-    a2i = __builtin_fma(cf, z2i, __builtin_fma(sf, z2r, a2i));
+    const float w2r = __builtin_fmaf(y, qi.qr, qi.kqi), w2i = __builtin_fmaf(y, qi.qi, qi.mkqr);
+    const float z2r = __builtin_fmaf(b2, w2r, qi.cr), z2i = __builtin_fmaf(b2, w2i, qi.ci);
+    a2r = __builtin_fmaf(cf, z2r, __builtin_fmaf(-sf, z2i, a2r));
+    a2i = __builtin_fmaf(cf, z2i, __builtin_fmaf(sf, z2r, a2i));
 }
 
 // One direction of the fused 3D Helmholtz DL / DL + ch*SL pair (target x,

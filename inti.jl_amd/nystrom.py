@@ -211,8 +211,10 @@ class IntegralOperator:
         """Reverse mode of y = A sigma on the GPU (kernel values recomputed):
         the gradients of Re<ubar, y> requested in `wrt` among "sigma"
         (A^H ubar), "source_nodes", "source_normals", "source_weights",
-        "target_nodes" and "k". dcoefs = (dc0/dk, dc1/dk) for combined kernels
-        with k-dependent coefficients. Returns a dict."""
+        "target_nodes", "nodes" (operators whose target is their source: the
+        gradient of the nodes, the sum of the source and target parts, in one
+        output) and "k". dcoefs = (dc0/dk, dc1/dk) for combined kernels with
+        k-dependent coefficients. Returns a dict."""
         m, n = self.shape
         dim = self.source.dim
         u = np.ascontiguousarray(ubar, dtype=self.dtype).reshape(m)
@@ -224,12 +226,20 @@ class IntegralOperator:
             "source_normals": np.zeros((dim, n)),
             "source_weights": np.zeros(n),
             "target_nodes": np.zeros((dim, m)),
+            "nodes": np.zeros((dim, n)),
             "k": np.zeros(1),
         }
         for w in wrt:
             if w not in bufs:
                 raise ValueError(f"unknown gradient {w!r}")
+        if "nodes" in wrt:
+            if self.target is not self.source:
+                raise ValueError("'nodes' needs an operator whose target is its source")
+            if "source_nodes" in wrt or "target_nodes" in wrt:
+                raise ValueError("'nodes' is the sum of 'source_nodes' and 'target_nodes': ask for one or the parts")
         ptr = {w: (_lib.dptr(bufs[w]) if w in wrt else None) for w in bufs}
+        if "nodes" in wrt:  # one output for both parts: the library sums them
+            ptr["source_nodes"] = ptr["target_nodes"] = ptr["nodes"]
         dc = None
         if dcoefs is not None:
             d0, d1 = complex(dcoefs[0]), complex(dcoefs[1])
@@ -285,6 +295,33 @@ def gmres(A, b, x0=None, *, restart=100, maxiter=100, rtol=1e-6, atol=0.0, use_g
                                      x.ctypes.data_as(C.c_void_p), restart, maxiter, rtol, atol,
                                      1 if use_graph else 0, C.byref(it), C.byref(rr)))
     return x, {"iterations": it.value, "relres": rr.value}
+
+
+def gmres_adjoint(A, g, x0=None, *, restart=100, maxiter=300, rtol=1e-10):
+    """Solves A^H lam = g, the adjoint solve of an objective that depends on
+    x = A^{-1} b (dJ/db = lam, dJ/dA = -lam x^H). A^H is applied on the GPU
+    as A's pullback with sigma-bar alone (the transposed direct sum at the
+    forward's cost, the correction's adjoint and the conjugated identity
+    included); the GMRES iteration runs on the host (scipy). Returns
+    (lam, info)."""
+    import scipy.sparse.linalg as sla
+    A = _iop(A)
+    m, n = A.shape
+    if m != n:
+        raise ValueError("gmres_adjoint needs a square operator")
+    dummy = np.zeros(n, dtype=A.dtype)
+    count = [0]
+
+    def mv(u):
+        count[0] += 1
+        return A.pullback(np.asarray(u, dtype=A.dtype), dummy, wrt=("sigma",))["sigma"]
+
+    op = sla.LinearOperator((n, n), matvec=mv, dtype=A.dtype)
+    g = np.ascontiguousarray(g, dtype=A.dtype)
+    lam, flag = sla.gmres(op, g, x0=x0, rtol=rtol, atol=0.0, restart=restart, maxiter=max(1, maxiter // restart))
+    res = float(np.linalg.norm(g - mv(lam)) / np.linalg.norm(g))
+    count[0] -= 1
+    return lam, {"iterations": count[0], "relres": res, "converged": flag == 0}
 
 
 def refine_solve(A_hi, A_lo, b, x0=None, *, restart=100, maxiter_inner=100, inner_rtol=1e-6,

@@ -94,6 +94,76 @@ TRIANGLE_VR_RULES = {2: (VR2N3_NODES, VR2N3_WEIGHTS), 4: (VR4N6_NODES, VR4N6_WEI
 TRIANGLE_VR_IORDER_TO_QORDER = {1: 2, 2: 4, 3: 5}
 
 
+# TETRAHEDRON_VR2N4, VR3N10, VR5N20 (src/quad_rules_tables_tetrahedron.jl:222-385):
+# nodes on the reference tetrahedron {u, v, w >= 0, u + v + w <= 1}, weights
+# summing to 1/6; interpolation degrees 1, 2, 3 (src/quad_rules_tables.jl:145-157).
+TET_VR2N4_NODES = np.array([
+    [0.1381966011250106, 0.13819660112501048, 0.13819660112501042],
+    [0.5854101966249685, 0.13819660112501053, 0.13819660112501053],
+    [0.13819660112501048, 0.5854101966249685, 0.13819660112501053],
+    [0.13819660112501053, 0.13819660112501053, 0.5854101966249683],
+])
+TET_VR2N4_WEIGHTS = np.array([0.04166666666666665, 0.041666666666666685, 0.041666666666666664,
+                              0.041666666666666685])
+TET_VR3N10_NODES = np.array([
+    [0.07714091970968884, 0.07714091970968884, 0.07714091970968884],
+    [0.40597452087424457, 0.09402547912575537, 0.09402547912575543],
+    [0.7685772408709335, 0.07714091970968878, 0.07714091970968884],
+    [0.09402547912575543, 0.4059745208742447, 0.09402547912575532],
+    [0.4059745208742447, 0.40597452087424457, 0.09402547912575543],
+    [0.07714091970968884, 0.7685772408709336, 0.07714091970968884],
+    [0.09402547912575543, 0.09402547912575537, 0.4059745208742445],
+    [0.4059745208742446, 0.09402547912575537, 0.4059745208742445],
+    [0.09402547912575543, 0.4059745208742447, 0.4059745208742445],
+    [0.07714091970968884, 0.07714091970968884, 0.7685772408709335],
+])
+TET_VR3N10_WEIGHTS = np.array([0.008403127078534032, 0.022175693058755084, 0.008403127078534035,
+                               0.022175693058755084, 0.022175693058755084, 0.008403127078534032,
+                               0.022175693058755098, 0.022175693058755098, 0.022175693058755098,
+                               0.008403127078534035])
+TET_VR5N20_NODES = np.array([
+    [0.06994279207826776, 0.0699427920782677, 0.06994279207826776],
+    [0.2928644237211868, 0.06164442753123828, 0.06164442753123833],
+    [0.5838467212163365, 0.06164442753123828, 0.06164442753123833],
+    [0.7901716237651969, 0.0699427920782677, 0.0699427920782677],
+    [0.06164442753123828, 0.2928644237211868, 0.06164442753123833],
+    [0.30974425517318926, 0.30974425517318926, 0.0707672344804322],
+    [0.5838467212163364, 0.2928644237211869, 0.06164442753123833],
+    [0.06164442753123833, 0.5838467212163365, 0.06164442753123828],
+    [0.2928644237211869, 0.5838467212163365, 0.06164442753123833],
+    [0.0699427920782677, 0.7901716237651969, 0.0699427920782677],
+    [0.06164442753123828, 0.06164442753123828, 0.2928644237211868],
+    [0.30974425517318926, 0.0707672344804322, 0.3097442551731892],
+    [0.5838467212163365, 0.06164442753123833, 0.2928644237211868],
+    [0.07076723448043226, 0.3097442551731893, 0.30974425517318915],
+    [0.30974425517318926, 0.30974425517318926, 0.30974425517318926],
+    [0.06164442753123828, 0.5838467212163365, 0.2928644237211868],
+    [0.06164442753123828, 0.06164442753123833, 0.5838467212163364],
+    [0.2928644237211868, 0.06164442753123833, 0.5838467212163365],
+    [0.06164442753123828, 0.2928644237211869, 0.5838467212163365],
+    [0.0699427920782677, 0.06994279207826765, 0.7901716237651969],
+])
+TET_VR5N20_WEIGHTS = np.array([0.004559181826208037, 0.006804733769335409, 0.0068047337693354125,
+                               0.004559181826208037, 0.006804733769335409, 0.016693283532452394,
+                               0.006804733769335416, 0.006804733769335412, 0.006804733769335412,
+                               0.004559181826208037, 0.006804733769335412, 0.016693283532452394,
+                               0.006804733769335415, 0.016693283532452394, 0.016693283532452394,
+                               0.0068047337693354125, 0.006804733769335416, 0.006804733769335416,
+                               0.006804733769335416, 0.004559181826208035])
+# TETRAHEDRON_VR_ORDER_TO_NPTS (src/quad_rules_tables.jl:128-140), the rules tabulated here
+TETRAHEDRON_VR_RULES = {2: (TET_VR2N4_NODES, TET_VR2N4_WEIGHTS), 3: (TET_VR3N10_NODES, TET_VR3N10_WEIGHTS),
+                        5: (TET_VR5N20_NODES, TET_VR5N20_WEIGHTS)}
+
+
+def vioreanu_rokhlin_tetrahedron(order):
+    if order not in TETRAHEDRON_VR_RULES:
+        raise ValueError(
+            f"VioreanuRokhlin quadrature of order {order} not available for ReferenceTetrahedron "
+            f"(available: {sorted(TETRAHEDRON_VR_RULES)})")
+    x, w = TETRAHEDRON_VR_RULES[order]
+    return x.copy(), w.copy()
+
+
 def triangle_vr_interpolation_order_to_quadrature_order(i):
     """Triangle_VR_interpolation_order_to_quadrature_order
     (src/reference_integration.jl:320-322)."""
@@ -145,17 +215,33 @@ def p2_triangle_eval(vals, uv):
 # --------------------------------------------------------------- Quadrature
 class Quadrature:
     """SoA quadrature: coords (dim, N), normals (dim, N), weights (N,),
-    qtags (nq, ne) element-major (node i of element e is qtags[i, e])."""
+    qtags (nq, ne) element-major (node i of element e is qtags[i, e]).
+
+    Several element types (Quadrature.etype2qtags, src/quadrature.jl:60-120):
+    pass qtags as a list with one (nq_t, ne_t) array per type; etype_qtags
+    is that list (one entry for a single type), and qtags / nq / ne are
+    defined only for a single type."""
 
     def __init__(self, coords, normals, weights, qtags, elements=None, name="", qorder=None):
         self.qorder = qorder
         self.coords = np.ascontiguousarray(coords, dtype=np.float64)
         self.normals = None if normals is None else np.ascontiguousarray(normals, dtype=np.float64)
         self.weights = np.ascontiguousarray(weights, dtype=np.float64)
-        self.qtags = np.ascontiguousarray(qtags, dtype=np.int64)
+        if isinstance(qtags, (list, tuple)):
+            self.etype_qtags = [np.ascontiguousarray(q, dtype=np.int64) for q in qtags]
+        else:
+            self.etype_qtags = [np.ascontiguousarray(qtags, dtype=np.int64)]
+        if not self.etype_qtags or any(q.ndim != 2 for q in self.etype_qtags):
+            raise ValueError("qtags: one (nq, ne) array per element type")
         self.elements = elements
         self.name = name
         self._dev = {}
+
+    @property
+    def qtags(self):
+        if len(self.etype_qtags) != 1:
+            raise ValueError(f"{len(self.etype_qtags)} element types: use etype_qtags")
+        return self.etype_qtags[0]
 
     @property
     def dim(self):
@@ -193,11 +279,33 @@ class Quadrature:
             except Exception:
                 pass
 
+    def etype_flat(self):
+        """(nq per type int32, ne per type int64, tags): the column-major
+        (nq_t, ne_t) blocks of every type one after another."""
+        nqs = np.array([q.shape[0] for q in self.etype_qtags], dtype=np.int32)
+        nes = np.array([q.shape[1] for q in self.etype_qtags], dtype=np.int64)
+        tags = np.concatenate([q.T.reshape(-1) for q in self.etype_qtags]).astype(np.int64)
+        return nqs, nes, np.ascontiguousarray(tags)
+
     def subset(self, idx):
         """Point set of the nodes idx (no element structure), e.g. target[idx]."""
         idx = np.asarray(idx, dtype=np.int64)
         return Quadrature(self.coords[:, idx], None if self.normals is None else self.normals[:, idx],
                           self.weights[idx], np.zeros((1, 0), dtype=np.int64), name=self.name + "[sub]")
+
+
+def join_quadratures(*quads, name="joined"):
+    """One quadrature whose element types are the given quadratures' (a mesh
+    with several element types, each with its own rule, as
+    Quadrature(msh, etype2qrule) builds, src/quadrature.jl:103-120); node tags
+    are offset by the nodes that precede them."""
+    off, tags = 0, []
+    for Q in quads:
+        tags += [q + off for q in Q.etype_qtags]
+        off += len(Q)
+    nrm = None if any(Q.normals is None for Q in quads) else np.concatenate([Q.normals for Q in quads], axis=1)
+    return Quadrature(np.concatenate([Q.coords for Q in quads], axis=1), nrm,
+                      np.concatenate([Q.weights for Q in quads]), tags, name=name)
 
 
 def points(coords):
@@ -409,6 +517,26 @@ def curve_quadrature(elements, qorder, name="curve"):
                       qorder=2 * len(gauss_legendre_order(qorder)[0]) - 1)
 
 
+def area_quadrature(elements, qorder, name="area"):
+    """Vioreanu-Rokhlin quadrature (order qorder) on planar P2 triangles
+    (ne, 6, 2), nodes in element order (a 2D volume quadrature)."""
+    coords, w, qtags = _triangle_quadrature_2d(elements, qorder)
+    return Quadrature(coords, None, w, qtags, elements=elements, name=name, qorder=qorder)
+
+
+def surface_quadrature(elements, qorder, name="surface", center=None):
+    """Vioreanu-Rokhlin quadrature (order qorder) on P2 triangles (ne, 6, 3),
+    nodes in element order; normals along x_u x x_v, turned away from center
+    element by element when center is given (star-shaped surfaces)."""
+    coords, normals, w, qtags = _triangle_quadrature(elements, qorder)
+    if center is not None:
+        c = np.asarray(center, dtype=np.float64)[:, None]
+        nq = qtags.shape[0]
+        s = np.sign(np.sum((coords - c) * normals, axis=0)).reshape(-1, nq).sum(axis=1)
+        normals = normals * np.repeat(np.where(s < 0, -1.0, 1.0), nq)[None, :]
+    return Quadrature(coords, normals, w, qtags, elements=elements, name=name, qorder=qorder)
+
+
 def boundary_quadrature(volume, qorder):
     """Quadrature of order qorder on the outer boundary of a meshed 2D volume
     (the boundary the reference's volume_potential builds from the parent mesh
@@ -490,3 +618,76 @@ def ellipse_disk_quadratures(meshsize, a=1.0, b=1.0, qorder=5, boundary_qorder=N
             raise RuntimeError("boundary chord in the triangulation")
     return _p2_disk(V, T, tpar, a, b, qorder, boundary_qorder,
                     f"ellipse-disk(a={a},b={b},h={h},q={qorder})")
+
+
+def box_quadratures(ncell, half=0.5, qorder=3, boundary_qorder=4):
+    """(volume, boundary) quadratures of the cube [-half, half]^3: ncell^3
+    cells, each cut into six tetrahedra around its main diagonal (every
+    tetrahedron positively oriented), with the Vioreanu-Rokhlin tetrahedron
+    rule of order qorder (3: ten nodes, interpolation degree 2; 5: twenty
+    nodes, degree 3); the boundary is the six faces cut into two flat
+    triangles per cell face, Vioreanu-Rokhlin triangle rule of order
+    boundary_qorder, outward normals. Straight elements are exact here."""
+    uvw, wref = vioreanu_rokhlin_tetrahedron(qorder)
+    g = np.linspace(-half, half, ncell + 1)
+    corner = np.array([[i, j, k] for k in (0, 1) for j in (0, 1) for i in (0, 1)], dtype=float)
+    # six tetrahedra sharing the diagonal (0,0,0)-(1,1,1): paths along the axes
+    paths = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    tets = []
+    for k in range(ncell):
+        for j in range(ncell):
+            for i in range(ncell):
+                o = np.array([g[i], g[j], g[k]])
+                h = g[1] - g[0]
+                for pth in paths:
+                    v = [np.zeros(3)]
+                    for ax in pth:
+                        nv = v[-1].copy()
+                        nv[ax] += 1.0
+                        v.append(nv)
+                    V = o + h * np.array(v)
+                    J = np.stack([V[1] - V[0], V[2] - V[0], V[3] - V[0]], axis=1)
+                    if np.linalg.det(J) < 0:
+                        V[[1, 2]] = V[[2, 1]]
+                    tets.append(V)
+    T = np.array(tets)                                               # (ne, 4, 3)
+    J = np.stack([T[:, 1] - T[:, 0], T[:, 2] - T[:, 0], T[:, 3] - T[:, 0]], axis=2)  # (ne, 3, 3)
+    x = T[:, None, 0, :] + np.einsum("eij,qj->eqi", J, uvw)          # (ne, nq, 3)
+    det = np.linalg.det(J)
+    w = det[:, None] * wref[None, :]
+    ne, nq, _ = x.shape
+    vol = Quadrature(x.reshape(ne * nq, 3).T, None, w.reshape(-1),
+                     np.arange(ne * nq, dtype=np.int64).reshape(ne, nq).T, name=f"box(n={ncell},q={qorder})",
+                     qorder=qorder)
+    vol.tetrahedra = T
+    # boundary: per face (axis, side), cells (a, b) -> two triangles, outward
+    uv, wt = vioreanu_rokhlin_triangle(boundary_qorder)
+    coords, normals, weights = [], [], []
+    for ax in range(3):
+        for side in (-1.0, 1.0):
+            a1, a2 = [d for d in range(3) if d != ax]
+            for b in range(ncell):
+                for a in range(ncell):
+                    P = np.zeros((4, 3))
+                    for c, (da, db) in enumerate(((0, 0), (1, 0), (1, 1), (0, 1))):
+                        P[c, ax] = side * half
+                        P[c, a1] = g[a + da]
+                        P[c, a2] = g[b + db]
+                    nrm = np.zeros(3)
+                    nrm[ax] = side
+                    for tri in ((0, 1, 2), (0, 2, 3)):
+                        A, B, Cc = P[list(tri)]
+                        if np.dot(np.cross(B - A, Cc - A), nrm) < 0:
+                            B, Cc = Cc, B
+                        area2 = np.linalg.norm(np.cross(B - A, Cc - A))
+                        pts = A[None, :] + uv[:, 0:1] * (B - A)[None, :] + uv[:, 1:2] * (Cc - A)[None, :]
+                        coords.append(pts)
+                        normals.append(np.repeat(nrm[None, :], len(wt), axis=0))
+                        weights.append(area2 * wt)
+    C = np.concatenate(coords)
+    nb = len(C)
+    bq = len(wt)
+    bnd = Quadrature(C.T, np.concatenate(normals).T, np.concatenate(weights),
+                     np.arange(nb, dtype=np.int64).reshape(nb // bq, bq).T, name=f"box-boundary(n={ncell})",
+                     qorder=boundary_qorder)
+    return vol, bnd

@@ -48,22 +48,31 @@ def vdim_correction(op, target, source, boundary, Sop, Dop, Vop, *, green_multip
     order = interpolation_order if interpolation_order is not None else source.qorder
     Theta, ps, b = vdim_auxiliary_quantities(op, target, source, boundary, Sop, Dop, Vop, mu,
                                              order, center)
-    nq, ne = source.qtags.shape
     nb = len(ps)
     # L[n, q] = p_n(y_jglob[q]) per element: b[qtags.T] is (ne, nq, nb), i.e.
-    # each element's nb x nq matrix in column-major order
-    A = np.ascontiguousarray(b[source.qtags.T])
+    # each element's nb x nq matrix in column-major order; element types one
+    # after another (vdim.jl:76-121 loops over source.etype2qtags)
+    types = source.etype_qtags
+    A = np.ascontiguousarray(np.concatenate([b[q.T].reshape(-1) for q in types]))
+    nqs, nes, tags = source.etype_flat()
     if near is None:
         near = etype_to_nearest_points(target, source, maxdist)
-    near_ptr, near_idx = (None, None) if near is None else near
+    if near is None:
+        near_ptr = near_idx = None
+    else:
+        lists = [near] if len(types) == 1 else list(near)
+        if len(lists) != len(types):
+            raise ValueError("near: one (near_ptr, near_idx) pair per element type")
+        near_ptr = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.int64) for p, _ in lists]))
+        near_idx = np.ascontiguousarray(np.concatenate([np.asarray(i, dtype=np.int64) for _, i in lists]))
     cplx = op.k != 0
     rhs = np.ascontiguousarray(Theta.T) if cplx else np.ascontiguousarray(Theta.T.real)
     ctx = _lib.default_context()
     h = C.c_void_p()
     diag = np.zeros(2)
-    ctx.check(ctx.lib.inti_lsq_build(
-        ctx.handle, 1 if cplx else 0, nb, nq, ne, _lib.dptr(A.reshape(-1)),
-        _lib.i64ptr(np.ascontiguousarray(source.qtags.T.reshape(-1))), len(target), len(source),
+    ctx.check(ctx.lib.inti_lsq_build_types(
+        ctx.handle, 1 if cplx else 0, nb, len(types), nqs.ctypes.data_as(C.POINTER(C.c_int)), _lib.i64ptr(nes),
+        _lib.dptr(A), _lib.i64ptr(tags), len(target), len(source),
         _lib.i64ptr(near_ptr), _lib.i64ptr(near_idx), _lib.dptr(rhs.reshape(-1)), C.byref(h),
         _lib.dptr(diag)))
     if diagnostics is not None:

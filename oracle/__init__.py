@@ -52,6 +52,10 @@ def _bind(path):
     lib.ork_bdim.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int64, _dp, _dp,
                              C.c_int64, _dp, _dp, _dp, C.c_int, C.c_int64, _i64p, _i64p, _i64p,
                              _dp, C.c_double, C.c_double, _i64p, _i64p, _dp, _dp, _dp]
+    lib.ork_bdim_types.restype = C.c_int64
+    lib.ork_bdim_types.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int64, _dp, _dp,
+                                   C.c_int64, _dp, _dp, _dp, C.c_int, C.POINTER(C.c_int), _i64p, _i64p,
+                                   _i64p, _i64p, _dp, C.c_double, C.c_double, _i64p, _i64p, _dp, _dp, _dp]
     lib.ork_nearest_lists.restype = C.c_int64
     lib.ork_nearest_lists.argtypes = [C.c_int, C.c_int64, _dp, C.c_int64, _dp, C.c_int, C.c_int64,
                                       _i64p, C.c_double, _i64p, _i64p]
@@ -136,45 +140,61 @@ def monopoles(dim, src_coords, nq, oversample=3.0, radius_mult=1.5):
     return zs
 
 
+def _etypes(src):
+    return getattr(src, "etype_qtags", None) or [src.qtags]
+
+
 def bdim(pde, dim, k, trg, src, mu, derivative=False, near=None, oversample=3.0,
          radius_mult=1.5, variant=None):
-    """CSR (rowptr, colind, valS, valD) and diagnostics of bdim_correction."""
+    """CSR (rowptr, colind, valS, valD) and diagnostics of bdim_correction;
+    a source with several element types takes one near list per type."""
     m, n = trg.coords.shape[1], src.coords.shape[1]
-    nq, ne = src.qtags.shape
-    qt = np.ascontiguousarray(src.qtags.T.reshape(-1), dtype=np.int64)
+    types = _etypes(src)
+    nqs = np.array([q.shape[0] for q in types], dtype=np.int32)
+    nes = np.array([q.shape[1] for q in types], dtype=np.int64)
+    qt = np.ascontiguousarray(np.concatenate([q.T.reshape(-1) for q in types]), dtype=np.int64)
     if near is None:
-        cnt = nq * ne
+        cnt_tri = int(np.sum(nqs.astype(np.int64) ** 2 * nes))
         near_ptr = near_idx = None
     else:
-        near_ptr = np.ascontiguousarray(near[0], dtype=np.int64)
-        near_idx = np.ascontiguousarray(near[1], dtype=np.int64)
-        cnt = int(near_ptr[-1])
-    nnz_max = max(cnt * nq, 1)
+        lists = [near] if len(types) == 1 else list(near)
+        if len(lists) != len(types):
+            raise ValueError("one near list per element type")
+        near_ptr = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.int64) for p, _ in lists]))
+        near_idx = np.ascontiguousarray(np.concatenate([np.asarray(i, dtype=np.int64) for _, i in lists]))
+        cnt_tri = sum(int(p[-1]) * int(nq) for (p, _), nq in zip(lists, nqs))
+    nnz_max = max(cnt_tri, 1)
     rowptr = np.zeros(m + 1, dtype=np.int64)
     colind = np.zeros(nnz_max, dtype=np.int64)
     valS = np.zeros(nnz_max, dtype=np.complex128)
     valD = np.zeros(nnz_max, dtype=np.complex128)
     diag = np.zeros(4)
-    nnz = _load(variant).ork_bdim(pde, dim, k, 1 if derivative else 0, m, _d(_f64(trg.coords)),
-                           _d(_f64(trg.normals)), n, _d(_f64(src.coords)), _d(_f64(src.normals)),
-                           _d(_f64(src.weights)), nq, ne, _i(qt), _i(near_ptr), _i(near_idx),
-                           _d(_f64(mu)), oversample, radius_mult, _i(rowptr), _i(colind),
-                           _d(valS.view(np.float64)), _d(valD.view(np.float64)), _d(diag))
+    nnz = _load(variant).ork_bdim_types(
+        pde, dim, k, 1 if derivative else 0, m, _d(_f64(trg.coords)), _d(_f64(trg.normals)), n,
+        _d(_f64(src.coords)), _d(_f64(src.normals)), _d(_f64(src.weights)), len(types),
+        nqs.ctypes.data_as(C.POINTER(C.c_int)), _i(nes), _i(qt), _i(near_ptr), _i(near_idx),
+        _d(_f64(mu)), oversample, radius_mult, _i(rowptr), _i(colind),
+        _d(valS.view(np.float64)), _d(valD.view(np.float64)), _d(diag))
     if nnz < 0:
         raise ValueError("oracle bdim rejected its arguments")
     return rowptr, colind[:nnz], valS[:nnz], valD[:nnz], diag
 
 
 def nearest_lists(trg, src, maxdist=np.inf):
+    """Element -> target lists; one (near_ptr, near_idx) pair per element type
+    when the source has several (quadrature.jl:264-286 runs per type)."""
     m, n = trg.coords.shape[1], src.coords.shape[1]
-    nq, ne = src.qtags.shape
-    qt = np.ascontiguousarray(src.qtags.T.reshape(-1), dtype=np.int64)
-    near_ptr = np.zeros(ne + 1, dtype=np.int64)
-    near_idx = np.zeros(max(m, 1), dtype=np.int64)
-    cnt = _load().ork_nearest_lists(trg.coords.shape[0], m, _d(_f64(trg.coords)), n,
-                                    _d(_f64(src.coords)), nq, ne, _i(qt), float(maxdist),
-                                    _i(near_ptr), _i(near_idx))
-    return near_ptr, near_idx[:cnt]
+    out = []
+    for qtags in _etypes(src):
+        nq, ne = qtags.shape
+        qt = np.ascontiguousarray(qtags.T.reshape(-1), dtype=np.int64)
+        near_ptr = np.zeros(ne + 1, dtype=np.int64)
+        near_idx = np.zeros(max(m, 1), dtype=np.int64)
+        cnt = _load().ork_nearest_lists(trg.coords.shape[0], m, _d(_f64(trg.coords)), n,
+                                        _d(_f64(src.coords)), nq, ne, _i(qt), float(maxdist),
+                                        _i(near_ptr), _i(near_idx))
+        out.append((near_ptr, near_idx[:cnt]))
+    return out[0] if len(out) == 1 else out
 
 
 def csr_matvec(m, n, rowptr, colind, val, x):

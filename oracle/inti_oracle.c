@@ -305,21 +305,40 @@ static int cmp_triplet(const void *a, const void *b) {
     return 0;
 }
 
-/* bdim.jl:50-196 */
-int64_t ork_bdim(int pde, int dim, double k, int derivative,
-                 int64_t m, const double *tx, const double *tn,
-                 int64_t n, const double *sx, const double *sn, const double *sw,
-                 int nq, int64_t ne, const int64_t *qtags,
-                 const int64_t *near_ptr, const int64_t *near_idx,
-                 const double *mu, double oversample, double radius_mult,
-                 int64_t *rowptr, int64_t *colind, double *valS, double *valD,
-                 double *diag) {
-    const int self = near_ptr == NULL;
+/* bdim.jl:50-196 over element types: the monopole count uses the sum of nq
+ * over the types (bdim.jl:79-80), the least squares run type by type
+ * (bdim.jl:138-185). Type t's tags are the nq[t] x ne[t] block of qtags at
+ * sum_{u<t} nq[u] ne[u]; its near lists the ne[t] + 1 entries of near_ptr at
+ * sum_{u<t} (ne[u] + 1), indexing the block of near_idx after the earlier
+ * types' lists. */
+int64_t ork_bdim_types(int pde, int dim, double k, int derivative,
+                       int64_t m, const double *tx, const double *tn,
+                       int64_t n, const double *sx, const double *sn, const double *sw,
+                       int ntypes, const int *nqs, const int64_t *nes, const int64_t *qtags_all,
+                       const int64_t *near_ptr_all, const int64_t *near_idx_all,
+                       const double *mu, double oversample, double radius_mult,
+                       int64_t *rowptr, int64_t *colind, double *valS, double *valD,
+                       double *diag) {
+    const int self = near_ptr_all == NULL;
     if (self && m != n) return -1;
-    int ns = (int)ceil(oversample * nq);
-    if (ns > 64 || ns < 2 * nq) return -1;
+    if (ntypes < 1 || ntypes > 16) return -1;
+    int qmax = 0, nqmax = 0;
+    for (int t = 0; t < ntypes; ++t) {
+        qmax += nqs[t];
+        nqmax = nqs[t] > nqmax ? nqs[t] : nqmax;
+    }
+    int ns = (int)ceil(oversample * qmax);
+    if (ns > 64 || ns < 2 * nqmax) return -1;
+    /* per-type offsets into qtags, near_ptr, near_idx */
+    int64_t tag_off[17], ptr_off[17], idx_off[17];
+    tag_off[0] = ptr_off[0] = idx_off[0] = 0;
+    for (int t = 0; t < ntypes; ++t) {
+        tag_off[t + 1] = tag_off[t] + (int64_t)nqs[t] * nes[t];
+        ptr_off[t + 1] = ptr_off[t] + nes[t] + 1;
+        idx_off[t + 1] = idx_off[t] + (self ? 0 : near_ptr_all[ptr_off[t] + nes[t]]);
+    }
     double *zs = (double *)malloc(sizeof(double) * dim * ns);
-    ork_dim_monopoles(dim, n, sx, nq, oversample, radius_mult, zs);
+    ork_dim_monopoles(dim, n, sx, qmax, oversample, radius_mult, zs);
     const int kG = derivative ? ORK_ADL : ORK_SL; /* bdim.jl:112 */
     const int kS = derivative ? ORK_ADL : ORK_SL;
     const int kD = derivative ? ORK_HS : ORK_DL;
@@ -339,20 +358,24 @@ int64_t ork_bdim(int pde, int dim, double k, int derivative,
         }
     }
 
-    /* targets needing a correction, in element order */
-    int64_t nlisted = 0;
-    if (self) nlisted = n;
-    else nlisted = near_ptr[ne];
+    /* targets needing a correction, in type and element order */
+    int64_t nlisted = self ? tag_off[ntypes] : idx_off[ntypes];
     /* local row of each global target (-1 = not corrected) */
     int64_t *loc = (int64_t *)malloc(sizeof(int64_t) * (m > 0 ? m : 1));
     for (int64_t i = 0; i < m; ++i) loc[i] = -1;
     int64_t *glob = (int64_t *)malloc(sizeof(int64_t) * (nlisted > 0 ? nlisted : 1));
     int64_t nloc = 0;
-    for (int64_t e = 0; e < ne; ++e) {
-        int64_t cnt = self ? nq : near_ptr[e + 1] - near_ptr[e];
-        for (int64_t t = 0; t < cnt; ++t) {
-            int64_t i = self ? qtags[t + (int64_t)nq * e] : near_idx[near_ptr[e] + t];
-            if (loc[i] < 0) { loc[i] = nloc; glob[nloc++] = i; }
+    for (int ty = 0; ty < ntypes; ++ty) {
+        const int nq = nqs[ty];
+        const int64_t ne = nes[ty], *qtags = qtags_all + tag_off[ty];
+        const int64_t *near_ptr = self ? NULL : near_ptr_all + ptr_off[ty];
+        const int64_t *near_idx = self ? NULL : near_idx_all + idx_off[ty];
+        for (int64_t e = 0; e < ne; ++e) {
+            int64_t cnt = self ? nq : near_ptr[e + 1] - near_ptr[e];
+            for (int64_t t = 0; t < cnt; ++t) {
+                int64_t i = self ? qtags[t + (int64_t)nq * e] : near_idx[near_ptr[e] + t];
+                if (loc[i] < 0) { loc[i] = nloc; glob[nloc++] = i; }
+            }
         }
     }
 
@@ -385,12 +408,27 @@ int64_t ork_bdim(int pde, int dim, double k, int derivative,
         for (int kk = 0; kk < ns; ++kk) Th[l + nloc * kk] = acc[kk] + sacc[kk] - dacc[kk];
     }
 
-    /* per-element least squares (bdim.jl:137-185) */
-    const int nc = 2 * nq;
-    int64_t nnz = 0;
-    for (int64_t e = 0; e < ne; ++e) nnz += (int64_t)nq * (self ? nq : near_ptr[e + 1] - near_ptr[e]);
+    /* per-element least squares (bdim.jl:137-185), type by type; the
+     * triplets of type t start at slot tri_off[t] */
+    int64_t tri_off[17];
+    tri_off[0] = 0;
+    for (int ty = 0; ty < ntypes; ++ty) {
+        int64_t c = 0;
+        for (int64_t e = 0; e < nes[ty]; ++e)
+            c += (int64_t)nqs[ty] *
+                 (self ? nqs[ty] : near_ptr_all[ptr_off[ty] + e + 1] - near_ptr_all[ptr_off[ty] + e]);
+        tri_off[ty + 1] = tri_off[ty] + c;
+    }
+    int64_t nnz = tri_off[ntypes];
     triplet *trip = (triplet *)malloc(sizeof(triplet) * (nnz > 0 ? nnz : 1));
     double dg[4] = {0, 0, 0, 0};
+    for (int ty = 0; ty < ntypes; ++ty) {
+    const int nq = nqs[ty];
+    const int64_t ne = nes[ty], *qtags = qtags_all + tag_off[ty];
+    const int64_t *near_ptr = self ? NULL : near_ptr_all + ptr_off[ty];
+    const int64_t *near_idx = self ? NULL : near_idx_all + idx_off[ty];
+    triplet *trip_t = trip + tri_off[ty];
+    const int nc = 2 * nq;
 #pragma omp parallel
     {
         cplx *A = (cplx *)malloc(sizeof(cplx) * ns * nc);
@@ -433,7 +471,7 @@ int64_t ork_bdim(int pde, int dim, double k, int derivative,
                 if (self) slot0 = ((int64_t)nq * e + t) * nq;
                 else slot0 = (near_ptr[e] + t) * nq;
                 for (int r = 0; r < nq; ++r) {
-                    triplet *tp = trip + slot0 + r;
+                    triplet *tp = trip_t + slot0 + r;
                     tp->row = i;
                     tp->col = jg[r];
                     tp->s = -b[nq + r]; /* bdim.jl:179 */
@@ -446,6 +484,7 @@ int64_t ork_bdim(int pde, int dim, double k, int derivative,
             for (int q = 0; q < 4; ++q) dg[q] = ldg[q] > dg[q] ? ldg[q] : dg[q];
         }
         free(A); free(tau); free(b);
+    }
     }
     /* sparse(Is, Js, V, m, n) semantics, emitted as CSR (sorted, summed) */
     qsort(trip, nnz, sizeof(triplet), cmp_triplet);
@@ -469,6 +508,20 @@ int64_t ork_bdim(int pde, int dim, double k, int derivative,
     if (diag) for (int q = 0; q < 4; ++q) diag[q] = dg[q];
     free(trip); free(Th); free(glob); free(loc); free(B0); free(B1); free(zs);
     return out;
+}
+
+/* bdim.jl:50-196 with one element type */
+int64_t ork_bdim(int pde, int dim, double k, int derivative,
+                 int64_t m, const double *tx, const double *tn,
+                 int64_t n, const double *sx, const double *sn, const double *sw,
+                 int nq, int64_t ne, const int64_t *qtags,
+                 const int64_t *near_ptr, const int64_t *near_idx,
+                 const double *mu, double oversample, double radius_mult,
+                 int64_t *rowptr, int64_t *colind, double *valS, double *valD,
+                 double *diag) {
+    return ork_bdim_types(pde, dim, k, derivative, m, tx, tn, n, sx, sn, sw, 1, &nq, &ne, qtags,
+                          near_ptr, near_idx, mu, oversample, radius_mult, rowptr, colind, valS, valD,
+                          diag);
 }
 
 /* quadrature.jl:264-286 with a brute-force nearest neighbour */

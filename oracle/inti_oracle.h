@@ -91,6 +91,21 @@ int64_t ork_bdim(int pde, int dim, double k, int derivative,
                  int64_t *rowptr, int64_t *colind, double *valS, double *valD,
                  double *diag);
 
+/* ork_bdim over ntypes element types (bdim.jl:79-80, 138-185): type t has
+ * nqs[t] nodes per element and nes[t] elements; qtags_all holds the column-
+ * major nq x ne blocks one after another; near_ptr_all holds nes[t] + 1
+ * entries per type (each block starting at 0) and near_idx_all the target
+ * blocks in the same order (NULL/NULL: self pattern). At most 16 types.
+ * Outputs as ork_bdim (nnz = sum over types of nq * |near lists|). */
+int64_t ork_bdim_types(int pde, int dim, double k, int derivative,
+                       int64_t m, const double *tx, const double *tn,
+                       int64_t n, const double *sx, const double *sn, const double *sw,
+                       int ntypes, const int *nqs, const int64_t *nes, const int64_t *qtags_all,
+                       const int64_t *near_ptr_all, const int64_t *near_idx_all,
+                       const double *mu, double oversample, double radius_mult,
+                       int64_t *rowptr, int64_t *colind, double *valS, double *valD,
+                       double *diag);
+
 /* Nearest-source-node assignment (quadrature.jl:264-286) by brute force:
  * each target goes to the element owning its nearest source node (lowest
  * node index on ties), dropped if the distance exceeds maxdist. Output is

@@ -292,8 +292,13 @@ def test_gpu_pullback_error_paths(inti):
     M = inti.IntegralOperator(inti.SingleLayerKernel(op), Q, precision="mixed")
     with pytest.raises(inti.IntiError):
         M.pullback(u, u)
+    # the combined node gradient needs target is source, and excludes the parts
+    T = inti.points(Q.coords[:, :5] * 2.0)
     with pytest.raises(ValueError):
-        inti.IntegralOperator(inti.SingleLayerKernel(op), Q).pullback(u, u, wrt=("nodes",))
+        inti.IntegralOperator(inti.SingleLayerKernel(op), T, Q).pullback(np.ones(5, dtype=complex), u,
+                                                                           wrt=("nodes",))
+    with pytest.raises(ValueError):
+        inti.IntegralOperator(inti.SingleLayerKernel(op), Q).pullback(u, u, wrt=("nodes", "source_nodes"))
 
 
 SYM_CASES = [(LAP, 2, SL), (LAP, 2, DL), (LAP, 3, SL), (LAP, 3, DL), (LAP, 3, COMB),
@@ -339,6 +344,18 @@ def test_gpu_symmetric_pullback(inti, oracle, pde, dim, kind, n):
     if pde == HEL:
         assert abs(g["k"] - o["kbar"]) <= 1e-11 * abs(o["kbar"])
         assert abs(g["k"] - h["k"]) <= 1e-12 * abs(o["kbar"])
+    # the node gradient in one output (source and target parts summed in the
+    # pass): symmetric and two-pass, against the sum of the oracle's parts
+    wrt_c = ("sigma", "nodes", "source_normals", "source_weights") + (("k",) if pde == HEL else ())
+    gc = A.pullback(u, s, wrt=wrt_c, dcoefs=dco)
+    hc = B.pullback(u, s, wrt=wrt_c, dcoefs=dco)
+    ref = o["ybar"] + o["xbar"]
+    nr = np.linalg.norm(ref)
+    assert np.linalg.norm(gc["nodes"] - ref) <= 1e-12 * nr
+    assert np.linalg.norm(hc["nodes"] - ref) <= 1e-12 * nr
+    for a in ("sigma", "source_normals", "source_weights"):
+        assert np.linalg.norm(gc[a] - g[a]) <= 1e-13 * np.linalg.norm(g[a]), a
+    assert np.array_equal(A.pullback(u, s, wrt=wrt_c, dcoefs=dco)["nodes"], gc["nodes"])
 
 
 @pytest.mark.gpu
@@ -418,3 +435,40 @@ def test_gpu_symmetric_pullback_automatic_on_a_large_operator(inti):
     for a in wrt[:-1]:
         assert np.linalg.norm(g[a] - h[a]) <= 1e-12 * np.linalg.norm(h[a]), a
     assert abs(g["k"] - h["k"]) <= 1e-12 * abs(h["k"])
+
+
+@pytest.mark.gpu
+def test_gpu_shape_gradient_through_the_corrected_solve(inti):
+    """The adjoint-state gradient of J = sum |u_inf|^2 through the corrected
+    CFIE solve (bench.shape_gradient: sound-soft scattering of e^{ikz}, k = 4,
+    perturbed sphere at icosphere frequency 8, 7,680 nodes, 16 directions)
+    against central differences (h = 1e-6) of the whole solve along 8 seeded
+    directions in nodes, normals, weights and k: with the DIM correction
+    matrices frozen at the base geometry (what the adjoint differentiates)
+    to 1e-5 of the largest directional derivative of the same kind (a random
+    direction in the nodes can give a derivative far smaller than the
+    gradient's size, and the FD error, h^2 times the third derivative plus
+    the solver's noise over h, does not shrink with it). With the correction
+    rebuilt on every perturbed geometry (the full solve) the difference is
+    the omitted derivative of the correction: reported, not bounded (it is
+    of the size of the gradient in the node, normal and weight directions; k
+    does not enter the correction's geometry)."""
+    import bench
+    print("shape gradient through the solve, 7,680 nodes", flush=True)
+    Q = inti.sphere_quadrature(8, radial=lambda th, ph: 1.0 + 0.1 * np.sin(3 * th) * np.cos(2 * ph))
+    k = 4.0
+    dirs = bench.fibonacci_directions(16)
+    J, grad, info = bench.shape_gradient(inti, Q.coords, Q.normals, Q.weights, Q.qtags, k, dirs)
+    print(f"  J {J:.6e}, solve {info['solve']}, adjoint {info['adjoint_solve']}", flush=True)
+    assert info["adjoint_solve"]["relres"] <= 1e-11
+    checks = bench.shape_fd_check(inti, Q.coords, Q.normals, Q.weights, Q.qtags, k, dirs, grad,
+                                  info["corrections"], 8, 7, sigma0=info["sigma"])
+    scale = {}
+    for w, _, fz, _ in checks:
+        scale[w] = max(scale.get(w, 0.0), abs(fz))
+    worst = 0.0
+    for w, a, fz, fl in checks:
+        ez, el = abs(a - fz) / scale[w], abs(a - fl) / abs(fl)
+        print(f"  {w:8s} adjoint {a:+.9e} fd(frozen) {fz:+.9e} fd(full) {fl:+.9e}  {ez:.1e} {el:.1e}", flush=True)
+        worst = max(worst, ez)
+    assert worst <= 1e-5

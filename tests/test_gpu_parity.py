@@ -300,3 +300,87 @@ def test_error_paths(inti):
         H.attach(dD)  # real correction on a complex operator
     with pytest.raises(inti.IntiError):
         A.attach(dD)  # shape mismatch
+
+
+def _mixed(inti, dim):
+    """Two element types of different nq: an ellipse with Gauss rules of 2 and
+    5 nodes on its two arcs; a sphere with 3- and 6-node triangle rules."""
+    if dim == 2:
+        E = inti.ellipse_quadrature(1.0, 0.6, 16).elements
+        return inti.join_quadratures(inti.curve_quadrature(E[:6], 3), inti.curve_quadrature(E[6:], 8))
+    E = inti.sphere_quadrature(2, base="ico").elements
+    return inti.join_quadratures(inti.surface_quadrature(E[:30], 2, center=(0, 0, 0)),
+                                 inti.surface_quadrature(E[30:], 4, center=(0, 0, 0)))
+
+
+@pytest.mark.parametrize("pde,dim", [(LAP, 2), (HEL, 2), (LAP, 3), (HEL, 3)])
+def test_bdim_element_types_match_oracle(inti, oracle, pde, dim):
+    """Several element types (bdim.jl:79-80, 138-185): pattern bit-exact,
+    weights within 10x the oracle's own FMA/no-FMA spread, the corrected
+    operators applied to a smooth density within 1e-12 (self pattern), and
+    off-surface targets with one near list per type."""
+    say(f"[gpu] bdim element types pde={pde} dim={dim}")
+    Q = _mixed(inti, dim)
+    k = 0.0 if pde == LAP else 2.5
+    op = inti.Laplace(dim=dim) if pde == LAP else inti.Helmholtz(k=k, dim=dim)
+    Sop = inti.IntegralOperator(inti.SingleLayerKernel(op), Q, Q)
+    Dop = inti.IntegralOperator(inti.DoubleLayerKernel(op), Q, Q)
+    mu = np.full(len(Q), -0.5)
+    dS, dD = inti.bdim_correction(op, Q, Q, Sop, Dop, green_multiplier=mu)
+    rp, ci, vS = dS.export()
+    _, _, vD = dD.export()
+    orp, oci, ovS, ovD, _ = oracle.bdim(pde, dim, k, Q, Q, mu)
+    _, _, fvS, fvD, _ = oracle.bdim(pde, dim, k, Q, Q, mu, variant="fma")
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+    if pde == LAP:
+        ovS, ovD, fvS, fvD = ovS.real, ovD.real, fvS.real, fvD.real
+    for v, o, f in ((vS, ovS, fvS), (vD, ovD, fvD)):
+        assert np.abs(v - o).max() <= 10 * np.abs(f - o).max() + 1e-14 * np.abs(o).max()
+    n = len(Q)
+    x = _smooth_density(Q, pde)
+    Sop.attach(dS)
+    Dop.attach(dD)
+    for A, kk, ov, fv in ((Dop, 1, ovD, fvD), (Sop, 0, ovS, fvS)):
+        y = A @ x
+        y0 = oracle.matvec(pde, dim, kk, k, Q, Q, x)
+        ref = y0 + oracle.csr_matvec(n, n, orp, oci, ov, x)
+        alt = y0 + oracle.csr_matvec(n, n, orp, oci, fv, x)
+        spread = np.linalg.norm(alt - ref) / np.linalg.norm(ref)
+        assert np.linalg.norm(y - ref) / np.linalg.norm(ref) <= max(1e-12, 10 * spread)
+    # targets off the surface, near lists per type
+    T = inti.points(Q.coords[:, ::3] * 1.03)
+    m = T.coords.shape[1]
+    near = oracle.nearest_lists(T, Q)
+    mu0 = np.zeros(m)
+    S2 = inti.IntegralOperator(inti.SingleLayerKernel(op), T, Q)
+    D2 = inti.IntegralOperator(inti.DoubleLayerKernel(op), T, Q)
+    eS, eD = inti.bdim_correction(op, T, Q, S2, D2, green_multiplier=mu0, near=near)
+    orp, oci, ovS, ovD, _ = oracle.bdim(pde, dim, k, T, Q, mu0, near=near)
+    rp, ci, wD = eD.export()
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+    D2.attach(eD)
+    y = D2 @ x
+    y0 = oracle.matvec(pde, dim, 1, k, T, Q, x)
+    yc = oracle.csr_matvec(m, n, orp, oci, ovD if pde == HEL else ovD.real, x)
+    scale = np.linalg.norm(y0) + np.linalg.norm(yc)
+    assert np.linalg.norm(y - (y0 + yc)) / scale <= 1e-12
+
+
+def test_bdim_element_types_split_is_one_type(inti):
+    """Elements cut into two types of the same nq with the oversample factor
+    halved: the same monopoles and least-squares problems, so the device
+    correction is bit-identical to the one-type build."""
+    say("[gpu] bdim split types")
+    E = inti.ellipse_quadrature(1.0, 0.6, 20).elements
+    one = inti.curve_quadrature(E, 5)
+    two = inti.join_quadratures(inti.curve_quadrature(E[:7], 5), inti.curve_quadrature(E[7:], 5))
+    op = inti.Helmholtz(k=2.0, dim=2)
+    out = []
+    for Q, ov in ((one, 3.0), (two, 1.5)):
+        S = inti.IntegralOperator(inti.SingleLayerKernel(op), Q, Q)
+        D = inti.IntegralOperator(inti.DoubleLayerKernel(op), Q, Q)
+        dS, dD = inti.bdim_correction(op, Q, Q, S, D, green_multiplier=np.full(len(Q), -0.5),
+                                      parameters=inti.DimParameters(sources_oversample_factor=ov))
+        out.append(dS.export() + dD.export())
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)

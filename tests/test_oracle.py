@@ -247,3 +247,78 @@ def test_dim_offsurface_targets(oracle, inti):
         v = oracle.matvec(LAP, 2, DL, 0.0, T, Q, ones) + oracle.csr_matvec(len(idx), len(Q.weights), rowptr, colind, vD, ones)
         target = -1.0 if inside else 0.0
         assert np.abs(v - target).max() < 1e-3
+
+
+def _mixed_curve(inti, split=6, q=(3, 8), nel=16):
+    """An ellipse whose first `split` P2 elements carry one Gauss rule and the
+    rest another: two element types with different nq."""
+    E = inti.ellipse_quadrature(1.0, 0.6, nel).elements
+    return inti.join_quadratures(inti.curve_quadrature(E[:split], q[0]), inti.curve_quadrature(E[split:], q[1]))
+
+
+def _mixed_sphere(inti, split=30, q=(2, 4), freq=2):
+    E = inti.sphere_quadrature(freq, base="ico").elements
+    return inti.join_quadratures(inti.surface_quadrature(E[:split], q[0], center=(0, 0, 0)),
+                                 inti.surface_quadrature(E[split:], q[1], center=(0, 0, 0)))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_bdim_types_split_reproduces_one_type(oracle, inti, dim):
+    """bdim.jl:79-80 counts monopoles over the sum of nq of the types. Cutting
+    a one-type quadrature's elements into two types of the same nq, with the
+    oversample factor halved, gives the same monopoles and the same
+    per-element problems, so the correction must be bit-identical; a wrong
+    per-type offset into the tags, near lists or triplets breaks it."""
+    if dim == 2:
+        E = inti.ellipse_quadrature(1.0, 0.6, 20).elements
+        one = inti.curve_quadrature(E, 5)
+        two = inti.join_quadratures(inti.curve_quadrature(E[:7], 5), inti.curve_quadrature(E[7:], 5))
+        pde, k = HEL, 2.0
+    else:
+        E = inti.sphere_quadrature(1, base="ico").elements
+        one = inti.surface_quadrature(E, 2, center=(0, 0, 0))
+        two = inti.join_quadratures(inti.surface_quadrature(E[:9], 2, center=(0, 0, 0)),
+                                    inti.surface_quadrature(E[9:], 2, center=(0, 0, 0)))
+        pde, k = LAP, 0.0
+    assert np.array_equal(one.coords, two.coords) and len(two.etype_qtags) == 2
+    mu = np.full(len(one), -0.5)
+    a = oracle.bdim(pde, dim, k, one, one, mu, oversample=3.0)
+    b = oracle.bdim(pde, dim, k, two, two, mu, oversample=1.5)
+    for x, y in zip(a[:4], b[:4]):
+        assert np.array_equal(x, y)
+    # off-surface targets with one near list per type
+    T = inti.points(one.coords[:, ::3] * 1.02)
+    mu = np.zeros(T.coords.shape[1])
+    near1, near2 = oracle.nearest_lists(T, one), oracle.nearest_lists(T, two)
+    assert len(near2) == 2
+    a = oracle.bdim(pde, dim, k, T, one, mu, near=near1, oversample=3.0)
+    b = oracle.bdim(pde, dim, k, T, two, mu, near=near2, oversample=1.5)
+    for x, y in zip(a[:4], b[:4]):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_bdim_types_green_identity(oracle, inti, dim):
+    """Green's identity (green_identities_test.jl:93-126 bar: e1 < 1e-2 and
+    e1 < e0) on a boundary with two element types of different nq."""
+    Q = _mixed_curve(inti) if dim == 2 else _mixed_sphere(inti)
+    assert len({q.shape[0] for q in Q.etype_qtags}) == 2
+    xs = (3.0, 3.0) if dim == 2 else (3.0, 3.0, 3.0)
+    for pde, k in ((LAP, 0.0), (HEL, 1.2)):
+        e0, e1 = _green_errors(oracle, pde, dim, k, Q, xs, 0.5)
+        assert e1 < 1e-2 and e1 < e0 / 10
+
+
+def test_near_lists_per_type_match_product(oracle, inti):
+    """The product's etype_to_nearest_points and the oracle's brute force give
+    the same per-type lists (targets placed away from nearest-node ties)."""
+    Q = _mixed_curve(inti)
+    rng = np.random.default_rng(4)
+    th = rng.uniform(0, 2 * np.pi, 60)
+    T = inti.points(np.stack([1.1 * np.cos(th), 0.7 * np.sin(th)]))
+    a = inti.etype_to_nearest_points(T, Q)
+    b = oracle.nearest_lists(T, Q)
+    assert len(a) == len(b) == 2
+    for (p, i), (q, j) in zip(a, b):
+        assert np.array_equal(p, q) and np.array_equal(i, j)
+    assert sum(len(i) for _, i in a) == 60

@@ -68,11 +68,78 @@ def test_particular_solutions_solve_the_pde_3d(inti, pde, k):
         np.testing.assert_allclose(lhs, P.evaluate(p, x), atol=1e-6)
 
 
-def _manufactured(inti, ov, nrings, pde, k, order=3):
+@pytest.mark.parametrize("pde,k", [(LAP, 0.0), (HEL, 1.3)])
+def test_particular_solutions_match_oracle_3d(inti, ov, pde, k):
+    """3D: the oracle's P (linear solve on coefficient arrays) solves the PDE
+    exactly on the coefficients, and equals the closed form: for Helmholtz
+    the polynomial solution is unique; for Laplace both are the solution of
+    the form |x|^2 q."""
+    P = inti.polynomials
+    x = np.random.default_rng(4).normal(size=(3, 9))
+    ps, Ps, idx = P.polynomial_solutions_vdim(_op(inti, pde, k, 3), 3)
+    assert [tuple(I) for I in idx] == ov.multiindices(3, 3)
+    for I, Pp in zip(idx, Ps):
+        c = ov.particular_solution(tuple(I), pde, k, 6)
+        p = np.zeros_like(c)
+        p[tuple(I)] = 1.0 / math.prod(math.factorial(i) for i in I)
+        np.testing.assert_allclose(ov._lap(c) + k * k * c, p, atol=1e-13)
+        np.testing.assert_allclose(P.evaluate(Pp, x), ov._eval(c, x), rtol=1e-12, atol=1e-13)
+
+
+def _manufactured_3d(inti, ov, ncell, pde, k, qorder, order):
+    """The cube [-1/2, 1/2]^3 (box_quadratures): (corrected V f, Green
+    reference, uncorrected V f, nodes with |x|_inf < 0.3) for u = cos(k0 th.x),
+    f = (k^2 - k0^2) u, all from the oracle."""
+    from oracle import csr_matvec, matvec
+    Vq, Bq = inti.box_quadratures(ncell, qorder=qorder, boundary_qorder=4)
+    m = len(Vq)
+    mu = np.full(m, -1.0)
+    k0, th = np.pi, np.array([1.0, 2.0, 2.0]) / 3
+    u = lambda x: np.cos(k0 * (th @ x))
+    f = (k * k - k0 * k0) * u(Vq.coords)
+    S, D = ov.boundary_operators(pde, k, Vq, Bq, mu)
+    rp, ci, val = ov.vdim(pde, k, Vq, Vq, Bq, mu, order, S, D)
+    V0 = matvec(pde, 3, 0, k, Vq, Vq, f)
+    Vf = V0 + csr_matvec(m, m, rp, ci, val, f)
+    du = -k0 * (th @ Bq.normals) * np.sin(k0 * (th @ Bq.coords))
+    vref = -u(Vq.coords) - D(u(Bq.coords)) + S(du)
+    inner = np.abs(Vq.coords).max(axis=0) < 0.3
+    return Vf, vref, V0, inner
+
+
+@pytest.mark.parametrize("pde,k", [(LAP, 0.0), (HEL, 2.0)])
+@pytest.mark.parametrize("qorder,order", [(3, 2), (5, 3)])
+def test_oracle_vdim_3d_greens_representation(inti, ov, pde, k, qorder, order):
+    """3D VDIM on the tetrahedral cube (27 cells x 6 tetrahedra, ten or twenty
+    Vioreanu-Rokhlin nodes each, interpolation order 2 or 3): at the nodes of
+    the inner region |x|_inf < 0.3 the corrected volume potential meets
+    Green's representation of the manufactured solution 20x better than the
+    plain quadrature (the cube's edges and corners limit the boundary
+    integrals elsewhere)."""
+    Vf, vref, V0, inner = _manufactured_3d(inti, ov, 3, pde, k, qorder, order)
+    ec, ep = np.abs(Vf - vref)[inner].max(), np.abs(V0 - vref)[inner].max()
+    assert ec < 5e-4 and ep > 20 * ec, (ec, ep)
+
+
+def _mixed_disk(inti, nrings, qorders=(4, 5), split=None):
+    """The disk of disk_quadratures(nrings) with its first `split` triangles
+    (default: half) on one Vioreanu-Rokhlin rule and the rest on another: a
+    volume quadrature with two element types."""
+    Vq, Bq = inti.disk_quadratures(nrings, qorder=5, boundary_qorder=10)
+    E = Vq.elements
+    s = len(E) // 2 if split is None else split
+    V2 = inti.join_quadratures(inti.area_quadrature(E[:s], qorders[0]), inti.area_quadrature(E[s:], qorders[1]))
+    return V2, Bq
+
+
+def _manufactured(inti, ov, nrings, pde, k, order=3, mixed=False):
     """(corrected V f, Green-representation reference, uncorrected V f) at
     the volume nodes for u = cos(k0 theta.x), f = (k^2 - k0^2) u."""
     from oracle import csr_matvec, matvec
-    Vq, Bq = inti.disk_quadratures(nrings, qorder=5, boundary_qorder=10)
+    if mixed:
+        Vq, Bq = _mixed_disk(inti, nrings)
+    else:
+        Vq, Bq = inti.disk_quadratures(nrings, qorder=5, boundary_qorder=10)
     m = len(Vq)
     mu = np.full(m, -1.0)
     k0, th = np.pi, np.array([np.cos(np.pi / 3), np.sin(np.pi / 3)])
@@ -99,6 +166,20 @@ def test_oracle_vdim_converges_to_greens_representation(inti, ov, pde, k):
         assert np.abs(V0 - vref).max() > 1e3 * e[-1]
     assert e[1] < 5e-6
     assert e[0] / e[1] > 2 ** 5
+
+
+@pytest.mark.parametrize("pde,k", [(LAP, 0.0), (HEL, 2.0)])
+def test_oracle_vdim_element_types(inti, ov, pde, k):
+    """A disk meshed with two element types (6- and 10-node triangle rules,
+    vdim.jl:76-121 loops over the types): the corrected volume potential
+    meets Green's representation of the manufactured solution far better than
+    the plain quadrature, and converges under refinement."""
+    e = []
+    for nr in (2, 4):
+        Vf, vref, V0 = _manufactured(inti, ov, nr, pde, k, mixed=True)
+        e.append(np.abs(Vf - vref).max())
+        assert np.abs(V0 - vref).max() > 1e2 * e[-1]
+    assert e[1] < 1e-4 and e[0] / e[1] > 2 ** 4, e
 
 
 # ---------------------------------------------------------------- GPU ----
@@ -155,6 +236,59 @@ def test_vdim_correction_matches_oracle(inti, ov, pde, k, case, order):
     g = G.data.astype(np.complex128)
     spread = np.abs(F.data - O.data).max()
     assert np.abs(g - O.data).max() <= 10 * spread + 1e-12 * np.abs(O.data).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pde,k", [(LAP, 0.0), (HEL, 2.0)])
+@pytest.mark.parametrize("qorder,order", [(3, 2), (5, 3)])
+def test_vdim_correction_3d_matches_oracle(inti, ov, pde, k, qorder, order):
+    """3D VDIM on the tetrahedral cube (volume targets, interpolation order 2
+    on ten-node or 3 on twenty-node tetrahedra: square Vandermonde systems):
+    CSR pattern bit-exact; weights within 10x the oracle's own rounding spread
+    (default vs FMA-contracted boundary correction) plus 1e-12 of their size;
+    the corrected volume potential on the manufactured solution as accurate
+    as the oracle's in the inner region. Helmholtz values are fixed by the
+    mathematics (the polynomial particular solution is unique); Laplace ones
+    follow the |x|^2 q particular solution (the reference's choice of P is
+    not pinned)."""
+    from oracle import csr_matvec, matvec
+    print(f"vdim 3D parity pde={pde} qorder={qorder} order={order}", flush=True)
+    Vq, Bq = inti.box_quadratures(3, qorder=qorder, boundary_qorder=4)
+    op = _op(inti, pde, k, 3)
+    m = len(Vq)
+    mu = np.full(m, -1.0)
+    nearB = ov.nearest_lists(Vq, Bq)  # the oracle's tie-break on both sides
+    Sop = inti.IntegralOperator(inti.SingleLayerKernel(op), Vq, Bq)
+    Dop = inti.IntegralOperator(inti.DoubleLayerKernel(op), Vq, Bq)
+    dS, dD = inti.bdim_correction(op, Vq, Bq, Sop, Dop, green_multiplier=mu, near=nearB)
+    Sop.attach(dS)
+    Dop.attach(dD)
+    Vop = inti.IntegralOperator(inti.SingleLayerKernel(op), Vq, Vq)
+    dV = inti.vdim_correction(op, Vq, Vq, Bq, Sop, Dop, Vop, green_multiplier=mu,
+                              interpolation_order=order)
+    G = _csr(dV)
+    res = []
+    for variant in (None, "fma"):
+        oS, oD = ov.boundary_operators(pde, k, Vq, Bq, mu, variant=variant, near=nearB)
+        rp, ci, val = ov.vdim(pde, k, Vq, Vq, Bq, mu, order, oS, oD)
+        res.append(_oracle_csr(m, m, rp, ci, val))
+    O, F = res
+    assert np.array_equal(G.indptr, O.indptr) and np.array_equal(G.indices, O.indices)
+    g = G.data.astype(np.complex128)
+    spread = np.abs(F.data - O.data).max()
+    err = np.abs(g - O.data).max()
+    print(f"  weights: GPU vs oracle {err:.2e}, oracle spread {spread:.2e}, size {np.abs(O.data).max():.2e}",
+          flush=True)
+    assert err <= 10 * spread + 1e-12 * np.abs(O.data).max()
+    # the corrected volume potential of the manufactured solution
+    Vf_o, vref, V0, inner = _manufactured_3d(inti, ov, 3, pde, k, qorder, order)
+    k0, th = np.pi, np.array([1.0, 2.0, 2.0]) / 3
+    f = (k * k - k0 * k0) * np.cos(k0 * (th @ Vq.coords))
+    Vop.attach(dV)
+    Vf = Vop @ f
+    e_gpu, e_or = np.abs(Vf - vref)[inner].max(), np.abs(Vf_o - vref)[inner].max()
+    print(f"  inner error: GPU {e_gpu:.2e}, oracle {e_or:.2e}", flush=True)
+    assert e_gpu <= 2 * e_or + 1e-10
 
 
 @pytest.mark.gpu
@@ -223,3 +357,60 @@ def test_poisson_example(inti):
     print(f"poisson: nodes {len(Vq)}, gmres its {info['iterations']}, norm(er) {np.linalg.norm(er):.3e}, "
           f"max {er.max():.3e}", flush=True)
     assert np.linalg.norm(er) < 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pde,k", [(LAP, 0.0), (HEL, 2.0)])
+@pytest.mark.parametrize("order", [2, 3])
+def test_vdim_element_types_match_oracle(inti, ov, pde, k, order):
+    """Two volume element types of 6 and 10 nodes (order 2: a square and a
+    wide system; order 3: a tall and a square one), one least-squares launch
+    per type: CSR pattern bit-exact, weights within 10x the oracle's own
+    rounding spread plus 1e-12 of their size."""
+    print(f"vdim element types pde={pde} order={order}", flush=True)
+    Vq, Bq = _mixed_disk(inti, 3)
+    assert [q.shape[0] for q in Vq.etype_qtags] == [6, 10]
+    op = _op(inti, pde, k)
+    m = len(Vq)
+    mu = np.full(m, -1.0)
+    nearB = ov.nearest_lists(Vq, Bq)
+    Sop = inti.IntegralOperator(inti.SingleLayerKernel(op), Vq, Bq)
+    Dop = inti.IntegralOperator(inti.DoubleLayerKernel(op), Vq, Bq)
+    dS, dD = inti.bdim_correction(op, Vq, Bq, Sop, Dop, green_multiplier=mu, near=nearB)
+    Sop.attach(dS)
+    Dop.attach(dD)
+    Vop = inti.IntegralOperator(inti.SingleLayerKernel(op), Vq, Vq)
+    dV = inti.vdim_correction(op, Vq, Vq, Bq, Sop, Dop, Vop, green_multiplier=mu, interpolation_order=order)
+    G = _csr(dV)
+    res = []
+    for variant in (None, "fma"):
+        oS, oD = ov.boundary_operators(pde, k, Vq, Bq, mu, variant=variant, near=nearB)
+        rp, ci, val = ov.vdim(pde, k, Vq, Vq, Bq, mu, order, oS, oD)
+        res.append(_oracle_csr(m, m, rp, ci, val))
+    O, F = res
+    assert np.array_equal(G.indptr, O.indptr) and np.array_equal(G.indices, O.indices)
+    g = G.data.astype(np.complex128)
+    spread = np.abs(F.data - O.data).max()
+    assert np.abs(g - O.data).max() <= 10 * spread + 1e-12 * np.abs(O.data).max()
+    # targets off the volume nodes: the boundary nodes, one near list per type
+    near = ov.nearest_lists(Bq, Vq, 0.4)
+    assert len(near) == 2
+    muB = np.full(len(Bq), -0.5)
+    S2 = inti.IntegralOperator(inti.SingleLayerKernel(op), Bq, Bq)
+    D2 = inti.IntegralOperator(inti.DoubleLayerKernel(op), Bq, Bq)
+    eS, eD = inti.bdim_correction(op, Bq, Bq, S2, D2, green_multiplier=muB)
+    S2.attach(eS)
+    D2.attach(eD)
+    V2 = inti.IntegralOperator(inti.SingleLayerKernel(op), Bq, Vq)
+    dV2 = inti.vdim_correction(op, Bq, Vq, Bq, S2, D2, V2, green_multiplier=muB, interpolation_order=order,
+                               near=near)
+    G2 = _csr(dV2)
+    res = []
+    for variant in (None, "fma"):
+        oS, oD = ov.boundary_operators(pde, k, Bq, Bq, muB, self_pattern=True, variant=variant)
+        rp, ci, val = ov.vdim(pde, k, Bq, Vq, Bq, muB, order, oS, oD, near=near)
+        res.append(_oracle_csr(len(Bq), m, rp, ci, val))
+    O2, F2 = res
+    assert np.array_equal(G2.indptr, O2.indptr) and np.array_equal(G2.indices, O2.indices)
+    spread = np.abs(F2.data - O2.data).max()
+    assert np.abs(G2.data - O2.data).max() <= 10 * spread + 1e-12 * np.abs(O2.data).max()
