@@ -347,6 +347,22 @@ int inti_refine_solve(inti_op *op_hi, inti_op *op_lo, int on_device,
                       int maxiter_inner, double inner_rtol, int max_outer,
                       double rtol, int use_graph, int *outer,
                       int *inner_total, double *relres);
+/* inti_gmres and inti_refine_solve with a right preconditioner prec (a
+ * square operator of the same size and value type, with its own correction
+ * and identity term; NULL: none): GMRES runs on A P y = r and updates
+ * x += P y, so the residual it minimises is that of A itself (the
+ * preconditioned solve of IterativeSolvers' gmres(A, b; Pr = P) that user
+ * code on the reference would write, e.g. the Calderon preconditioner of a
+ * first-kind single-layer equation). Under refinement P is applied in the
+ * precision it was created with. */
+int inti_gmres_prec(inti_op *op, inti_op *prec, int on_device, const double *b,
+                    double *x, int restart, int maxiter, double rtol,
+                    double atol, int use_graph, int *iters, double *relres);
+int inti_refine_solve_prec(inti_op *op_hi, inti_op *op_lo, inti_op *prec,
+                           int on_device, const double *b, double *x,
+                           int restart, int maxiter_inner, double inner_rtol,
+                           int max_outer, double rtol, int use_graph,
+                           int *outer, int *inner_total, double *relres);
 
 /* --------------------------------------------------------------------- CSR */
 int inti_csr_info(const inti_csr *csr, int64_t *m, int64_t *n, int64_t *nnz,

@@ -87,6 +87,11 @@ SIGNATURES = {
                                        C.c_int64, C.c_int64, _i64p, _i64p, _dp, C.POINTER(_vp), _dp]),
     "inti_gmres": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_double, C.c_double,
                              C.c_int, C.POINTER(C.c_int), _dp]),
+    "inti_gmres_prec": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_double, C.c_double,
+                                  C.c_int, C.POINTER(C.c_int), _dp]),
+    "inti_refine_solve_prec": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_double,
+                                         C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int), _dp]),
     "inti_refine_solve": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_double,
                                     C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int),
                                     C.POINTER(C.c_int), _dp]),

@@ -124,11 +124,32 @@ struct AdjRadial {
 };
 // MASK = false: the pair is known to be farther apart than the coincidence
 // tolerance (distinct nodes, different nodes): no clamp, no mask.
-template <int PDE, int DIM, int KIND, bool MASK = true>
+// CHIK: the combined field of the scattering examples, c0 = -i k, c1 = 1,
+// dc0/dk = -i, dc1/dk = 0 (3D Helmholtz), with the coefficient products
+// written out instead of taken as general complex products.
+template <int PDE, int DIM, int KIND, bool MASK = true, bool CHIK = false>
 __device__ __forceinline__ AdjRadial adj_radial(double d2, const AdjParams &p) {
     constexpr bool WG = KIND != INTI_DL, WH = KIND != INTI_SL;
     AdjRadial R;
     R.ok = !MASK || d2 > kSamePointTol2;
+    if constexpr (CHIK) {
+        static_assert(PDE == INTI_HELMHOLTZ && DIM == 3 && KIND == INTI_COMB_SD, "CHIK is the 3D combined field");
+        const double d2s = MASK ? fmax(d2, kClampD2) : d2;
+        const double id = rsq64(d2s), d = d2s * id, id2 = id * id, id3 = id2 * id;
+        double s, c;
+        sincos_rev<false>(p.kappa * d, s, c);
+        const double2 E = c2(kInv4Pi * c, kInv4Pi * s);
+        const double k = p.k, kd = k * d, kid = k * id;
+        const double2 E0 = c2(k * E.y, -k * E.x);  // -i k E
+        R.g = cs(E0, id);
+        R.gpd = cs(c2(-E0.x - kd * E0.y, kd * E0.x - E0.y), id3);
+        R.gk = c2(E.y * id - E0.y, E0.x - E.x * id);  // dc0 E / d + c0 i E, dc0 E = -i E
+        R.h = cs(c2(E.x + kd * E.y, E.y - kd * E.x), id3);
+        const double A = kd * kd - 3.0, B = 3.0 * kd;
+        R.hpd = cs(c2(E.x * A - E.y * B, E.x * B + E.y * A), id3 * id2);
+        R.hk = cs(E, kid);  // c1 E k / d
+        return R;
+    }
     if constexpr (PDE == INTI_HELMHOLTZ && DIM == 3 && KIND == INTI_COMB_SD) {
         // every radial function is E = e^{ikd}/(4 pi) times a simple factor:
         // fold the coefficients into E once (E0 = c0 E, E1 = c1 E, D0 = dc0 E,
@@ -463,6 +484,7 @@ struct AdjSymArgs {
     int64_t ntiles;
     int distinct;   // no two nodes within twice the coincidence tolerance
     int comb;       // one node gradient, source and target parts summed
+    int chik;       // 3D combined field with c0 = -i k, c1 = 1, dc0 = -i, dc1 = 0
     AdjParams p;
 };
 
@@ -485,21 +507,19 @@ __device__ __forceinline__ void adj_sym_node(const AdjSymArgs &A, int64_t j, dou
 }
 
 // one ordered pair (target t, source s): the source's term of S0 (c_t K),
-// the node-gradient term g = alpha v + beta n_s (Y_s += g, X_t -= g) and
-// gamma (N_s += gamma v); returns Re(sc gk)
+// the factors of the node-gradient term g = al v + be n_s (Y_s += g,
+// X_t -= g) and ga (N_s += ga v); returns Re(sc gk)
 template <int PDE, int DIM, int KIND, bool MASK>
 __device__ __forceinline__ double adj_sym_terms(const double *v, const double *ns, const AdjRadial &R,
                                                 const AdjParams &p, double2 ct, double2 ss, double2 &cK,
-                                                double *g, double &ga) {
+                                                double &al, double &be, double &ga) {
     double2 K, gk, a_r, a_n, b_r;
     adj_combine<PDE, DIM, KIND, MASK>(v, ns, R, p, K, gk, a_r, a_n, b_r);
     const double2 sc = cm(ss, ct);
     cK = make_double2(__builtin_fma(ct.x, K.x, -ct.y * K.y), __builtin_fma(ct.x, K.y, ct.y * K.x));
-    const double al = sc.x * a_r.x - sc.y * a_r.y;
-    const double be = sc.x * a_n.x - sc.y * a_n.y;
+    al = sc.x * a_r.x - sc.y * a_r.y;
+    be = sc.x * a_n.x - sc.y * a_n.y;
     ga = sc.x * b_r.x - sc.y * b_r.y;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) g[d] = __builtin_fma(al, v[d], be * ns[d]);
     return sc.x * gk.x - sc.y * gk.y;
 }
 
@@ -513,7 +533,7 @@ __device__ __forceinline__ void lds_add(double *a, double v) {
 // asked for by passing the same output for both): NR = 2 DIM + 2 sums
 // [S0 (2), G (DIM), N (DIM)] instead of 3 DIM + 2, and each pair adds its
 // difference g1 - g2 once to each node.
-template <int PDE, int DIM, int KIND, bool COMB>
+template <int PDE, int DIM, int KIND, bool COMB, bool CHIK = false>
 __global__ __launch_bounds__(kSymBS) __attribute__((amdgpu_waves_per_eu(COMB ? 3 : 1))) void k_adj_sym(AdjSymArgs A) {
     constexpr int NR = COMB ? 2 * DIM + 2 : 3 * DIM + 2;
     constexpr int NW = kSymBS / 64;
@@ -566,11 +586,11 @@ __global__ __launch_bounds__(kSymBS) __attribute__((amdgpu_waves_per_eu(COMB ? 3
                     vn[d] = -v[d];
                 }
                 const double2 cj = s_c[jl], sj = s_s[jl];
-                const AdjRadial R = adj_radial<PDE, DIM, KIND, MASK>(dotv<DIM>(v, v), p);
-                // i target, j source
+                const AdjRadial R = adj_radial<PDE, DIM, KIND, MASK, CHIK>(dotv<DIM>(v, v), p);
+                // i target, j source: g1 = al1 v + be1 nj
                 double2 cK1;
-                double g1[DIM], ga1;
-                kb += adj_sym_terms<PDE, DIM, KIND, MASK>(v, nj, R, p, ci, sj, cK1, g1, ga1);
+                double al1, be1, ga1;
+                kb += adj_sym_terms<PDE, DIM, KIND, MASK>(v, nj, R, p, ci, sj, cK1, al1, be1, ga1);
                 double *aj = acc + js;
                 lds_add(aj + 0 * 64, cK1.x);
                 lds_add(aj + 1 * 64, cK1.y);
@@ -579,28 +599,37 @@ __global__ __launch_bounds__(kSymBS) __attribute__((amdgpu_waves_per_eu(COMB ? 3
                 if (diag) {
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) {
-                        lds_add(aj + (iY + d) * 64, g1[d]);
-                        own[iX + d] -= g1[d];
+                        const double g1 = __builtin_fma(al1, v[d], be1 * nj[d]);
+                        lds_add(aj + (iY + d) * 64, g1);
+                        own[iX + d] -= g1;
                     }
                 } else {
-                    // j target, i source
+                    // j target, i source: g2 = al2 vn + be2 ni = -al2 v + be2 ni
                     double2 cK2;
-                    double g2[DIM], ga2;
-                    kb += adj_sym_terms<PDE, DIM, KIND, MASK>(vn, ni, R, p, cj, si, cK2, g2, ga2);
+                    double al2, be2, ga2;
+                    kb += adj_sym_terms<PDE, DIM, KIND, MASK>(vn, ni, R, p, cj, si, cK2, al2, be2, ga2);
                     own[0] += cK2.x;
                     own[1] += cK2.y;
+                    if constexpr (COMB) {
+                        // g1 - g2 = (al1 + al2) v + be1 nj - be2 ni
+                        const double a12 = al1 + al2;
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d) {
-                        own[iN + d] = __builtin_fma(ga2, vn[d], own[iN + d]);
-                        if constexpr (COMB) {
-                            const double df = g1[d] - g2[d];
+                        for (int d = 0; d < DIM; ++d) {
+                            own[iN + d] = __builtin_fma(-ga2, v[d], own[iN + d]);
+                            const double df = __builtin_fma(a12, v[d], __builtin_fma(be1, nj[d], -be2 * ni[d]));
                             lds_add(aj + (iY + d) * 64, df);
                             own[iY + d] -= df;
-                        } else {
-                            lds_add(aj + (iY + d) * 64, g1[d]);
-                            lds_add(aj + (iX + d) * 64, -g2[d]);
-                            own[iY + d] += g2[d];
-                            own[iX + d] -= g1[d];
+                        }
+                    } else {
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) {
+                            own[iN + d] = __builtin_fma(ga2, vn[d], own[iN + d]);
+                            const double g1 = __builtin_fma(al1, v[d], be1 * nj[d]);
+                            const double g2 = __builtin_fma(al2, vn[d], be2 * ni[d]);
+                            lds_add(aj + (iY + d) * 64, g1);
+                            lds_add(aj + (iX + d) * 64, -g2);
+                            own[iY + d] += g2;
+                            own[iX + d] -= g1;
                         }
                     }
                 }
@@ -743,6 +772,15 @@ using SymAdjFn = void (*)(const AdjSymArgs &, int64_t, hipStream_t);
 
 template <int PDE, int DIM, int KIND>
 void adj_sym_launch(const AdjSymArgs &a, int64_t nblocks, hipStream_t s) {
+    if constexpr (PDE == INTI_HELMHOLTZ && DIM == 3 && KIND == INTI_COMB_SD) {
+        if (a.chik) {
+            if (a.comb)
+                hipLaunchKernelGGL((k_adj_sym<PDE, DIM, KIND, true, true>), dim3((unsigned)nblocks), dim3(kSymBS), 0, s, a);
+            else
+                hipLaunchKernelGGL((k_adj_sym<PDE, DIM, KIND, false, true>), dim3((unsigned)nblocks), dim3(kSymBS), 0, s, a);
+            return;
+        }
+    }
     if (a.comb) hipLaunchKernelGGL((k_adj_sym<PDE, DIM, KIND, true>), dim3((unsigned)nblocks), dim3(kSymBS), 0, s, a);
     else hipLaunchKernelGGL((k_adj_sym<PDE, DIM, KIND, false>), dim3((unsigned)nblocks), dim3(kSymBS), 0, s, a);
 }
@@ -1055,6 +1093,11 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
         sa.ntiles = symT;
         sa.distinct = op->src->distinct ? 1 : 0;
         sa.comb = comb ? 1 : 0;
+        sa.chik = op->pde == INTI_HELMHOLTZ && dim == 3 && kind == INTI_COMB_SD && p.c1.x == 1.0 && p.c1.y == 0.0 &&
+                          p.c0.x == 0.0 && p.c0.y == -p.k && p.dc0.x == 0.0 && p.dc0.y == -1.0 && p.dc1.x == 0.0 &&
+                          p.dc1.y == 0.0
+                      ? 1
+                      : 0;
         sa.p = p;
         if (ctx->profiling) INTI_HIP(ctx, hipEventRecord(ctx->ev0, s));
         fsym(sa, symB, s);

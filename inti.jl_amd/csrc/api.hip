@@ -788,19 +788,49 @@ static int stage_in(Ctx *ctx, Staged &S, size_t bytes, const double *b, const do
     return INTI_OK;
 }
 
-int inti_gmres(inti_op *op, int on_device, const double *b, double *x, int restart, int maxiter,
-               double rtol, double atol, int use_graph, int *iters, double *relres) {
+int inti_gmres_prec(inti_op *op, inti_op *prec, int on_device, const double *b, double *x, int restart,
+                    int maxiter, double rtol, double atol, int use_graph, int *iters, double *relres) {
     if (!op) return INTI_ERR_ARG;
     Ctx *ctx = op->ctx;
     REQUIRE(ctx, b && x, "gmres: null vector");
     REQUIRE(ctx, rtol >= 0 && atol >= 0, "gmres: negative tolerance");
     LOCK(ctx);
-    if (on_device) return gmres_solve(op, b, x, restart, maxiter, rtol, atol, use_graph, iters, relres);
+    if (on_device) return gmres_solve(op, prec, b, x, restart, maxiter, rtol, atol, use_graph, iters, relres);
     const size_t bytes = sizeof(double) * (op_is_complex(op) ? 2 : 1) * op->src->n;
     Staged S;
     int st = stage_in(ctx, S, bytes, b, x);
     if (st) return st;
-    st = gmres_solve(op, (const double *)S.b, (double *)S.x, restart, maxiter, rtol, atol, use_graph, iters, relres);
+    st = gmres_solve(op, prec, (const double *)S.b, (double *)S.x, restart, maxiter, rtol, atol, use_graph, iters,
+                     relres);
+    if (st) return st;
+    INTI_HIP(ctx, hipMemcpyAsync(x, S.x, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return INTI_OK;
+}
+
+int inti_gmres(inti_op *op, int on_device, const double *b, double *x, int restart, int maxiter,
+               double rtol, double atol, int use_graph, int *iters, double *relres) {
+    return inti_gmres_prec(op, nullptr, on_device, b, x, restart, maxiter, rtol, atol, use_graph, iters, relres);
+}
+
+int inti_refine_solve_prec(inti_op *op_hi, inti_op *op_lo, inti_op *prec, int on_device, const double *b,
+                           double *x, int restart, int maxiter_inner, double inner_rtol, int max_outer,
+                           double rtol, int use_graph, int *outer, int *inner_total, double *relres) {
+    if (!op_hi || !op_lo) return INTI_ERR_ARG;
+    Ctx *ctx = op_hi->ctx;
+    REQUIRE(ctx, op_lo->ctx == ctx, "refine: operators on different contexts");
+    REQUIRE(ctx, b && x, "refine: null vector");
+    REQUIRE(ctx, restart >= 1 && maxiter_inner >= 1 && max_outer >= 0, "refine: bad iteration limits");
+    LOCK(ctx);
+    if (on_device)
+        return refine_solve(op_hi, op_lo, prec, b, x, restart, maxiter_inner, inner_rtol, max_outer, rtol, use_graph,
+                            outer, inner_total, relres);
+    const size_t bytes = sizeof(double) * (op_is_complex(op_hi) ? 2 : 1) * op_hi->src->n;
+    Staged S;
+    int st = stage_in(ctx, S, bytes, b, x);
+    if (st) return st;
+    st = refine_solve(op_hi, op_lo, prec, (const double *)S.b, (double *)S.x, restart, maxiter_inner, inner_rtol,
+                      max_outer, rtol, use_graph, outer, inner_total, relres);
     if (st) return st;
     INTI_HIP(ctx, hipMemcpyAsync(x, S.x, bytes, hipMemcpyDeviceToHost, ctx->stream));
     INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -810,25 +840,8 @@ int inti_gmres(inti_op *op, int on_device, const double *b, double *x, int resta
 int inti_refine_solve(inti_op *op_hi, inti_op *op_lo, int on_device, const double *b, double *x,
                       int restart, int maxiter_inner, double inner_rtol, int max_outer, double rtol,
                       int use_graph, int *outer, int *inner_total, double *relres) {
-    if (!op_hi || !op_lo) return INTI_ERR_ARG;
-    Ctx *ctx = op_hi->ctx;
-    REQUIRE(ctx, op_lo->ctx == ctx, "refine: operators on different contexts");
-    REQUIRE(ctx, b && x, "refine: null vector");
-    REQUIRE(ctx, restart >= 1 && maxiter_inner >= 1 && max_outer >= 0, "refine: bad iteration limits");
-    LOCK(ctx);
-    if (on_device)
-        return refine_solve(op_hi, op_lo, b, x, restart, maxiter_inner, inner_rtol, max_outer, rtol, use_graph,
-                            outer, inner_total, relres);
-    const size_t bytes = sizeof(double) * (op_is_complex(op_hi) ? 2 : 1) * op_hi->src->n;
-    Staged S;
-    int st = stage_in(ctx, S, bytes, b, x);
-    if (st) return st;
-    st = refine_solve(op_hi, op_lo, (const double *)S.b, (double *)S.x, restart, maxiter_inner, inner_rtol,
-                      max_outer, rtol, use_graph, outer, inner_total, relres);
-    if (st) return st;
-    INTI_HIP(ctx, hipMemcpyAsync(x, S.x, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return INTI_OK;
+    return inti_refine_solve_prec(op_hi, op_lo, nullptr, on_device, b, x, restart, maxiter_inner, inner_rtol,
+                                  max_outer, rtol, use_graph, outer, inner_total, relres);
 }
 
 // --------------------------------------------------------------------- CSR

@@ -67,7 +67,9 @@ __global__ __launch_bounds__(256) void k_traces(TraceArgs a) {
 // quarter of the block's sources; the quarters are added in LDS in wave
 // order (deterministic). With one source chunk (split = 1) the block writes
 // Theta itself, mu G included; otherwise partial sums go to `part` and
-// k_theta_reduce adds them.
+// k_theta_reduce adds them. (Measured at 63,480 nodes, 3D Helmholtz, ns = 18:
+// 4 waves and 3 source chunks 32.3 ms, 24 chunks 29.6 ms; 3 waves and one
+// chunk, all 992 blocks nominally resident, 46.6 ms.)
 constexpr int kThetaWaves = 4;
 constexpr int kThetaRed = 8;  // doubles per lane per LDS reduction round
 
@@ -175,9 +177,9 @@ __global__ __launch_bounds__(64 * kThetaWaves) void k_theta(ThetaArgs a) {
     // this wavefront's sources: a quarter of the block's chunk
     const int64_t b0 = chunk * a.chunk < a.n ? chunk * a.chunk : a.n;
     const int64_t b1 = b0 + a.chunk < a.n ? b0 + a.chunk : a.n;
-    const int64_t quarter = (b1 - b0 + kThetaWaves - 1) / kThetaWaves;
-    const int64_t j0 = b0 + wave * quarter;
-    const int64_t j1 = j0 + quarter < b1 ? j0 + quarter : b1;
+    const int64_t share = (b1 - b0 + kThetaWaves - 1) / kThetaWaves;
+    const int64_t j0 = b0 + wave * share;
+    const int64_t j1 = j0 + share < b1 ? j0 + share : b1;
     const uniform_double *rec = (const uniform_double *)a.rec;
     for (int64_t j = j0; j < j1; ++j) {
         const uniform_double *R = rec + j * a.rl;
@@ -212,7 +214,7 @@ __global__ __launch_bounds__(64 * kThetaWaves) void k_theta(ThetaArgs a) {
             }
         }
     }
-    // add the quarters in wave order, kThetaRed accumulators per round
+    // add the waves' sums in wave order, kThetaRed accumulators per round
 #pragma unroll
     for (int rd = 0; rd < NR; ++rd) {
         if (wave > 0) {
@@ -660,10 +662,14 @@ int bdim_build_types(Op *opS, Op *opD, int deriv, int ntypes, const int *nq, con
         fns.pack(pa, dim3((unsigned)((n * (nst + 1) + 255) / 256)), st);
         INTI_HIP(ctx, hipGetLastError());
         // source split: the fewest rounds of resident blocks per unit of work,
-        // the smaller split on ties (every split adds nloc x nst partials)
+        // the smaller split on ties. Every split adds nloc x nst partial sums
+        // written and read once: up to 3 when the tiles alone fill a round
+        // (partial traffic <= 3x Theta), more only for small builds
         const int64_t ntiles = (nloc + 63) / 64;
         const int64_t slots = (int64_t)fns.theta_occ * ctx->num_cu;
-        const int64_t max_split = std::max<int64_t>(1, std::min<int64_t>(64, n / (64 * kThetaWaves)));
+        const int64_t want = std::max<int64_t>(3, (slots + ntiles - 1) / ntiles);
+        const int64_t max_split = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(64, want),
+                                                                         n / (64 * kThetaWaves)));
         int64_t best = 1;
         double best_cost = 1e300;
         for (int64_t sp = 1; sp <= max_split; ++sp) {

@@ -117,11 +117,11 @@ struct DirectLaunch {
 int launch_apply(const DirectLaunch &L);
 // allocate the scratch an apply of nrhs columns needs (before graph capture)
 int apply_reserve(const Op *op, int nrhs);
-int gmres_solve(Op *op, const double *b, double *x, int restart, int maxiter, double rtol,
+int gmres_solve(Op *op, Op *prec, const double *b, double *x, int restart, int maxiter, double rtol,
                 double atol, int use_graph, int *iters, double *relres);
-int refine_solve(Op *op_hi, Op *op_lo, const double *b, double *x, int restart, int maxiter_inner,
-                 double inner_rtol, int max_outer, double rtol, int use_graph, int *outer,
-                 int *inner_total, double *relres);
+int refine_solve(Op *op_hi, Op *op_lo, Op *prec, const double *b, double *x, int restart, int maxiter_inner,
+                 double inner_rtol, int max_outer, double rtol, int use_graph, int *outer, int *inner_total,
+                 double *relres);
 int launch_assemble(const Op *op, double *out_dev);
 void op_free_correction(Op *op);
 int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_bar, double *ybar,

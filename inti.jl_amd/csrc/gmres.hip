@@ -265,7 +265,7 @@ struct GmresWork {
     int m;
     int64_t n;
     int cplx;
-    void *V = nullptr, *w = nullptr, *vk = nullptr, *r = nullptr;
+    void *V = nullptr, *w = nullptr, *vk = nullptr, *r = nullptr, *t = nullptr;  // t: P v (preconditioned)
     void *partial = nullptr, *h = nullptr, *H = nullptr, *sn = nullptr, *g = nullptr, *y = nullptr;
     double *cs = nullptr, *res = nullptr, *npart = nullptr;
     GState *st = nullptr;
@@ -280,7 +280,7 @@ struct GmresWork {
         if (stream) (void)hipStreamSynchronize(stream);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        void *ps[] = {V, w, vk, r, partial, h, H, sn, g, y, cs, res, npart, st};
+        void *ps[] = {V, w, vk, r, t, partial, h, H, sn, g, y, cs, res, npart, st};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -294,7 +294,7 @@ static int gwork_alloc(Ctx *ctx, GmresWork &W, int m, int64_t n, int cplx) {
     const size_t hm = (size_t)(m + 1) * m;
     bool ok = hipMalloc(&W.V, es * (size_t)n * (m + 1)) == hipSuccess &&
               hipMalloc(&W.w, es * n) == hipSuccess && hipMalloc(&W.vk, es * n) == hipSuccess &&
-              hipMalloc(&W.r, es * n) == hipSuccess &&
+              hipMalloc(&W.r, es * n) == hipSuccess && hipMalloc(&W.t, es * n) == hipSuccess &&
               hipMalloc(&W.partial, es * (size_t)W.nb * (m + 1)) == hipSuccess &&
               hipMalloc(&W.h, es * (m + 1)) == hipSuccess && hipMalloc(&W.H, es * hm) == hipSuccess &&
               hipMalloc(&W.sn, es * m) == hipSuccess && hipMalloc(&W.g, es * (m + 1)) == hipSuccess &&
@@ -307,17 +307,25 @@ static int gwork_alloc(Ctx *ctx, GmresWork &W, int m, int64_t n, int cplx) {
     return INTI_OK;
 }
 
-// one Arnoldi step on the stream (also what the graph records)
+// y = alpha A x + beta y (real alpha, beta) on the context's stream
+static int apply_vec(Op *op, const void *x, void *y, int64_t n, double alpha, double beta) {
+    DirectLaunch L;
+    L.op = op; L.nrhs = 1; L.x = (const double *)x; L.ldx = n; L.y = (double *)y; L.ldy = n;
+    L.alpha[0] = alpha; L.alpha[1] = 0.0; L.beta[0] = beta; L.beta[1] = 0.0;
+    return launch_apply(L);
+}
+
+// one Arnoldi step on the stream (also what the graph records); with a
+// right preconditioner P the step applies A P
 template <bool CPLX>
-static int arnoldi_step(Op *op, GmresWork &W, hipStream_t s) {
+static int arnoldi_step(Op *op, Op *prec, GmresWork &W, hipStream_t s) {
     using T = typename Sc<CPLX>::T;
     const int64_t n = W.n;
     const dim3 gv((unsigned)((n + kGB - 1) / kGB));
     hipLaunchKernelGGL(k_gcopy<CPLX>, gv, dim3(kGB), 0, s, (const T *)W.V, n, W.st, (T *)W.vk, n);
-    DirectLaunch L;
-    L.op = op; L.nrhs = 1; L.x = (const double *)W.vk; L.ldx = n; L.y = (double *)W.w; L.ldy = n;
-    L.alpha[0] = 1.0; L.alpha[1] = 0.0; L.beta[0] = 0.0; L.beta[1] = 0.0;
-    int rc = launch_apply(L);
+    int rc = prec ? apply_vec(prec, W.vk, W.t, n, 1.0, 0.0) : INTI_OK;
+    if (rc) return rc;
+    rc = apply_vec(op, prec ? W.t : W.vk, W.w, n, 1.0, 0.0);
     if (rc) return rc;
     for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(k_gdots<CPLX>, dim3((unsigned)W.nb, (unsigned)(W.m + 1)), dim3(kGB), 0, s,
@@ -366,8 +374,10 @@ static int residual(Op *op, GmresWork &W, const double *b, const double *x, doub
 }
 
 // GMRES(m) solving A x = b to ||b - A x|| <= tol_abs; x in/out (device).
+// prec (or null): a right preconditioner P, GMRES on A P y = b - A x0 with
+// x = x0 + P y (the residual it minimises is the true one).
 template <bool CPLX>
-static int gmres_core(Op *op, GmresWork &W, const double *b, double *x, int maxiter, double tol_abs,
+static int gmres_core(Op *op, Op *prec, GmresWork &W, const double *b, double *x, int maxiter, double tol_abs,
                       int use_graph, int check_every, int *iters, double *resnorm) {
     using T = typename Sc<CPLX>::T;
     using cd = std::complex<double>;
@@ -403,9 +413,10 @@ static int gmres_core(Op *op, GmresWork &W, const double *b, double *x, int maxi
                         // the apply's scratch must exist before capture: no
                         // allocation may happen while the stream records
                         rc = apply_reserve(op, 1);
+                        if (!rc && prec) rc = apply_reserve(prec, 1);
                         if (rc) { ctx->profiling = saved_prof; return rc; }
                         INTI_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                        rc = arnoldi_step<CPLX>(op, W, s);
+                        rc = arnoldi_step<CPLX>(op, prec, W, s);
                         hipGraph_t gph = nullptr;
                         hipError_t e = hipStreamEndCapture(s, &gph);
                         if (rc) { if (gph) (void)hipGraphDestroy(gph); ctx->profiling = saved_prof; return rc; }
@@ -415,7 +426,7 @@ static int gmres_core(Op *op, GmresWork &W, const double *b, double *x, int maxi
                     }
                     INTI_HIP(ctx, hipGraphLaunch(W.exec, s));
                 } else {
-                    rc = arnoldi_step<CPLX>(op, W, s);
+                    rc = arnoldi_step<CPLX>(op, prec, W, s);
                     if (rc) { ctx->profiling = saved_prof; return rc; }
                 }
             }
@@ -448,8 +459,19 @@ static int gmres_core(Op *op, GmresWork &W, const double *b, double *x, int maxi
             else yd[i] = y[i].real();
         }
         INTI_HIP(ctx, hipMemcpyAsync(W.y, yd.data(), es * kk, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_gxupdate<CPLX>, gv, dim3(kGB), 0, s, (const T *)W.V, n, kk, (const T *)W.y,
-                           (T *)x, n);
+        if (prec) {
+            // x += P (V y): V y into t, then the preconditioner's apply adds onto x
+            hipLaunchKernelGGL(k_gfill<double>, dim3((unsigned)((n * (CPLX ? 2 : 1) + kGB - 1) / kGB)), dim3(kGB),
+                               0, s, n * (CPLX ? 2 : 1), (double *)W.t, 0.0);
+            hipLaunchKernelGGL(k_gxupdate<CPLX>, gv, dim3(kGB), 0, s, (const T *)W.V, n, kk, (const T *)W.y,
+                               (T *)W.t, n);
+            INTI_HIP(ctx, hipGetLastError());
+            rc = apply_vec(prec, W.t, x, n, 1.0, 1.0);
+            if (rc) { ctx->profiling = saved_prof; return rc; }
+        } else {
+            hipLaunchKernelGGL(k_gxupdate<CPLX>, gv, dim3(kGB), 0, s, (const T *)W.V, n, kk, (const T *)W.y,
+                               (T *)x, n);
+        }
         INTI_HIP(ctx, hipGetLastError());
         it += kk;
         rc = residual<CPLX>(op, W, b, x, (double *)W.r, rn);
@@ -474,11 +496,22 @@ static double dev_norm(Ctx *ctx, GmresWork &W, const double *v, bool cplx) {
     return std::sqrt(t);
 }
 
-int gmres_solve(Op *op, const double *b, double *x, int restart, int maxiter, double rtol, double atol,
+static int check_prec(Op *op, Op *prec) {
+    if (!prec) return INTI_OK;
+    if (prec->ctx != op->ctx) return set_error(op->ctx, INTI_ERR_ARG, "gmres: preconditioner on another context");
+    if (prec->trg->n != op->src->n || prec->src->n != op->src->n)
+        return set_error(op->ctx, INTI_ERR_SIZE, "gmres: preconditioner must be square of the operator's size");
+    if (op_is_complex(prec) != op_is_complex(op))
+        return set_error(op->ctx, INTI_ERR_ARG, "gmres: preconditioner and operator value types differ");
+    return INTI_OK;
+}
+
+int gmres_solve(Op *op, Op *prec, const double *b, double *x, int restart, int maxiter, double rtol, double atol,
                 int use_graph, int *iters, double *relres) {
     Ctx *ctx = op->ctx;
     const int64_t n = op->src->n;
     if (op->trg->n != n) return set_error(ctx, INTI_ERR_SIZE, "gmres: operator is not square");
+    if (int rc = check_prec(op, prec)) return rc;
     if (restart < 1 || maxiter < 0) return set_error(ctx, INTI_ERR_ARG, "gmres: restart >= 1, maxiter >= 0");
     const bool cplx = op_is_complex(op);
     GmresWork W;
@@ -487,20 +520,21 @@ int gmres_solve(Op *op, const double *b, double *x, int restart, int maxiter, do
     const double bn = dev_norm(ctx, W, b, cplx);
     const double tol = std::max(rtol * bn, atol);
     double rn = 0.0;
-    rc = cplx ? gmres_core<true>(op, W, b, x, maxiter, tol, use_graph, 8, iters, &rn)
-              : gmres_core<false>(op, W, b, x, maxiter, tol, use_graph, 8, iters, &rn);
+    rc = cplx ? gmres_core<true>(op, prec, W, b, x, maxiter, tol, use_graph, 8, iters, &rn)
+              : gmres_core<false>(op, prec, W, b, x, maxiter, tol, use_graph, 8, iters, &rn);
     if (rc) return rc;
     if (relres) *relres = bn > 0 ? rn / bn : rn;
     return INTI_OK;
 }
 
-int refine_solve(Op *op_hi, Op *op_lo, const double *b, double *x, int restart, int maxiter_inner,
+int refine_solve(Op *op_hi, Op *op_lo, Op *prec, const double *b, double *x, int restart, int maxiter_inner,
                  double inner_rtol, int max_outer, double rtol, int use_graph, int *outer,
                  int *inner_total, double *relres) {
     Ctx *ctx = op_hi->ctx;
     const int64_t n = op_hi->src->n;
     if (op_lo->src != op_hi->src || op_lo->trg != op_hi->trg)
         return set_error(ctx, INTI_ERR_ARG, "refine: operators must share target and source");
+    if (int rc = check_prec(op_hi, prec)) return rc;
     if (op_hi->trg->n != n) return set_error(ctx, INTI_ERR_SIZE, "refine: operator is not square");
     const bool cplx = op_is_complex(op_hi);
     if (cplx != op_is_complex(op_lo)) return set_error(ctx, INTI_ERR_ARG, "refine: value types differ");
@@ -527,8 +561,10 @@ int refine_solve(Op *op_hi, Op *op_lo, const double *b, double *x, int restart, 
         int it = 0;
         double irn = 0.0;
         const double itol = inner_rtol * rn;
-        rc = cplx ? gmres_core<true>(op_lo, W, (const double *)r, (double *)d, maxiter_inner, itol, use_graph, 8, &it, &irn)
-                  : gmres_core<false>(op_lo, W, (const double *)r, (double *)d, maxiter_inner, itol, use_graph, 8, &it, &irn);
+        rc = cplx ? gmres_core<true>(op_lo, prec, W, (const double *)r, (double *)d, maxiter_inner, itol, use_graph, 8,
+                                     &it, &irn)
+                  : gmres_core<false>(op_lo, prec, W, (const double *)r, (double *)d, maxiter_inner, itol, use_graph, 8,
+                                      &it, &irn);
         if (rc) return rc;
         tot += it;
         if (cplx)

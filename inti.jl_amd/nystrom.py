@@ -280,20 +280,24 @@ def _iop(A):
     return A.iop if hasattr(A, "iop") else A
 
 
-def gmres(A, b, x0=None, *, restart=100, maxiter=100, rtol=1e-6, atol=0.0, use_graph=True):
+def gmres(A, b, x0=None, *, restart=100, maxiter=100, rtol=1e-6, atol=0.0, use_graph=True, precond=None):
     """Restarted GMRES on the GPU for the square operator A (IntegralOperator
     or corrected operator): the `gmres(L, rhs; restart, maxiter)` of
     IterativeSolvers used on Inti's operators
     (docs/src/examples/helmholtz_scattering.jl:494-495). Stops when
-    ||b - A x|| <= max(rtol ||b||, atol). Returns (x, info)."""
+    ||b - A x|| <= max(rtol ||b||, atol). precond: a right preconditioner P
+    (an operator of the same size and value type): GMRES on A P y = b,
+    x = P y. Returns (x, info)."""
     A = _iop(A)
+    P = None if precond is None else _iop(precond)
     b = np.ascontiguousarray(b, dtype=A.dtype)
     x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, dtype=A.dtype).copy()
     it = C.c_int()
     rr = C.c_double()
-    A.ctx.check(A.ctx.lib.inti_gmres(A.handle, 0, b.ctypes.data_as(C.c_void_p),
-                                     x.ctypes.data_as(C.c_void_p), restart, maxiter, rtol, atol,
-                                     1 if use_graph else 0, C.byref(it), C.byref(rr)))
+    A.ctx.check(A.ctx.lib.inti_gmres_prec(A.handle, None if P is None else P.handle, 0,
+                                          b.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p),
+                                          restart, maxiter, rtol, atol, 1 if use_graph else 0, C.byref(it),
+                                          C.byref(rr)))
     return x, {"iterations": it.value, "relres": rr.value}
 
 
@@ -325,16 +329,18 @@ def gmres_adjoint(A, g, x0=None, *, restart=100, maxiter=300, rtol=1e-10):
 
 
 def refine_solve(A_hi, A_lo, b, x0=None, *, restart=100, maxiter_inner=100, inner_rtol=1e-6,
-                 max_outer=10, rtol=1e-10, use_graph=True):
+                 max_outer=10, rtol=1e-10, use_graph=True, precond=None):
     """Iterative refinement: FP64 residuals with A_hi, GMRES corrections with
-    A_lo (e.g. the MIXED operator). Returns (x, info)."""
+    A_lo (e.g. the MIXED operator), right-preconditioned by precond if given.
+    Returns (x, info)."""
     A_hi, A_lo = _iop(A_hi), _iop(A_lo)
+    P = None if precond is None else _iop(precond)
     b = np.ascontiguousarray(b, dtype=A_hi.dtype)
     x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, dtype=A_hi.dtype).copy()
     o, tot = C.c_int(), C.c_int()
     rr = C.c_double()
-    A_hi.ctx.check(A_hi.ctx.lib.inti_refine_solve(
-        A_hi.handle, A_lo.handle, 0, b.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p),
-        restart, maxiter_inner, inner_rtol, max_outer, rtol, 1 if use_graph else 0, C.byref(o),
-        C.byref(tot), C.byref(rr)))
+    A_hi.ctx.check(A_hi.ctx.lib.inti_refine_solve_prec(
+        A_hi.handle, A_lo.handle, None if P is None else P.handle, 0, b.ctypes.data_as(C.c_void_p),
+        x.ctypes.data_as(C.c_void_p), restart, maxiter_inner, inner_rtol, max_outer, rtol,
+        1 if use_graph else 0, C.byref(o), C.byref(tot), C.byref(rr)))
     return x, {"outer": o.value, "inner_iterations": tot.value, "relres": rr.value}

@@ -443,16 +443,15 @@ def test_gpu_shape_gradient_through_the_corrected_solve(inti):
     CFIE solve (bench.shape_gradient: sound-soft scattering of e^{ikz}, k = 4,
     perturbed sphere at icosphere frequency 8, 7,680 nodes, 16 directions)
     against central differences (h = 1e-6) of the whole solve along 8 seeded
-    directions in nodes, normals, weights and k: with the DIM correction
-    matrices frozen at the base geometry (what the adjoint differentiates)
-    to 1e-5 of the largest directional derivative of the same kind (a random
-    direction in the nodes can give a derivative far smaller than the
-    gradient's size, and the FD error, h^2 times the third derivative plus
-    the solver's noise over h, does not shrink with it). With the correction
-    rebuilt on every perturbed geometry (the full solve) the difference is
-    the omitted derivative of the correction: reported, not bounded (it is
-    of the size of the gradient in the node, normal and weight directions; k
-    does not enter the correction's geometry)."""
+    directions in nodes, normals, weights and k: the full gradient (the DIM
+    corrections differentiated too) against FD of the solve with the
+    corrections rebuilt on every perturbed geometry, and the gradient with the
+    corrections held fixed against FD with them frozen. Errors are measured
+    against the largest directional derivative of the same kind (a random
+    node direction can give a derivative far below the gradient's size; the
+    FD error, h^2 times the third derivative plus the solver's noise over h,
+    does not shrink with it): 1e-5 frozen, 1e-4 full (the rebuilt
+    corrections' least squares are ill-conditioned)."""
     import bench
     print("shape gradient through the solve, 7,680 nodes", flush=True)
     Q = inti.sphere_quadrature(8, radial=lambda th, ph: 1.0 + 0.1 * np.sin(3 * th) * np.cos(2 * ph))
@@ -462,13 +461,16 @@ def test_gpu_shape_gradient_through_the_corrected_solve(inti):
     print(f"  J {J:.6e}, solve {info['solve']}, adjoint {info['adjoint_solve']}", flush=True)
     assert info["adjoint_solve"]["relres"] <= 1e-11
     checks = bench.shape_fd_check(inti, Q.coords, Q.normals, Q.weights, Q.qtags, k, dirs, grad,
-                                  info["corrections"], 8, 7, sigma0=info["sigma"])
-    scale = {}
-    for w, _, fz, _ in checks:
-        scale[w] = max(scale.get(w, 0.0), abs(fz))
-    worst = 0.0
-    for w, a, fz, fl in checks:
-        ez, el = abs(a - fz) / scale[w], abs(a - fl) / abs(fl)
-        print(f"  {w:8s} adjoint {a:+.9e} fd(frozen) {fz:+.9e} fd(full) {fl:+.9e}  {ez:.1e} {el:.1e}", flush=True)
-        worst = max(worst, ez)
-    assert worst <= 1e-5
+                                  info["corrections"], 8, 7, sigma0=info["sigma"], grad_frozen=info["grad_frozen"])
+    sfull, sfroz = {}, {}
+    for w, _, fz, fl, _ in checks:
+        sfull[w] = max(sfull.get(w, 0.0), abs(fl))
+        sfroz[w] = max(sfroz.get(w, 0.0), abs(fz))
+    worst_full = worst_frozen = 0.0
+    for w, a, fz, fl, af in checks:
+        el, ez = abs(a - fl) / sfull[w], abs(af - fz) / sfroz[w]
+        print(f"  {w:8s} gradient {a:+.9e} fd(full) {fl:+.9e} {el:.1e}; frozen {af:+.9e} fd {fz:+.9e} {ez:.1e}",
+              flush=True)
+        worst_full, worst_frozen = max(worst_full, el), max(worst_frozen, ez)
+    assert worst_frozen <= 1e-5
+    assert worst_full <= 1e-4

@@ -149,3 +149,41 @@ def test_bumpy_sphere_refined_solve_to_1e10(inti, oracle):
     say(f"  residual with the oracle's matrix {res_or:.2e}; vs dense LU solution {e_sol:.2e}")
     assert res_or <= 2e-10
     assert e_sol <= 4.3e5 * 1e-10
+
+
+def test_bumpy_sphere_calderon_preconditioned_solve(inti, oracle):
+    """The same 12,000-node bumpy-sphere single-layer problem, right-
+    preconditioned by the corrected hypersingular operator (Calderon): the
+    refined solve reaches a 1e-10 FP64 residual in far fewer inner steps than
+    the unpreconditioned one (604), the residual with the oracle's corrected
+    matrix is <= 2e-10, and plain preconditioned GMRES in FP64 returns a
+    solution with that residual too (x = P y applied correctly)."""
+    import bench
+    say("[gpu] bumpy sphere Calderon-preconditioned solve, 12,000 nodes")
+    Q = bench.bumpy_quadrature(inti, 10)
+    n = len(Q)
+    op = inti.Laplace(dim=3)
+    S64 = inti.IntegralOperator(inti.SingleLayerKernel(op), Q)
+    D64 = inti.IntegralOperator(inti.DoubleLayerKernel(op), Q)
+    dS, _ = inti.bdim_correction(op, Q, Q, S64, D64, green_multiplier=np.full(n, -0.5))
+    S64.attach(dS)
+    S32 = inti.IntegralOperator(inti.SingleLayerKernel(op), Q, precision="mixed")
+    S32.attach(dS)
+    P32 = bench.calderon_preconditioner(inti, Q)
+    th = np.arccos(np.clip(Q.coords[2] / np.linalg.norm(Q.coords, axis=0), -1, 1))
+    ph = np.arctan2(Q.coords[1], Q.coords[0])
+    b = bench._real_harmonics(th, ph, 8, np.random.default_rng(bench.SEED))
+    x, info = inti.refine_solve(S64, S32, b, restart=200, maxiter_inner=200, inner_rtol=1e-6, max_outer=6,
+                                rtol=1e-10, precond=P32)
+    say(f"  preconditioned refined solve {info}")
+    assert info["relres"] <= 1e-10 and info["inner_iterations"] < 300
+    rp, ci, vS, _, _ = oracle.bdim(LAP, 3, 0.0, Q, Q, np.full(n, -0.5))
+    A = oracle.assemble(LAP, 3, 0, 0.0, Q, Q).real + sp.csr_matrix((vS.real, ci, rp), shape=(n, n)).toarray()
+    res_or = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    say(f"  residual with the oracle's matrix {res_or:.2e}")
+    assert res_or <= 2e-10
+    P64 = bench.calderon_preconditioner(inti, Q, precision="fp64")
+    x2, info2 = inti.gmres(S64, b, restart=200, maxiter=400, rtol=1e-10, precond=P64)
+    res2 = np.linalg.norm(b - A @ x2) / np.linalg.norm(b)
+    say(f"  preconditioned FP64 gmres {info2}, residual with the oracle's matrix {res2:.2e}")
+    assert info2["relres"] <= 1e-10 and res2 <= 2e-10
