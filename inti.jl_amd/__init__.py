@@ -12,7 +12,8 @@ from .api import (CorrectedOperator, adj_double_layer_hypersingular, combined_fi
                   single_double_layer, single_double_layer_potential)
 from .batch import BatchOperator
 from .dense import DenseOperator
-from .bdim import DimParameters, bdim_correction, etype_to_nearest_points
+from .bdim import (DimParameters, bdim_correction, element_centers_and_diameters, etype_to_nearest_points,
+                   near_interaction_list)
 from .bdim_pullback import bdim_pullback
 from .kernels import (AdjointDoubleLayerKernel, CombinedFieldKernel, DoubleLayerKernel,
                       FarFieldDoubleLayerKernel, FarFieldSingleLayerKernel, Helmholtz,
@@ -44,7 +45,8 @@ __all__ = [
     "single_double_layer_potential", "FarFieldSingleLayerKernel", "FarFieldDoubleLayerKernel",
     "rrule", "BatchOperator", "DenseOperator",
     "default_context", "default_density_eltype", "default_kernel_eltype",
-    "ellipse_quadrature", "etype_to_nearest_points", "bdim_pullback", "gauss_legendre", "gauss_legendre_order", "gmres", "join_quadratures",
+    "ellipse_quadrature", "etype_to_nearest_points", "bdim_pullback", "near_interaction_list",
+    "element_centers_and_diameters", "gauss_legendre", "gauss_legendre_order", "gmres", "join_quadratures",
     "gmres_adjoint",
     "refine_solve",
     "points", "single_double_layer", "sphere_quadrature", "surface_quadrature", "vioreanu_rokhlin_triangle",

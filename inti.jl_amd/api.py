@@ -54,7 +54,10 @@ def _dim_corrections(op, target, source, Sop, Dop, correction, derivative):
     else:
         loc = "on" if target is source else correction["target_location"]
         mu = np.full(len(target), _green_multiplier(loc))
-    near = etype_to_nearest_points(target, source, correction["maxdist"])
+    # "near": element -> target lists given by the caller (e.g. a stencil of
+    # near_interaction_list), in place of the nearest-node assignment
+    near = correction["near"] if "near" in correction else etype_to_nearest_points(target, source,
+                                                                                    correction["maxdist"])
     return bdim_correction(op, target, source, Sop, Dop, green_multiplier=mu,
                            parameters=correction.get("parameters", DimParameters()),
                            derivative=derivative, near=near)

@@ -50,6 +50,51 @@ def etype_to_nearest_points(target, source, maxdist=np.inf):
     return out[0] if len(out) == 1 else out
 
 
+def element_centers_and_diameters(quad):
+    """Per element type: the centres of the elements (the image of the
+    reference element's centre, s = 1/2 on P2 lines and (1/3, 1/3) on P2
+    triangles, when the quadrature keeps its elements; else the weighted mean
+    of the element's nodes) and their diameters (the largest distance between
+    two of the element's geometry nodes, or quadrature nodes)."""
+    out = []
+    els = quad.elements if len(quad.etype_qtags) == 1 else None
+    for qtags in quad.etype_qtags:
+        X = quad.coords[:, qtags]                       # (dim, nq, ne)
+        if els is not None and els.ndim == 3 and els.shape[1] in (3, 6):
+            V = np.asarray(els, dtype=np.float64)       # (ne, 3 or 6, dim)
+            if V.shape[1] == 3:                         # P2 line: nodes at s = 0, 1/2, 1
+                c = V[:, 1, :].T
+            else:                                       # P2 triangle at (1/3, 1/3)
+                c = (-(V[:, 0] + V[:, 2] + V[:, 5]) / 9 + 4 * (V[:, 1] + V[:, 3] + V[:, 4]) / 9).T
+            P = V.transpose(2, 1, 0)                    # (dim, nodes, ne)
+        else:
+            w = quad.weights[qtags]
+            c = np.sum(X * w[None], axis=1) / np.sum(w, axis=0)[None]
+            P = X
+        diff = P[:, :, None, :] - P[:, None, :, :]
+        diam = np.sqrt(np.sum(diff * diff, axis=0)).max(axis=(0, 1))
+        out.append((c, diam))
+    return out
+
+
+def near_interaction_list(X, quad, tol):
+    """For each element of each type of quad, the points of X (dim x m)
+    within tol of the element's centre (mesh.jl:610-642), as one
+    (near_ptr, near_idx) CSR over elements per type, indices ascending;
+    a stencil for bdim_correction(..., near=...)."""
+    from scipy.spatial import cKDTree
+    tree = cKDTree(np.asarray(X, dtype=np.float64).T)
+    out = []
+    for c, _ in element_centers_and_diameters(quad):
+        lists = tree.query_ball_point(c.T, r=tol, workers=-1, return_sorted=True)
+        ptr = np.zeros(len(lists) + 1, dtype=np.int64)
+        ptr[1:] = np.cumsum([len(l) for l in lists])
+        idx = (np.concatenate([np.asarray(l, dtype=np.int64) for l in lists]) if len(lists)
+               else np.zeros(0, dtype=np.int64))
+        out.append((ptr, idx))
+    return out[0] if len(out) == 1 else out
+
+
 def bdim_correction(op, target, source, Sop, Dop, *, green_multiplier, parameters=None,
                     derivative=False, maxdist=np.inf, near=None, diagnostics=None):
     """Corrections (dS, dD) such that S + dS and D + dD are accurate

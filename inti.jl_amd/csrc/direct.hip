@@ -94,6 +94,70 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
     }
 }
 
+// Long correction rows (a stencil of many elements per target: hundreds of
+// entries per row): W lanes share a row, so the value and index loads of a
+// wave are contiguous, and y_i += alpha (dA x)_i after k_reduce (which then
+// skips the rows). The W partial sums of a row combine by a fixed butterfly,
+// so the result does not depend on the launch. One row per W lanes, 256
+// threads per block, one right-hand side per blockIdx.y.
+struct CsrAddArgs {
+    int64_t m;
+    int64_t ldx, ldy;
+    const int32_t *rowptr;
+    const int32_t *colind;
+    const double *cval;
+    const double *x;
+    double *y;
+    double ar, ai;
+};
+
+template <bool CPLX, int W>
+__global__ __launch_bounds__(256) void k_csr_add(CsrAddArgs a) {
+    const int64_t i = ((int64_t)blockIdx.x * 256 + threadIdx.x) / W;
+    const int lane = threadIdx.x % W;
+    if (i >= a.m) return;  // whole groups of W lanes leave together
+    const int r = blockIdx.y;
+    const double *x = a.x + (CPLX ? 2 : 1) * (int64_t)r * a.ldx;
+    double *y = a.y + (CPLX ? 2 : 1) * (int64_t)r * a.ldy;
+    const int32_t p0 = a.rowptr[i], p1 = a.rowptr[i + 1];
+    double sr = 0.0, si = 0.0;
+    if constexpr (CPLX) {
+        const double2 *v = reinterpret_cast<const double2 *>(a.cval);
+        const double2 *xv = reinterpret_cast<const double2 *>(x);
+#pragma unroll 4
+        for (int32_t p = p0 + lane; p < p1; p += W) {
+            const double2 c = v[p], xx = xv[a.colind[p]];
+            sr = fma(c.x, xx.x, fma(-c.y, xx.y, sr));
+            si = fma(c.x, xx.y, fma(c.y, xx.x, si));
+        }
+    } else {
+#pragma unroll 4
+        for (int32_t p = p0 + lane; p < p1; p += W) sr = fma(a.cval[p], x[a.colind[p]], sr);
+    }
+#pragma unroll
+    for (int o = W / 2; o > 0; o /= 2) {
+        sr += __shfl_xor(sr, o, W);
+        if constexpr (CPLX) si += __shfl_xor(si, o, W);
+    }
+    if (lane == 0) {
+        if constexpr (CPLX) {
+            y[2 * i] += a.ar * sr - a.ai * si;
+            y[2 * i + 1] += a.ar * si + a.ai * sr;
+        } else {
+            y[i] += a.ar * sr;
+        }
+    }
+}
+
+// lanes per row of the correction: the fused per-thread walk of k_reduce
+// for short rows (the element's own nodes: a few entries), a group of 16 or
+// 64 lanes for long ones; 0 = fused
+static int csr_lanes(const Csr *c) {
+    if (!c || c->m == 0) return 0;
+    const int64_t avg = c->nnz / c->m;
+    return avg >= 96 ? 64 : avg >= 16 ? 16 : 0;
+}
+
 // ------------------------------------------------------------- dispatch
 static ApplyFn pick_apply(int pde, int dim, int kind, int mixed, int nr) {
     if (pde == INTI_LAPLACE && dim == 2) return apply_pick_lap2(kind, mixed, nr);
@@ -332,9 +396,10 @@ int launch_apply(const DirectLaunch &L) {
         ReduceArgs ra;
         ra.m = m; ra.split = n > 0 ? nsplit : 0; ra.part = part;
         ra.ldx = L.ldx; ra.ldy = L.ldy;
-        ra.rowptr = op->corr ? op->corr->rowptr : nullptr;
-        ra.colind = op->corr ? op->corr->colind : nullptr;
-        ra.cval = op->corr ? op->corr->val : nullptr;
+        const int cl = csr_lanes(op->corr);
+        ra.rowptr = op->corr && !cl ? op->corr->rowptr : nullptr;
+        ra.colind = op->corr && !cl ? op->corr->colind : nullptr;
+        ra.cval = op->corr && !cl ? op->corr->val : nullptr;
         ra.x = xr; ra.y = yr;
         ra.ar = L.alpha[0]; ra.ai = L.alpha[1];
         ra.br = L.beta[0]; ra.bi = L.beta[1];
@@ -346,6 +411,22 @@ int launch_apply(const DirectLaunch &L) {
         if (cplx) hipLaunchKernelGGL(k_reduce<true>, rg, dim3(256), 0, ctx->stream, ra);
         else hipLaunchKernelGGL(k_reduce<false>, rg, dim3(256), 0, ctx->stream, ra);
         INTI_HIP(ctx, hipGetLastError());
+        if (cl) {
+            CsrAddArgs ca;
+            ca.m = m; ca.ldx = L.ldx; ca.ldy = L.ldy;
+            ca.rowptr = op->corr->rowptr; ca.colind = op->corr->colind; ca.cval = op->corr->val;
+            ca.x = xr; ca.y = yr;
+            ca.ar = L.alpha[0]; ca.ai = L.alpha[1];
+            dim3 cg((unsigned)((m * cl + 255) / 256), (unsigned)nr);
+            if (cplx) {
+                if (cl == 64) hipLaunchKernelGGL((k_csr_add<true, 64>), cg, dim3(256), 0, ctx->stream, ca);
+                else hipLaunchKernelGGL((k_csr_add<true, 16>), cg, dim3(256), 0, ctx->stream, ca);
+            } else {
+                if (cl == 64) hipLaunchKernelGGL((k_csr_add<false, 64>), cg, dim3(256), 0, ctx->stream, ca);
+                else hipLaunchKernelGGL((k_csr_add<false, 16>), cg, dim3(256), 0, ctx->stream, ca);
+            }
+            INTI_HIP(ctx, hipGetLastError());
+        }
         r0 += nr;
     }
     return INTI_OK;

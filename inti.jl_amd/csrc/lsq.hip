@@ -14,19 +14,36 @@ namespace inti {
 
 void build_pattern(int64_t m, const std::vector<int32_t> &tri_row, const std::vector<int32_t> &tri_col,
                    TripletPattern &P) {
+    // triplets in (row, col) order, equal pairs in their original order: two
+    // stable counting sorts (by column, then by row), linear in the number
+    // of triplets (tens of millions for a stencil of many elements per
+    // target)
     const int64_t ntri = (int64_t)tri_row.size();
-    std::vector<int64_t> order(ntri);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int64_t u, int64_t v) {
-        if (tri_row[u] != tri_row[v]) return tri_row[u] < tri_row[v];
-        return tri_col[u] < tri_col[v];
-    });
+    int32_t ncol = 0;
+    for (int64_t q = 0; q < ntri; ++q) ncol = std::max(ncol, tri_col[q] + 1);
+    std::vector<int32_t> bycol(ntri), order(ntri);
+    {
+        std::vector<int64_t> at((size_t)ncol + 1, 0);
+        for (int64_t q = 0; q < ntri; ++q) ++at[tri_col[q] + 1];
+        for (int32_t j = 0; j < ncol; ++j) at[j + 1] += at[j];
+        for (int64_t q = 0; q < ntri; ++q) bycol[at[tri_col[q]]++] = (int32_t)q;
+    }
+    {
+        std::vector<int64_t> at((size_t)m + 1, 0);
+        for (int64_t q = 0; q < ntri; ++q) ++at[tri_row[q] + 1];
+        for (int64_t i = 0; i < m; ++i) at[i + 1] += at[i];
+        for (int64_t q = 0; q < ntri; ++q) {
+            const int32_t u = bycol[q];
+            order[at[tri_row[u]]++] = u;
+        }
+    }
     P.seg.clear(); P.colind.clear();
+    P.seg.reserve(ntri + 1); P.colind.reserve(ntri);
     P.perm.assign(ntri, 0);
     P.rowptr.assign(m + 1, 0);
     for (int64_t q = 0; q < ntri; ++q) {
-        const int64_t u = order[q];
-        P.perm[q] = (int32_t)u;
+        const int32_t u = order[q];
+        P.perm[q] = u;
         if (q == 0 || tri_row[u] != tri_row[order[q - 1]] || tri_col[u] != tri_col[order[q - 1]]) {
             P.seg.push_back((int32_t)q);
             P.colind.push_back(tri_col[u]);
