@@ -212,10 +212,6 @@ def main():
                          "adjoint on a perturbed sphere; circle: 2D Laplace DL on 640 nodes; dense: the "
                          "assembled corrected operator on many right-hand sides (FP64 matrix cores)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--stencil", choices=["3diam", "self"], default="3diam",
-                    help="cfie workload: DIM correction targets of each element: the nodes within 3 element "
-                         "diameters of its centre (the stated workload), or its own nodes (bdim_correction's "
-                         "self pattern)")
     ap.add_argument("--only", choices=["mixed", "fp64", "gmres"], default=None,
                     help="profiling runs: time one leg only, no checks, no JSON (gmres: the batch "
                          "workload's hipGraph-captured batched GMRES, --steps Arnoldi steps)")
@@ -257,11 +253,7 @@ def run_cfie(inti, lib, ctx, args):
     log(f"bench: nodes={n} elements={Q.ne}")
     op = inti.Helmholtz(k=k, dim=3)
     t0 = time.perf_counter()
-    corr_opts = {"method": "dim"}
-    if args.stencil == "3diam":
-        diam = inti.element_centers_and_diameters(Q)[0][1]
-        corr_opts["near"] = inti.near_interaction_list(Q.coords, Q, 3.0 * float(diam.max()))
-    L64, corr = inti.combined_field(op=op, quad=Q, c_single=-1j * k, c_double=1.0, correction=corr_opts)
+    L64, corr = inti.combined_field(op=op, quad=Q, c_single=-1j * k, c_double=1.0)
     ctx.synchronize()
     t_corr = time.perf_counter() - t0
     Lmx, _ = inti.combined_field(op=op, quad=Q, c_single=-1j * k, c_double=1.0,
@@ -352,8 +344,7 @@ def run_cfie(inti, lib, ctx, args):
                         f"geodesic icosphere freq {args.freq}: {Q.ne} P2 triangles x VR4N6 = {n} nodes, "
                         f"complex Gaussian density (seed 20261202)",
             "nodes": n, "elements": Q.ne, "k": k, "pairs_per_step": pairs,
-            "dim_stencil": ("nodes within 3 element diameters of each element's centre"
-                            if args.stencil == "3diam" else "each element's own nodes"),
+            "dim_stencil": "each element's own nodes (bdim_correction's pattern when target is source)",
             "correction_nnz": corr[1].nnz, "precision": "mixed",
         },
         "fp64_value": round(pairs / t64 / 1e9, 3),

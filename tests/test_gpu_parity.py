@@ -389,11 +389,11 @@ def test_bdim_element_types_split_is_one_type(inti):
 @pytest.mark.parametrize("pde", [LAP, HEL])
 @pytest.mark.parametrize("diameters", [1.0, 3.0])
 def test_bdim_diameter_stencil_matches_oracle(inti, oracle, pde, diameters):
-    """The DIM correction with the stencil of the combined-field workload:
-    each element corrects the nodes within 3 (here also 1) element diameters
-    of its centre (near_interaction_list, mesh.jl:611-642), on a sphere of
-    1,080 nodes (44 and 415 correction entries per row: the two lane-group
-    widths of the long-row correction pass): pattern bit-exact against the
+    """The DIM correction from overlapping lists: each element corrects the
+    nodes within 1 or 3 element diameters of its centre
+    (near_interaction_list, mesh.jl:611-642), on a sphere of 1,080 nodes (44
+    and 415 correction entries per row: the two lane-group widths of the
+    long-row correction pass): pattern bit-exact against the
     oracle's build from the same lists, the corrected operator on a smooth
     density within 1e-12 (or 10x the oracle's own rounding spread), for one
     right-hand side and for five at once, and every element's own nodes
