@@ -125,6 +125,13 @@ int inti_op_destroy(inti_op *op);
 /* Adds cid * x to every apply (square operators only), e.g. the I/2 of
  * I/2 + D - i k S (docs/src/examples/helmholtz_scattering.jl:230). */
 int inti_op_set_identity(inti_op *op, double re, double im);
+/* Imaginary part of the wavenumber: Helmholtz(k = kr + i kimag) of
+ * src/kernels.jl:322-345, whose kernels take any k::Number. 3D, FP64, the
+ * SL/DL/ADL/HS and combined kinds; the sums run in one direction. Set it
+ * before the first apply, assembly or DIM build; a nonzero kimag on another
+ * operator returns INTI_ERR_UNSUPPORTED, and so does the adjoint of such an
+ * operator. */
+int inti_op_set_wavenumber_imag(inti_op *op, double kimag);
 /* Launch shape: threads per block, targets per thread, source splits
  * (0 = automatic). Used by the tuner; defaults are sensible. */
 int inti_op_set_launch(inti_op *op, int block, int tpt, int split);

@@ -50,6 +50,7 @@ SIGNATURES = {
                                  C.c_int, C.POINTER(_vp)]),
     "inti_op_destroy": (C.c_int, [_vp]),
     "inti_op_set_identity": (C.c_int, [_vp, C.c_double, C.c_double]),
+    "inti_op_set_wavenumber_imag": (C.c_int, [_vp, C.c_double]),
     "inti_op_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "inti_op_set_symmetric": (C.c_int, [_vp, C.c_int, C.c_int]),
     "inti_op_apply": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),

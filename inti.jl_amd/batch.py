@@ -28,6 +28,8 @@ class BatchOperator:
         if len(kernels) != len(quads) or not kernels:
             raise ValueError("one kernel per quadrature, at least one problem")
         k0 = kernels[0]
+        if any(getattr(K.op, "ki", 0.0) != 0 for K in kernels):
+            raise NotImplementedError("batches take real wavenumbers")
         kind, pde, dim = k0.kind, k0.op.pde, k0.op.dim
         for K in kernels:
             if K.kind != kind or K.op.pde != pde or K.op.dim != dim:

@@ -94,6 +94,8 @@ def bdim_pullback(op, quad, Sbar, Dbar, rowptr, colind, *, green_multiplier, par
     "weights": (n,), "k": float}."""
     if quad.dim != 3:
         raise NotImplementedError("bdim_pullback: 3D quadratures")
+    if getattr(op, "ki", 0.0) != 0:
+        raise NotImplementedError("bdim_pullback: real wavenumbers only")
     if len(quad.etype_qtags) != 1:
         raise NotImplementedError("bdim_pullback: one element type")
     params = parameters or DimParameters()

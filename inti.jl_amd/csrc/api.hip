@@ -398,6 +398,23 @@ int inti_op_set_identity(inti_op *op, double re, double im) {
     return INTI_OK;
 }
 
+int inti_op_set_wavenumber_imag(inti_op *op, double kimag) {
+    if (!op) return INTI_ERR_ARG;
+    Ctx *ctx = op->ctx;
+    REQUIRE(ctx, std::isfinite(kimag), "op_set_wavenumber_imag: kimag must be finite");
+    if (kimag != 0.0) {
+        if (op->pde != INTI_HELMHOLTZ || op->dim != 3)
+            return set_error(ctx, INTI_ERR_UNSUPPORTED, "op_set_wavenumber_imag: complex wavenumbers are 3D Helmholtz only");
+        if (op->kind >= INTI_FF_SL)
+            return set_error(ctx, INTI_ERR_UNSUPPORTED, "op_set_wavenumber_imag: no far-field kernels for complex wavenumbers");
+        if (op->precision != INTI_FP64)
+            return set_error(ctx, INTI_ERR_UNSUPPORTED, "op_set_wavenumber_imag: complex wavenumbers run in FP64");
+    }
+    LOCK(ctx);
+    op->kimag = kimag;
+    return INTI_OK;
+}
+
 int inti_op_set_launch(inti_op *op, int block, int tpt, int split) {
     if (!op) return INTI_ERR_ARG;
     if (block != 256 || tpt != 1 || split < 0)
@@ -562,6 +579,8 @@ int inti_op_adjoint(inti_op *op, const double *ubar, const double *sigma, double
     if (!op) return INTI_ERR_ARG;
     Ctx *ctx = op->ctx;
     REQUIRE(ctx, ubar != nullptr && sigma != nullptr, "adjoint: ubar and sigma are required");
+    if (op->kimag != 0.0)
+        return set_error(ctx, INTI_ERR_UNSUPPORTED, "adjoint: complex wavenumbers have no pullback");
     LOCK(ctx);
     return op_adjoint(op, ubar, sigma, sigma_bar, src_xyz_bar, src_normal_bar, src_weight_bar, trg_xyz_bar,
                       dcoefs, k_bar);
@@ -683,7 +702,7 @@ int inti_bdim_build(inti_op *opS, inti_op *opD, int derivative, int nq, int64_t 
     REQUIRE(ctx, opD && qtags && mu && dS && dD, "bdim: null argument");
     REQUIRE(ctx, opD->ctx == ctx, "bdim: operators on different contexts");
     REQUIRE(ctx, opS->trg == opD->trg && opS->src == opD->src, "bdim: S and D must share target and source");
-    REQUIRE(ctx, opS->pde == opD->pde && opS->dim == opD->dim && opS->k == opD->k,
+    REQUIRE(ctx, opS->pde == opD->pde && opS->dim == opD->dim && opS->k == opD->k && opS->kimag == opD->kimag,
             "bdim: S and D must be the same PDE");
     const int kS = derivative ? INTI_ADL : INTI_SL, kD = derivative ? INTI_HS : INTI_DL;
     REQUIRE(ctx, opS->kind == kS && opD->kind == kD,
@@ -709,7 +728,7 @@ int inti_bdim_build_types(inti_op *opS, inti_op *opD, int derivative, int ntypes
     REQUIRE(ctx, opD && nq && ne && qtags && mu && dS && dD, "bdim: null argument");
     REQUIRE(ctx, opD->ctx == ctx, "bdim: operators on different contexts");
     REQUIRE(ctx, opS->trg == opD->trg && opS->src == opD->src, "bdim: S and D must share target and source");
-    REQUIRE(ctx, opS->pde == opD->pde && opS->dim == opD->dim && opS->k == opD->k,
+    REQUIRE(ctx, opS->pde == opD->pde && opS->dim == opD->dim && opS->k == opD->k && opS->kimag == opD->kimag,
             "bdim: S and D must be the same PDE");
     const int kS = derivative ? INTI_ADL : INTI_SL, kD = derivative ? INTI_HS : INTI_DL;
     REQUIRE(ctx, opS->kind == kS && opD->kind == kD,

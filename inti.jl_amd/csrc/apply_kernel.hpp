@@ -55,7 +55,7 @@ constexpr bool apply_fused() {
 template <int PDE, int DIM, int KIND, bool MIXED, int NR, bool TRANS = false>
 __device__ __forceinline__ void apply_block(const ApplyArgs &a, int64_t chunk, int64_t tile,
                                             double2 *s_geo, double2 *s_q) {
-    constexpr bool CPLX = PDE == INTI_HELMHOLTZ;
+    constexpr bool CPLX = PDE != INTI_LAPLACE;
     constexpr bool FUSED = apply_fused<PDE, DIM, KIND, MIXED, NR, TRANS>();
     constexpr int BS = kApplyBS;
     constexpr int TT = ApplyTile<NR>::TT;
@@ -203,7 +203,7 @@ constexpr int apply_threads() { return NR == 16 ? 64 * (kApplyBS / 16) / kMfmaNF
 template <int PDE, int DIM, int KIND, bool MIXED>
 __device__ __forceinline__ void apply_block_mfma(const ApplyArgs &a, int64_t chunk, int64_t tile,
                                                  double2 *s_geo, double2 *s_q) {
-    constexpr bool CPLX = PDE == INTI_HELMHOLTZ;
+    constexpr bool CPLX = PDE != INTI_LAPLACE;
     constexpr int NR = 16, TT = kMfmaTT, NF = kMfmaNF, NT = apply_threads<16>();
     constexpr int NG = DIM == 3 ? 3 : 2;
     const int tid = threadIdx.x;
@@ -367,7 +367,7 @@ struct BatchArgs {
 
 template <int PDE, int DIM, int KIND, bool MIXED, int NR>
 __global__ __launch_bounds__(apply_threads<NR>()) void k_apply_batch(BatchArgs b) {
-    constexpr int CW = PDE == INTI_HELMHOLTZ ? 2 : 1;
+    constexpr int CW = PDE != INTI_LAPLACE ? 2 : 1;
     using L = ApplyLds<DIM, NR, apply_fused<PDE, DIM, KIND, MIXED, NR, false>()>;
     __shared__ double2 s_geo[L::GEO];
     __shared__ double2 s_q[L::Q];
@@ -507,5 +507,6 @@ ApplyFn apply_pick_lap2(int kind, int mixed, int nr);
 ApplyFn apply_pick_lap3(int kind, int mixed, int nr);
 ApplyFn apply_pick_helm2(int kind, int mixed, int nr);
 ApplyFn apply_pick_helm3(int kind, int mixed, int nr);
+ApplyFn apply_pick_helm3c(int kind, int nr);  // complex wavenumber, FP64
 
 }  // namespace inti

@@ -15,7 +15,7 @@ struct AsmArgs {
 
 template <int PDE, int DIM, int KIND>
 __global__ __launch_bounds__(256) void k_assemble(AsmArgs a) {
-    constexpr bool CPLX = PDE == INTI_HELMHOLTZ;
+    constexpr bool CPLX = PDE != INTI_LAPLACE;
     const int64_t total = a.m * a.n;
     for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * 256) {
@@ -78,6 +78,7 @@ int launch_assemble(const Op *op, double *out_dev) {
     if (op->pde == INTI_LAPLACE && op->dim == 3) fn = asm_kind<INTI_LAPLACE, 3>(kind);
     if (op->pde == INTI_HELMHOLTZ && op->dim == 2) fn = asm_kind<INTI_HELMHOLTZ, 2>(kind);
     if (op->pde == INTI_HELMHOLTZ && op->dim == 3) fn = asm_kind<INTI_HELMHOLTZ, 3>(kind);
+    if (op_kernel_pde(op) == kHelmCK) fn = op->dim == 3 ? asm_kind<kHelmCK, 3>(kind) : nullptr;
     if (!fn) return set_error(op->ctx, INTI_ERR_UNSUPPORTED, "no assembly kernel for this operator");
     AsmArgs a;
     a.m = op->trg->n; a.n = op->src->n;

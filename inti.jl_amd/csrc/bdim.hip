@@ -76,7 +76,7 @@ constexpr int kThetaRed = 8;  // doubles per lane per LDS reduction round
 typedef const __attribute__((address_space(4))) double uniform_double;
 
 template <int PDE>
-constexpr int theta_per_k() { return PDE == INTI_HELMHOLTZ ? 4 : 2; }
+constexpr int theta_per_k() { return PDE != INTI_LAPLACE ? 4 : 2; }
 
 struct ThetaPackArgs {
     int64_t n;
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(64 * kThetaWaves) void k_theta(ThetaArgs a) {
     constexpr int KS = DERIV ? INTI_ADL : INTI_SL;
     constexpr int KD = DERIV ? INTI_HS : INTI_DL;
     constexpr int PK = theta_per_k<PDE>();
-    constexpr int NA = PDE == INTI_HELMHOLTZ ? 2 * NS : NS;  // accumulators per lane
+    constexpr int NA = PDE != INTI_LAPLACE ? 2 * NS : NS;  // accumulators per lane
     constexpr int NR = (NA + kThetaRed - 1) / kThetaRed;
     __shared__ double s_red[kThetaWaves - 1][kThetaRed][64];
     const int lane = threadIdx.x & 63;
@@ -251,8 +251,8 @@ __global__ __launch_bounds__(64 * kThetaWaves) void k_theta(ThetaArgs a) {
 #pragma unroll
             for (int kk = 0; kk < NS; ++kk)
                 if (kk == k) {
-                    vr = PDE == INTI_HELMHOLTZ ? acc[2 * kk] : acc[kk];
-                    vi = PDE == INTI_HELMHOLTZ ? acc[2 * kk + 1] : 0.0;
+                    vr = PDE != INTI_LAPLACE ? acc[2 * kk] : acc[kk];
+                    vi = PDE != INTI_LAPLACE ? acc[2 * kk + 1] : 0.0;
                 }
             a.theta[l + a.nloc * k] = make_double2(mu * gr + vr, mu * gi + vi);
         }
@@ -260,8 +260,8 @@ __global__ __launch_bounds__(64 * kThetaWaves) void k_theta(ThetaArgs a) {
         double2 *pp = a.part + chunk * NS * a.nloc + l;
 #pragma unroll
         for (int k = 0; k < NS; ++k)
-            pp[k * a.nloc] = make_double2(PDE == INTI_HELMHOLTZ ? acc[2 * k] : acc[k],
-                                          PDE == INTI_HELMHOLTZ ? acc[2 * k + 1] : 0.0);
+            pp[k * a.nloc] = make_double2(PDE != INTI_LAPLACE ? acc[2 * k] : acc[k],
+                                          PDE != INTI_LAPLACE ? acc[2 * k + 1] : 0.0);
     }
 }
 
@@ -433,6 +433,7 @@ static bool pick_bdim(int pde, int dim, int deriv, int nst, BdimFns &f) {
     INTI_B(INTI_LAPLACE, 3, 0) INTI_B(INTI_LAPLACE, 3, 1)
     INTI_B(INTI_HELMHOLTZ, 2, 0) INTI_B(INTI_HELMHOLTZ, 2, 1)
     INTI_B(INTI_HELMHOLTZ, 3, 0) INTI_B(INTI_HELMHOLTZ, 3, 1)
+    INTI_B(kHelmCK, 3, 0) INTI_B(kHelmCK, 3, 1)
 #undef INTI_B
     return false;
 }
@@ -534,7 +535,7 @@ int bdim_build_types(Op *opS, Op *opD, int deriv, int ntypes, const int *nq, con
     for (int c : kThetaNs)
         if (c >= ns) { nst = c; break; }
     BdimFns fns;
-    if (!pick_bdim(opS->pde, dim, deriv, nst, fns))
+    if (!pick_bdim(op_kernel_pde(opS), dim, deriv, nst, fns))
         return set_error(ctx, INTI_ERR_UNSUPPORTED, "bdim: no kernel for this operator");
 
     // ---- host-side lists per type (element -> targets), concatenated
@@ -635,6 +636,7 @@ int bdim_build_types(Op *opS, Op *opD, int deriv, int ntypes, const int *nq, con
     INTI_HIP(ctx, up(dmu.p, mu, sizeof(double) * m));
 
     KParams kp = basic_params(opS->k);
+    kp.kimag = opS->kimag;
 
     // traces
     {

@@ -67,6 +67,7 @@ struct Op {
     Ctx *ctx = nullptr;
     int pde = 0, dim = 0, kind = 0, precision = 0;
     double k = 0.0;
+    double kimag = 0.0;  // imaginary part of k (3D Helmholtz, FP64, one-direction sums)
     double coefs[4] = {1, 0, 1, 0};  // c0, c1 (complex) of the combined kinds
     double cid[2] = {0, 0};          // identity coefficient (square ops)
     const Quad *trg = nullptr, *src = nullptr;
@@ -128,6 +129,9 @@ int op_adjoint(Op *op, const double *ubar, const double *sigma, double *sigma_ba
                double *nbar, double *wbar, double *xbar, const double *dcoefs, double *kbar);
 
 bool op_is_complex(const Op *op);
+// PDE code of the kernel templates: the complex-wavenumber 3D Helmholtz
+// kernels (2) for a Helmholtz operator with kimag != 0, else op->pde
+int op_kernel_pde(const Op *op);
 int microbench(Ctx *ctx, int op, double *rate);
 int dense_gemm(Ctx *ctx, int cplx, int64_t m, int64_t n, int nrhs, const double *A, int64_t lda, const double *X,
                int64_t ldx, double *Y, int64_t ldy, const double alpha[2], const double beta[2]);

@@ -164,6 +164,7 @@ static ApplyFn pick_apply(int pde, int dim, int kind, int mixed, int nr) {
     if (pde == INTI_LAPLACE && dim == 3) return apply_pick_lap3(kind, mixed, nr);
     if (pde == INTI_HELMHOLTZ && dim == 2) return apply_pick_helm2(kind, mixed, nr);
     if (pde == INTI_HELMHOLTZ && dim == 3) return apply_pick_helm3(kind, mixed, nr);
+    if (pde == kHelmCK && dim == 3 && !mixed) return apply_pick_helm3c(kind, nr);
     return nullptr;
 }
 
@@ -176,6 +177,7 @@ static SymFn pick_sym(int pde, int dim, int kind, int mixed, int G) {
 }
 
 bool op_is_complex(const Op *op) { return op->pde == INTI_HELMHOLTZ; }
+int op_kernel_pde(const Op *op) { return op->pde == INTI_HELMHOLTZ && op->kimag != 0.0 ? kHelmCK : op->pde; }
 
 // partial sums of the symmetric evaluation are capped: beyond this the
 // plain direct sum runs
@@ -185,7 +187,7 @@ constexpr size_t kSymMaxBytes = size_t(4) << 30;
 // (the smallest of 1, 2, 4 whose S x n partials fit kSymMaxBytes). False if
 // the operator does not qualify.
 static bool plan_sym(const Op *op, int kind, int nr, int &G, int64_t &T, int64_t &S) {
-    if (op->sym == 0 || nr != 1 || op->trg != op->src || kind >= INTI_FF_SL) return false;
+    if (op->sym == 0 || nr != 1 || op->trg != op->src || kind >= INTI_FF_SL || op->kimag != 0.0) return false;
     const int64_t n = op->src->n;
     T = (n + kSymBS - 1) / kSymBS;
     if (T < 2) return false;
@@ -208,6 +210,7 @@ static bool plan_sym(const Op *op, int kind, int nr, int &G, int64_t &T, int64_t
 KParams op_params(const Op *op, int *eff_kind) {
     using cd = std::complex<double>;
     KParams p = basic_params(op->k);
+    p.kimag = op->kimag;
     int kind = op->kind;
     const cd c0(op->coefs[0], op->coefs[1]), c1(op->coefs[2], op->coefs[3]);
     cd post, ch(0.0, 0.0);
@@ -295,7 +298,7 @@ int apply_reserve(const Op *op, int nrhs) {
         if (nrhs < nr && nr != 1) continue;
         int split;
         int64_t chunk;
-        plan_split(op, m, n, pick_apply(op->pde, op->dim, kind, op->precision == INTI_MIXED, nr), nr, split, chunk);
+        plan_split(op, m, n, pick_apply(op_kernel_pde(op), op->dim, kind, op->precision == INTI_MIXED, nr), nr, split, chunk);
         bytes = std::max(bytes, sizeof(double) * cw * (size_t)m * split * nr);
     }
     if (nrhs >= 1 && plan_sym(op, kind, 1, G, T, S))
@@ -339,7 +342,7 @@ int launch_apply(const DirectLaunch &L) {
         const int nr = left >= 16 ? 16 : left >= 4 ? 4 : 1;
         const double *xr = L.x + (int64_t)r0 * L.ldx * cw;
         double *yr = L.y + (int64_t)r0 * L.ldy * cw;
-        ApplyFn fn = sym && nr == 1 ? nullptr : pick_apply(op->pde, op->dim, kind, op->precision == INTI_MIXED, nr);
+        ApplyFn fn = sym && nr == 1 ? nullptr : pick_apply(op_kernel_pde(op), op->dim, kind, op->precision == INTI_MIXED, nr);
         int split = 1;
         int64_t chunk = n;
         if (fn) plan_split(op, m, n, fn, nr, split, chunk);

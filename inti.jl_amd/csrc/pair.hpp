@@ -17,8 +17,13 @@
 namespace inti {
 
 // Per-operator constants seen by the pair kernels.
+// internal PDE code of the 3D Helmholtz kernels with a complex wavenumber
+// k + i kimag (an operator with a nonzero imaginary part is routed to it)
+constexpr int kHelmCK = 2;
+
 struct KParams {
-    double k;          // wavenumber
+    double k;          // wavenumber (real part)
+    double kimag;      // imaginary part of the wavenumber (kHelmCK only)
     double kappa;      // k / (2 pi): phase in revolutions per unit length
     double kappa_half; // kappa / 2
     double chr, chi;   // normalised combined-field coefficient (see post_factor)
@@ -122,6 +127,55 @@ __device__ __forceinline__ void helm3_raw(const double *r, double d2s, bool ok, 
         kr = cf * br - sf * bi;
         ki = cf * bi + sf * br;
     }
+}
+
+// 3D Helmholtz with a complex wavenumber k = kr + i kc (kernels.jl:322-345
+// take any k::Number; FP64 only). The same raw layout as helm3_raw, raw =
+// e^{i kr d} e^{-kc d} f B, with every occurrence of ik in B written as
+// -kc + i kr and k^2 as kr^2 - kc^2 + 2 i kr kc:
+//   DL, ADL  B = a (1/d - ik),                 a = (r.n) / d^2
+//   HS       B = nx.ny (1/d - ik) + g (k^2 d^2 + 3 ik d - 3),
+//                                              g = (r.nx)(r.ny) / d^3
+// (G'(d) = G (ik - 1/d), G''(d) = G ((ik - 1/d)^2 + 1/d^2)).
+template <int KIND>
+__device__ __forceinline__ void helm3c_raw(const double *r, double d2s, bool ok, const double *nx,
+                                           const double *ny, const KParams &p, double &kr,
+                                           double &ki) {
+    const double k = p.k, kc = p.kimag;
+    const double id = rsqp<false>(d2s);
+    const double d = d2s * id;
+    double s, c;
+    sincos_rev<false>(p.kappa * d, s, c);
+    const double damp = exp(-kc * d);
+    double br, bi, f;
+    if constexpr (KIND == INTI_SL) {
+        br = 1.0; bi = 0.0; f = mask(ok, id);
+    } else if constexpr (KIND == INTI_DL || KIND == INTI_ADL) {
+        const double a = mask(ok, dotv<3>(r, KIND == INTI_DL ? ny : nx) * id * id);
+        br = a * (id + kc); bi = -k * a; f = 1.0;
+    } else if constexpr (KIND == INTI_HS || KIND == INTI_COMB_AH) {
+        const double nxny = dotv<3>(nx, ny);
+        const double rdnx = dotv<3>(r, nx);
+        const double g = rdnx * dotv<3>(r, ny) * id * id * id;
+        br = nxny * (id + kc) + g * ((k * k - kc * kc) * d2s - 3.0 * kc * d - 3.0);
+        bi = -k * nxny + g * (3.0 * k * d + 2.0 * k * kc * d2s);
+        if constexpr (KIND == INTI_COMB_AH) {  // + ch * (-(r.nx) (1/d - ik)), see helm3_raw
+            const double ar = -rdnx * (id + kc), ai = k * rdnx;
+            br += p.chr * ar - p.chi * ai;
+            bi += p.chr * ai + p.chi * ar;
+        }
+        f = mask(ok, id * id);
+    } else {  // COMB_SD: DL + ch*SL = e*id*[ch + a (1/d - ik)], a = (r.ny)/d
+        static_assert(KIND == INTI_COMB_SD, "kinds of helm3c_raw");
+        const double a = dotv<3>(r, ny) * id;
+        br = p.chr + a * (id + kc);
+        bi = p.chi - k * a;
+        f = mask(ok, id);
+    }
+    f *= damp;
+    const double cf = c * f, sf = s * f;
+    kr = cf * br - sf * bi;
+    ki = cf * bi + sf * br;
 }
 
 // MIXED 3D Helmholtz SL, DL, ADL and DL + ch*SL with the fewest VALU
@@ -280,6 +334,9 @@ __device__ __forceinline__ void pair_raw(const double *r, const double *nx, cons
     } else if constexpr (PDE == INTI_LAPLACE) {
         kr = laplace_raw<DIM, KIND, MIXED>(r, d2s, ok, nx, ny, p);
         ki = 0.0;
+    } else if constexpr (PDE == kHelmCK) {
+        static_assert(DIM == 3 && !MIXED, "complex wavenumbers: 3D, FP64");
+        helm3c_raw<KIND>(r, d2s, ok, nx, ny, p, kr, ki);
     } else if constexpr (DIM == 3 && MIXED &&
                          (KIND == INTI_SL || KIND == INTI_DL || KIND == INTI_ADL ||
                           KIND == INTI_COMB_SD)) {
@@ -429,6 +486,7 @@ __device__ __forceinline__ void helm3_fused_one(const double *r, const double *n
 inline KParams basic_params(double k) {
     KParams p;
     p.k = k;
+    p.kimag = 0.0;
     p.kappa = k * kInv2Pi;
     p.kappa_half = 0.5 * p.kappa;
     p.chr = p.chi = 0.0;

@@ -75,7 +75,9 @@ kind_id(::Inti.HyperSingularKernel) = HS
 pde_id(::Inti.Laplace) = LAPLACE
 pde_id(::Inti.Helmholtz) = HELMHOLTZ
 wavenumber(::Inti.Laplace) = 0.0
-wavenumber(op::Inti.Helmholtz) = Float64(op.k)
+wavenumber(op::Inti.Helmholtz) = Float64(real(op.k))
+wavenumber_imag(::Inti.Laplace) = 0.0
+wavenumber_imag(op::Inti.Helmholtz) = Float64(imag(op.k))
 
 const QUADS = IdDict{Any,Ptr{Cvoid}}()
 # one device operator per (IntegralOperator, precision): a mixed-precision
@@ -102,6 +104,10 @@ function op_handle(iop::Inti.IntegralOperator; precision = FP64)
          Ref{Ptr{Cvoid}}),
         CTX[].ptr, pde_id(op), kind_id(K), N, wavenumber(op), C_NULL,
         quad_handle(iop.target), quad_handle(iop.source), precision, h))
+    ki = wavenumber_imag(op)
+    if ki != 0   # Helmholtz(; k) with a complex k (3D, FP64)
+        check(CTX[].ptr, ccall((:inti_op_set_wavenumber_imag, libinti), Cint, (Ptr{Cvoid}, Cdouble), h[], ki))
+    end
     HANDLES[key] = (iop, h[])   # the operator is kept alive with its handle
     return h[]
 end

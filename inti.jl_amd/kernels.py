@@ -3,6 +3,8 @@
 Only the Laplace (:118-204) and Helmholtz (:310-413) operators in 2D/3D are
 provided; values are computed on the GPU by libinti_amd.
 """
+import numpy as np
+
 from . import _lib
 
 
@@ -24,23 +26,33 @@ class Laplace:
 class Helmholtz:
     """Helmholtz(; k, dim): -Delta u - k^2 u (kernels.jl:322-335).
 
-    Complex k with zero imaginary part is taken as real; a genuinely complex
-    or purely imaginary k (the reference's Yukawa remap) is not supported by
-    the GPU kernels and raises."""
+    k may be complex (kernels.jl takes any k::Number): k = kr + i ki with
+    kr > 0 runs the 3D kernels in FP64 (one-direction sums; no mixed
+    precision, far fields, batches or pullbacks). In 2D a complex k needs
+    the Hankel function of a complex argument and raises, and so does a
+    purely imaginary k (the reference's Yukawa remap). A complex k with zero
+    imaginary part is taken as real."""
 
     pde = _lib.HELMHOLTZ
 
     def __init__(self, *, k, dim):
         if dim not in (2, 3):
             raise ValueError("dim must be 2 or 3")
-        if isinstance(k, complex):
-            if k.imag != 0:
-                raise NotImplementedError("complex wavenumbers are not supported on the GPU path")
-            k = k.real
-        if not k > 0:
-            raise ValueError("k must be positive")
+        kr, ki = (k.real, k.imag) if isinstance(k, complex) else (k, 0.0)
+        if ki != 0 and dim != 3:
+            raise NotImplementedError("complex wavenumbers: 3D only (2D needs complex-argument Hankel functions)")
+        if not kr > 0:
+            raise ValueError("k must have a positive real part")
+        if not np.isfinite(ki):
+            raise ValueError("k must be finite")
         self.dim = dim
-        self.k = float(k)
+        self.kr = float(kr)
+        self.ki = float(ki)
+        self.k = complex(self.kr, self.ki) if self.ki != 0 else self.kr
+
+    @property
+    def is_complex_k(self):
+        return self.ki != 0
 
     def __repr__(self):
         return f"Helmholtz operator in {self.dim} dimensions: -Δu - k²u"

@@ -147,10 +147,16 @@ class IntegralOperator:
                 coefs = np.array([k.c0.real, k.c0.imag, k.c1.real, k.c1.imag])
             h = C.c_void_p()
             prec = _lib.MIXED if self.precision == "mixed" else _lib.FP64
+            kr, ki = (k.op.kr, k.op.ki) if hasattr(k.op, "kr") else (k.op.k, 0.0)
             self.ctx.check(self.ctx.lib.inti_op_create(
-                self.ctx.handle, k.op.pde, k.kind, k.op.dim, k.op.k,
+                self.ctx.handle, k.op.pde, k.kind, k.op.dim, kr,
                 _lib.dptr(coefs), self.target.device(self.ctx), self.source.device(self.ctx),
                 prec, C.byref(h)))
+            if ki != 0:
+                st = self.ctx.lib.inti_op_set_wavenumber_imag(h, ki)
+                if st != 0:
+                    self.ctx.lib.inti_op_destroy(h)
+                    self.ctx.check(st)
             self._h = h
             if _SYMMETRIC_DEFAULT is not None:
                 self.ctx.check(self.ctx.lib.inti_op_set_symmetric(h, _SYMMETRIC_DEFAULT, 0))

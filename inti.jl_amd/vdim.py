@@ -42,6 +42,8 @@ def vdim_correction(op, target, source, boundary, Sop, Dop, Vop, *, green_multip
     (vdim.jl:30-129). Sop/Dop map the boundary to the targets (already
     corrected), Vop maps the volume source to the targets. Returns a
     SparseCorrection."""
+    if getattr(op, "ki", 0.0) != 0:
+        raise NotImplementedError("vdim_correction: real wavenumbers only")
     mu = np.ascontiguousarray(green_multiplier, dtype=np.float64)
     if mu.shape != (len(target),):
         raise ValueError("green_multiplier must have one entry per target")
@@ -89,6 +91,8 @@ def volume_potential(*, op, target, source, compression=None, correction=None):
     default) or {"method": "none"} (dense)."""
     from .api import (CorrectedOperator, _normalize_compression, _normalize_correction,
                       single_double_layer)
+    if getattr(op, "ki", 0.0) != 0:
+        raise NotImplementedError("volume_potential: real wavenumbers only")
     compression = _normalize_compression(compression or {"method": "gpu"})
     correction = _normalize_correction(correction or {"method": "none"}, target, source)
     V = IntegralOperator(SingleLayerKernel(op), target, source)
