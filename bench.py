@@ -734,13 +734,21 @@ def run_shape(inti, lib, ctx, args):
         rng = np.random.default_rng(1)
         sig = rng.normal(size=n) + 1j * rng.normal(size=n)
         A = inti.IntegralOperator(inti.CombinedFieldKernel(op, -1j * k, 1.0), Q)
+        A.apply(sig)  # warm-up of both kernels
+        A.pullback(sig, sig, wrt=("sigma", "nodes", "source_normals", "source_weights", "k"), dcoefs=(-1j, 0.0))
+        tf, tc, ts = [], [], []
         for _ in range(3):
             A.apply(sig)
+            tf.append(ctx.last_kernel_ms())
             A.pullback(sig, sig, wrt=("sigma", "nodes", "source_normals", "source_weights", "k"), dcoefs=(-1j, 0.0))
+            tc.append(ctx.last_kernel_ms())
             A.pullback(sig, sig, wrt=("sigma", "source_nodes", "source_normals", "source_weights",
                                       "target_nodes", "k"), dcoefs=(-1j, 0.0))
+            ts.append(ctx.last_kernel_ms())
         ctx.synchronize()
-        log("bench shape: profiling legs done")
+        f, c_, s_ = (float(np.median(t)) for t in (tf, tc, ts))
+        log(f"bench shape: profiling legs done: forward {f:.3f} ms, combined pullback {c_:.3f} ms ({c_ / f:.3f}x), "
+            f"separate outputs {s_:.3f} ms ({s_ / f:.3f}x)")
         return None
     t0 = time.perf_counter()
     J0, grad, info = shape_gradient(inti, Q.coords, Q.normals, Q.weights, Q.qtags, k, dirs)

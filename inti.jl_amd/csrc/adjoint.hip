@@ -79,7 +79,7 @@ __device__ __forceinline__ void radial(double d2, const AdjParams &p, double2 &g
     } else if constexpr (DIM == 3) {
         const double k = p.k;
         double s, c;
-        sincos_rev<false>(p.kappa * d, s, c);
+        sincos_rev64_vop3(p.kappa * d, s, c);
         const double2 E = c2(kInv4Pi * c, kInv4Pi * s);
         const double id3 = id2 * id;
         if constexpr (WANT_G) {
@@ -126,7 +126,9 @@ struct AdjRadial {
 // tolerance (distinct nodes, different nodes): no clamp, no mask.
 // CHIK: the combined field of the scattering examples, c0 = -i k, c1 = 1,
 // dc0/dk = -i, dc1/dk = 0 (3D Helmholtz), with the coefficient products
-// written out instead of taken as general complex products.
+// written out instead of taken as general complex products, and without the
+// factor 1/(4 pi) of E: every output of the pair is linear in E, and
+// k_adj_sym applies it once to its sums (OSC).
 template <int PDE, int DIM, int KIND, bool MASK = true, bool CHIK = false>
 __device__ __forceinline__ AdjRadial adj_radial(double d2, const AdjParams &p) {
     constexpr bool WG = KIND != INTI_DL, WH = KIND != INTI_SL;
@@ -137,8 +139,8 @@ __device__ __forceinline__ AdjRadial adj_radial(double d2, const AdjParams &p) {
         const double d2s = MASK ? fmax(d2, kClampD2) : d2;
         const double id = rsq64(d2s), d = d2s * id, id2 = id * id, id3 = id2 * id;
         double s, c;
-        sincos_rev<false>(p.kappa * d, s, c);
-        const double2 E = c2(kInv4Pi * c, kInv4Pi * s);
+        sincos_rev64_vop3(p.kappa * d, s, c);
+        const double2 E = c2(c, s);  // e^{ikd}; the 1/(4 pi) goes to the sums
         const double k = p.k, kd = k * d, kid = k * id;
         const double2 E0 = c2(k * E.y, -k * E.x);  // -i k E
         R.g = cs(E0, id);
@@ -157,7 +159,7 @@ __device__ __forceinline__ AdjRadial adj_radial(double d2, const AdjParams &p) {
         const double d2s = MASK ? fmax(d2, kClampD2) : d2;
         const double id = rsq64(d2s), d = d2s * id, id2 = id * id, id3 = id2 * id;
         double s, c;
-        sincos_rev<false>(p.kappa * d, s, c);
+        sincos_rev64_vop3(p.kappa * d, s, c);
         const double2 E = c2(kInv4Pi * c, kInv4Pi * s);
         const double2 E0 = cm(p.c0, E), E1 = cm(p.c1, E), D0 = cm(p.dc0, E), D1 = cm(p.dc1, E);
         const double kd = p.k * d, kid = p.k * id;
@@ -225,7 +227,7 @@ __device__ __forceinline__ void pair_grad(const double *x, const double *y, cons
         const double k = p.k;
         const double xy = dotv<DIM>(x, y);
         double s, c;
-        sincos_rev<false>(-p.kappa * xy, s, c);
+        sincos_rev64_vop3(-p.kappa * xy, s, c);
         const double2 E = c2(c, s);  // e^{-ik xh.y}
         const double2 PE = cm(p.P, E);
         const double2 mixy = c2(0.0, -xy);  // -i xh.y
@@ -536,6 +538,7 @@ __device__ __forceinline__ void lds_add(double *a, double v) {
 template <int PDE, int DIM, int KIND, bool COMB, bool CHIK = false>
 __global__ __launch_bounds__(kSymBS) __attribute__((amdgpu_waves_per_eu(COMB ? 3 : 1))) void k_adj_sym(AdjSymArgs A) {
     constexpr int NR = COMB ? 2 * DIM + 2 : 3 * DIM + 2;
+    constexpr double OSC = CHIK ? kInv4Pi : 1.0;  // see adj_radial (CHIK)
     constexpr int NW = kSymBS / 64;
     constexpr int iY = 2, iX = COMB ? 2 : 2 + DIM, iN = COMB ? 2 + DIM : 2 + 2 * DIM;
     __shared__ double s_x[DIM][kSymBS], s_n[DIM][kSymBS];
@@ -649,7 +652,7 @@ __global__ __launch_bounds__(kSymBS) __attribute__((amdgpu_waves_per_eu(COMB ? 3
                     double v = s_acc[0][r][lane];
 #pragma unroll
                     for (int w = 1; w < NW; ++w) v += s_acc[w][r][lane];
-                    A.part[(a * NR + r) * n + j] = v;
+                    A.part[(a * NR + r) * n + j] = OSC * v;
                 }
             }
         }
@@ -662,7 +665,7 @@ __global__ __launch_bounds__(kSymBS) __attribute__((amdgpu_waves_per_eu(COMB ? 3
         const int64_t slot = diag ? a : b;
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            double v = own[r];
+            double v = OSC * own[r];
             if (diag) v += A.part[(slot * NR + r) * n + i];
             A.part[(slot * NR + r) * n + i] = v;
         }
@@ -673,7 +676,7 @@ __global__ __launch_bounds__(kSymBS) __attribute__((amdgpu_waves_per_eu(COMB ? 3
         if (tid < off) s_k[tid] += s_k[tid + off];
         __syncthreads();
     }
-    if (tid == 0) A.kpart[blockIdx.x] = s_k[0];
+    if (tid == 0) A.kpart[blockIdx.x] = OSC * s_k[0];
 }
 
 // per node: sum the S slots and form the outputs

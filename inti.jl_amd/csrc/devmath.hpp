@@ -98,6 +98,34 @@ __device__ __forceinline__ void sincos_rev64(double u, double &s, double &c) {
     c = ((qi ^ (qi >> 1)) & 1) ? -c0 : c0;
 }
 
+// The same (sin 2 pi u, cos 2 pi u) with every Horner step issued as one
+// three-address v_fma_f64. Where many values are live (the pullback kernels)
+// the register allocator otherwise keeps the coefficients in VGPRs and
+// turns each step into a copy of the coefficient plus a two-address
+// v_fmac_f64 (14 extra moves per evaluation).
+__device__ __forceinline__ double fma_vop3(double a, double b, double c) {
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+__device__ __forceinline__ void sincos_rev64_vop3(double u, double &s, double &c) {
+    const double q = __builtin_rint(4.0 * u);
+    const double f = __builtin_fma(-0.25, q, u);
+    const double z = f * f;
+    double ps = kSinRev[7];
+#pragma unroll
+    for (int k = 6; k >= 0; --k) ps = fma_vop3(ps, z, kSinRev[k]);
+    double pc = kCosRev[8];
+#pragma unroll
+    for (int k = 7; k >= 0; --k) pc = fma_vop3(pc, z, kCosRev[k]);
+    const double S = f * ps, C = pc;
+    const int qi = (int)q & 3;
+    const double s0 = (qi & 1) ? C : S;
+    const double c0 = (qi & 1) ? S : C;
+    s = (qi & 2) ? -s0 : s0;
+    c = ((qi ^ (qi >> 1)) & 1) ? -c0 : c0;
+}
+
 // (sin 2 pi u, cos 2 pi u) for a phase u given in revolutions. FP64:
 // sincos_rev64. MIXED: u is reduced to [-1/2, 1/2] in FP64 and fed to the FP32
 // sin/cos unit, whose argument is in revolutions.
