@@ -398,12 +398,13 @@ def run_batch(inti, lib, ctx, args):
     qs, a, b, ks, rng = batch_workload(inti, P)
     ops = [inti.Helmholtz(k=ks[p], dim=2) for p in range(P)]
     t0 = time.perf_counter()
-    corr = []
+    corr, corr_s = [], []
     for p in range(P):
         Sop = inti.IntegralOperator(inti.SingleLayerKernel(ops[p]), qs[p])
         Dop = inti.IntegralOperator(inti.DoubleLayerKernel(ops[p]), qs[p])
-        corr.append(inti.bdim_correction(ops[p], qs[p], qs[p], Sop, Dop,
-                                         green_multiplier=np.full(len(qs[p]), -0.5))[1])
+        dS, dD = inti.bdim_correction(ops[p], qs[p], qs[p], Sop, Dop, green_multiplier=np.full(len(qs[p]), -0.5))
+        corr.append(dD)
+        corr_s.append(dS)
     ctx.synchronize()
     t_dim = time.perf_counter() - t0
     Ks = [inti.DoubleLayerKernel(o) for o in ops]
@@ -498,6 +499,28 @@ def run_batch(inti, lib, ctx, args):
         f"{t_rf:.2f} s, {int(rconv.sum())}/{P} problems converged to 1e-10; vs the FP64 GMRES solutions "
         f"(problems both converged): median {np.median(diffs):.2e}, max {agree:.2e}, "
         f"{int((diffs <= 1e-10).sum())}/{len(diffs)} within 1e-10")
+    # resonance check: I/2 + D alone (the exterior Dirichlet problem with a
+    # double-layer density) is singular at the interior Neumann eigenvalues
+    # of each ellipse; the combined field I/2 + D - i k S has no such k. The
+    # same draws, right-hand sides and GMRES(100) with this operator tell
+    # resonant draws from solver trouble.
+    Kc = [inti.CombinedFieldKernel(o, -1j * o.k, 1.0) for o in ops]
+    Bc = inti.BatchOperator(Kc, qs)
+    Bc.set_identity(0.5)
+    for p in range(P):
+        Bc.attach(p, corr_s[p], -1j * ops[p].k)
+        Bc.attach(p, corr[p], 1.0)
+    Bc.autotune(nrhs=R, reps=1)
+    t0 = time.perf_counter()
+    _, cinfo = Bc.gmres(Xh, restart=100, maxiter=600, rtol=1e-10)
+    t_cf = time.perf_counter() - t0
+    cconv = cinfo["relres"].max(axis=1) <= 1e-10
+    ka = ks * a  # wavenumber x semi-major axis
+    log(f"bench batch: combined field I/2 + D - i k S: gmres {cinfo['iterations']} steps in {t_cf:.2f} s, "
+        f"{int(cconv.sum())}/{P} problems converged to 1e-10 (of the {int((~conv).sum())} that I/2 + D left "
+        f"unconverged: {int((cconv & ~conv).sum())}); median relres {np.median(cinfo['relres']):.2e}; "
+        f"median k a: converged {np.median(ka[cconv]) if cconv.any() else 0:.1f}, unconverged "
+        f"{np.median(ka[~cconv]) if (~cconv).any() else 0:.1f}")
     ctx.check(lib.inti_device_free(ctx.handle, dx))
     ctx.check(lib.inti_device_free(ctx.handle, dy))
     cpu = None
@@ -539,6 +562,13 @@ def run_batch(inti, lib, ctx, args):
                           "rel_diff_vs_fp64_gmres_solution": {"median": float(np.median(diffs)), "max": agree,
                                                               "within_1e-10": int((diffs <= 1e-10).sum()),
                                                               "problems": int(len(diffs))}},
+        "combined_field_gmres": {"operator": "I/2 + D - i k S (no interior resonances)", "steps": cinfo["iterations"],
+                                 "seconds": round(t_cf, 3), "problems_converged": int(cconv.sum()),
+                                 "of_those_unconverged_with_dl": int((cconv & ~conv).sum()),
+                                 "median_relres": float(np.median(cinfo["relres"])),
+                                 "median_k_times_a": {"converged": float(np.median(ka[cconv])) if cconv.any() else None,
+                                                      "unconverged": float(np.median(ka[~cconv])) if (~cconv).any()
+                                                      else None}},
         "cpu_baseline": cpu,
     }
 
