@@ -570,6 +570,8 @@ __global__ __launch_bounds__(kSymBS) __attribute__((amdgpu_waves_per_eu(COMB ? 3
 #pragma unroll
     for (int r = 0; r < NR; ++r) own[r] = 0.0;
     double kb = 0.0;
+    // unrolled: the sub-tile offsets u * 64 become immediate LDS offsets
+#pragma unroll
     for (int u = 0; u < 4; ++u) {
         __syncthreads();  // the previous sub-tile's sums have been read; s_x.. are written
         double *acc = &s_acc[wave][0][0];
@@ -578,23 +580,30 @@ __global__ __launch_bounds__(kSymBS) __attribute__((amdgpu_waves_per_eu(COMB ? 3
         // off-diagonal tiles of distinct nodes hold no coincident pairs
         auto pairs = [&](auto mask_tag) {
             constexpr bool MASK = decltype(mask_tag)::value;
+            // source js = (lane + s) mod 64 of the sub-tile, kept as the byte
+            // offset o8 = 8 js (two integer operations per step)
+            int o8 = lane * 8;
             for (int s = 0; s < 64; ++s) {
-                const int js = (lane + s) & 63, jl = u * 64 + js;
+                const int js8 = o8;
+                o8 = (o8 + 8) & 511;
                 double xj[DIM], nj[DIM], v[DIM], vn[DIM];
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
-                    xj[d] = s_x[d][jl];
-                    nj[d] = s_n[d][jl];
+                    xj[d] = *reinterpret_cast<const double *>(reinterpret_cast<const char *>(&s_x[d][u * 64]) + js8);
+                    nj[d] = *reinterpret_cast<const double *>(reinterpret_cast<const char *>(&s_n[d][u * 64]) + js8);
                     v[d] = xi[d] - xj[d];
                     vn[d] = -v[d];
                 }
-                const double2 cj = s_c[jl], sj = s_s[jl];
+                const double2 cj =
+                    *reinterpret_cast<const double2 *>(reinterpret_cast<const char *>(&s_c[u * 64]) + 2 * js8);
+                const double2 sj =
+                    *reinterpret_cast<const double2 *>(reinterpret_cast<const char *>(&s_s[u * 64]) + 2 * js8);
                 const AdjRadial R = adj_radial<PDE, DIM, KIND, MASK, CHIK>(dotv<DIM>(v, v), p);
                 // i target, j source: g1 = al1 v + be1 nj
                 double2 cK1;
                 double al1, be1, ga1;
                 kb += adj_sym_terms<PDE, DIM, KIND, MASK>(v, nj, R, p, ci, sj, cK1, al1, be1, ga1);
-                double *aj = acc + js;
+                double *aj = reinterpret_cast<double *>(reinterpret_cast<char *>(acc) + js8);
                 lds_add(aj + 0 * 64, cK1.x);
                 lds_add(aj + 1 * 64, cK1.y);
 #pragma unroll
