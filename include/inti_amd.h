@@ -242,6 +242,18 @@ int inti_batch_info(const inti_batch *batch, int64_t *ntot, int64_t *offsets);
 /* identity coefficients, 2 doubles (re, im) per problem */
 int inti_batch_set_identity(inti_batch *batch, const double *cid);
 /* add coef * csr (problem-local indices, n_p x n_p) to problem p */
+/* The DIM corrections (dS, dD) of every problem of the batch, as
+ * bdim_correction (src/bdim.jl:50-196) gives them for each problem's
+ * quadrature with itself (the self pattern, one element type of nq nodes):
+ * ne[p] elements per problem, qtags the column-major (nq, ne[p]) node
+ * tables of the problems one after another (0-based within each problem),
+ * mu the green multiplier of every node (ntot), params the DIM parameters
+ * (oversampling, radius multiplier; NULL for 3, 1.5). dS[p], dD[p] receive
+ * one CSR per problem. The batch's quadratures need normals (DL or combined
+ * batches). Replaces nprob separate bdim_correction calls (the reference
+ * loops over problems). */
+int inti_batch_bdim_build(inti_batch *batch, int nq, const int64_t *ne, const int64_t *qtags,
+                          const double *mu, const double *params, inti_csr **dS, inti_csr **dD);
 int inti_batch_attach_correction(inti_batch *batch, int p, const inti_csr *csr,
                                  double coef_re, double coef_im);
 /* source chunk length of a work item (0 = 2048) and persistent grid size

@@ -69,6 +69,7 @@ SIGNATURES = {
     "inti_batch_info": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "inti_batch_set_identity": (C.c_int, [_vp, _dp]),
     "inti_batch_attach_correction": (C.c_int, [_vp, C.c_int, _vp, C.c_double, C.c_double]),
+    "inti_batch_bdim_build": (C.c_int, [_vp, C.c_int, _i64p, _i64p, _dp, _dp, C.POINTER(_vp), C.POINTER(_vp)]),
     "inti_batch_set_launch": (C.c_int, [_vp, C.c_int64, C.c_int]),
     "inti_batch_apply_device": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64, _dp, _dp]),
     "inti_batch_apply": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),

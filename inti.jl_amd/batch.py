@@ -76,6 +76,34 @@ class BatchOperator:
         cid = np.ascontiguousarray(np.stack([c.real, c.imag], axis=1).reshape(-1))
         self.ctx.check(self.ctx.lib.inti_batch_set_identity(self.handle, _lib.dptr(cid)))
 
+    def bdim_correction(self, *, green_multiplier=None, parameters=None):
+        """The DIM corrections of every problem's quadrature with itself
+        (bdim_correction, bdim.jl:50-196, per problem: self pattern, one
+        element type shared by the problems) in one pass over the batch:
+        traces in one launch, Theta by two batched applies, the least
+        squares of all elements in one launch. green_multiplier: one value
+        for all nodes (default -1/2) or one per node (ntot). Returns [(dS,
+        dD)] per problem, SparseCorrection objects."""
+        from .bdim import DimParameters
+        from .nystrom import SparseCorrection
+        nq = {q.qtags.shape[0] for q in self.quads}
+        if len(nq) != 1 or any(len(q.etype_qtags) != 1 for q in self.quads):
+            raise NotImplementedError("bdim_correction of a batch: one element type, the same nq in every problem")
+        nq = nq.pop()
+        ne = np.ascontiguousarray([q.qtags.shape[1] for q in self.quads], dtype=np.int64)
+        tags = np.ascontiguousarray(np.concatenate([q.qtags.T.reshape(-1) for q in self.quads]), dtype=np.int64)
+        mu = np.full(self.ntot, -0.5) if green_multiplier is None else np.broadcast_to(
+            np.asarray(green_multiplier, dtype=np.float64), (self.ntot,))
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        params = parameters or DimParameters()
+        pv = np.array([params.sources_oversample_factor, params.sources_radius_multiplier])
+        hS = (C.c_void_p * self.nprob)()
+        hD = (C.c_void_p * self.nprob)()
+        self.ctx.check(self.ctx.lib.inti_batch_bdim_build(self.handle, nq, _lib.i64ptr(ne), _lib.i64ptr(tags),
+                                                          _lib.dptr(mu), _lib.dptr(pv), hS, hD))
+        return [(SparseCorrection(self.ctx, C.c_void_p(hS[p])), SparseCorrection(self.ctx, C.c_void_p(hD[p])))
+                for p in range(self.nprob)]
+
     def attach(self, p, corr, coef=1.0):
         """Add coef * corr (a SparseCorrection of problem p) to problem p."""
         coef = complex(coef)

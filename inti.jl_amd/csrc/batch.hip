@@ -19,6 +19,7 @@
 
 #include "apply_kernel.hpp"
 #include "common.hpp"
+#include "lsq.hpp"
 
 namespace inti {
 
@@ -952,6 +953,279 @@ int inti_batch_set_identity(inti_batch *B, const double *cid) {
     std::memcpy(B->cid.data(), cid, sizeof(double) * 2 * B->nprob);
     INTI_HIP(ctx, hipMemcpyAsync(B->cid_d, B->cid.data(), sizeof(double) * 2 * B->nprob, hipMemcpyHostToDevice, ctx->stream));
     INTI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return INTI_OK;
+}
+
+namespace inti {
+
+// real <-> complex layouts of a column block (Laplace batches apply real
+// vectors; the least squares read complex ones)
+__global__ void k_real_part(int64_t n, const double2 *z, double *x) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) x[i] = z[i].x;
+}
+__global__ void k_widen(int64_t n, const double *x, double2 *z) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) z[i] = make_double2(x[i], 0.0);
+}
+
+// The DIM corrections (dS, dD) of every problem of B (bdim.jl:50-196 per
+// problem: quadrature with itself, the self pattern, one element type) in
+// one pass over the batch: the traces of all problems in one launch (each
+// against its own monopoles and wavenumber), Theta = mu B0 + S B1 - D B0 by
+// two batched applies of the problems' uncorrected SL and DL operators
+// (ns right-hand sides, the persistent launches of k_apply_batch), the
+// least squares of all elements in one launch, and the triplets summed into
+// one block-diagonal CSR that is then cut into one CSR per problem.
+static int batch_bdim_build(Batch *B, int nq, const int64_t *ne, const int64_t *qtags, const double *mu,
+                            const double *params, Csr **outS, Csr **outD) {
+    Ctx *ctx = B->ctx;
+    const int P = B->nprob, dim = B->dim;
+    const int64_t ntot = B->ntot;
+    const double oversample = params ? params[0] : 3.0, radius_mult = params ? params[1] : 1.5;
+    const int ns = (int)std::ceil(oversample * nq);
+    if (ns > kMaxNs || 2 * nq > kMaxNc || ns < 2 * nq)
+        return set_error(ctx, INTI_ERR_UNSUPPORTED, "batch_bdim: need 2*nq <= ns <= 32 and 2*nq <= 24");
+    if (!B->normals) return set_error(ctx, INTI_ERR_ARG, "batch_bdim: the batch's quadratures need normals");
+    // element tags, global; the self pattern covers every node once
+    int64_t ne_tot = 0;
+    for (int p = 0; p < P; ++p) {
+        if (ne[p] < 1) return set_error(ctx, INTI_ERR_ARG, "batch_bdim: every problem needs elements");
+        if ((int64_t)nq * ne[p] != B->off[p + 1] - B->off[p])
+            return set_error(ctx, INTI_ERR_SIZE, "batch_bdim: the elements of a problem must cover its nodes once");
+        ne_tot += ne[p];
+    }
+    std::vector<int32_t> qt((size_t)nq * ne_tot);
+    {
+        int64_t q = 0;
+        for (int p = 0; p < P; ++p) {
+            const int64_t n = B->off[p + 1] - B->off[p];
+            for (int64_t t = 0; t < (int64_t)nq * ne[p]; ++t, ++q) {
+                if (qtags[q] < 0 || qtags[q] >= n) return set_error(ctx, INTI_ERR_ARG, "batch_bdim: qtags out of range");
+                qt[q] = (int32_t)(B->off[p] + qtags[q]);
+            }
+        }
+    }
+    hipStream_t st = ctx->stream;
+    // monopoles per problem around its bounding box
+    std::vector<double> hx((size_t)dim * ntot);
+    INTI_HIP(ctx, hipMemcpyAsync(hx.data(), B->coords, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, st));
+    INTI_HIP(ctx, hipStreamSynchronize(st));
+    std::vector<double> zs_all((size_t)P * dim * ns), zs;
+    for (int p = 0; p < P; ++p) {
+        double lo[3], hi[3];
+        for (int d = 0; d < dim; ++d) {
+            lo[d] = INFINITY; hi[d] = -INFINITY;
+            for (int64_t j = B->off[p]; j < B->off[p + 1]; ++j) {
+                lo[d] = std::min(lo[d], hx[(size_t)d * ntot + j]);
+                hi[d] = std::max(hi[d], hx[(size_t)d * ntot + j]);
+            }
+        }
+        monopoles_box(dim, lo, hi, ns, radius_mult, zs);
+        std::copy(zs.begin(), zs.end(), zs_all.begin() + (size_t)p * dim * ns);
+    }
+    // device buffers
+    const bool cplx = B->pde == INTI_HELMHOLTZ;
+    const size_t nb = (size_t)ntot * ns;
+    DevBuf dzs, dkp, dmu, dB0, dB1, dth, dqt, dpp, del, dtriS, dtriD, ddiag, dseg, dperm, dvS, dvD, dr0, dr1, drt;
+    INTI_ALLOC(ctx, dzs, sizeof(double) * zs_all.size());
+    INTI_ALLOC(ctx, dkp, sizeof(KParams) * P);
+    INTI_ALLOC(ctx, dmu, sizeof(double) * ntot);
+    INTI_ALLOC(ctx, dB0, sizeof(double2) * nb);
+    INTI_ALLOC(ctx, dB1, sizeof(double2) * nb);
+    INTI_ALLOC(ctx, dth, sizeof(double2) * nb);
+    INTI_HIP(ctx, hipMemcpyAsync(dzs.p, zs_all.data(), sizeof(double) * zs_all.size(), hipMemcpyHostToDevice, st));
+    INTI_HIP(ctx, hipMemcpyAsync(dkp.p, B->kp.data(), sizeof(KParams) * P, hipMemcpyHostToDevice, st));
+    INTI_HIP(ctx, hipMemcpyAsync(dmu.p, mu, sizeof(double) * ntot, hipMemcpyHostToDevice, st));
+    {
+        TraceBatchArgs a;
+        a.ntot = ntot; a.ns = ns;
+        a.sx = B->coords; a.sn = B->normals; a.zs = (const double *)dzs.p;
+        a.row2p = B->row2p; a.kp = (const KParams *)dkp.p; a.mu = (const double *)dmu.p;
+        a.B0 = (double2 *)dB0.p; a.B1 = (double2 *)dB1.p; a.theta = (double2 *)dth.p;
+        if (!launch_dim_traces_batch(B->pde, dim, a, st))
+            return set_error(ctx, INTI_ERR_UNSUPPORTED, "batch_bdim: no trace kernel for this operator");
+        INTI_HIP(ctx, hipGetLastError());
+    }
+    // Theta += S B1 - D B0 by the batched applies of the uncorrected SL and DL
+    // operators of the same problems (views sharing B's geometry and row map)
+    const unsigned g1 = (unsigned)((nb + 255) / 256);
+    const int64_t npairs = (int64_t)nq * ne_tot, ntri = npairs * nq;
+    if (ntri >= (int64_t)INT32_MAX) return set_error(ctx, INTI_ERR_SIZE, "batch_bdim: too many triplets");
+    TripletPattern pat;
+    {
+        const double *x1 = (const double *)dB1.p, *x0 = (const double *)dB0.p;
+        double *yt = (double *)dth.p;
+        if (!cplx) {  // real column blocks for the Laplace applies
+            INTI_ALLOC(ctx, dr0, sizeof(double) * nb);
+            INTI_ALLOC(ctx, dr1, sizeof(double) * nb);
+            INTI_ALLOC(ctx, drt, sizeof(double) * nb);
+            hipLaunchKernelGGL(k_real_part, dim3(g1), dim3(256), 0, st, (int64_t)nb, (const double2 *)dB0.p, (double *)dr0.p);
+            hipLaunchKernelGGL(k_real_part, dim3(g1), dim3(256), 0, st, (int64_t)nb, (const double2 *)dB1.p, (double *)dr1.p);
+            hipLaunchKernelGGL(k_real_part, dim3(g1), dim3(256), 0, st, (int64_t)nb, (const double2 *)dth.p, (double *)drt.p);
+            INTI_HIP(ctx, hipGetLastError());
+            x1 = (const double *)dr1.p; x0 = (const double *)dr0.p; yt = (double *)drt.p;
+        }
+        Batch V[2];
+        // declared after V, so destroyed first: V must not free B's arrays
+        struct Unshare {
+            Batch *v;
+            ~Unshare() {
+                for (int q = 0; q < 2; ++q) {
+                    v[q].coords = v[q].normals = v[q].weights = nullptr;
+                    v[q].row2p = nullptr;
+                }
+            }
+        } unshare{V};
+        for (int pass = 0; pass < 2; ++pass) {
+            Batch &W = V[pass];
+            const int kind = pass == 0 ? INTI_SL : INTI_DL;
+            W.ctx = ctx; W.pde = B->pde; W.kind = kind; W.dim = dim; W.precision = INTI_FP64;
+            W.nprob = P; W.ntot = ntot; W.off = B->off;
+            W.coefs.assign(4 * (size_t)P, 0.0);
+            W.cid.assign(2 * (size_t)P, 0.0);
+            W.kp.resize(P);
+            for (int p = 0; p < P; ++p) {
+                Op tmp;
+                tmp.ctx = ctx; tmp.pde = B->pde; tmp.kind = kind; tmp.dim = dim; tmp.precision = INTI_FP64;
+                tmp.k = B->kp[p].k;
+                int ek = kind;
+                W.kp[p] = op_params(&tmp, &ek);
+            }
+            W.coords = B->coords; W.normals = B->normals; W.weights = B->weights; W.row2p = B->row2p;
+            W.chunk_len = B->chunk_len; W.grid = B->grid;
+            W.c_rowptr.resize(P); W.c_colind.resize(P); W.c_val.resize(P);
+            if (hipMalloc((void **)&W.cid_d, sizeof(double) * 2 * P) != hipSuccess)
+                return set_error(ctx, INTI_ERR_NOMEM, "batch_bdim: allocation failed");
+            INTI_HIP(ctx, hipMemsetAsync(W.cid_d, 0, sizeof(double) * 2 * P, st));
+            const int rc = batch_reserve(&W);  // the launch plan (synchronises once)
+            if (rc) return rc;
+        }
+        for (int pass = 0; pass < 2; ++pass) {
+            const double alpha[2] = {pass == 0 ? 1.0 : -1.0, 0.0}, beta[2] = {1.0, 0.0};
+            const int rc = batch_apply_dev(&V[pass], ns, pass == 0 ? x1 : x0, ntot, yt, ntot, alpha, beta);
+            if (rc) {
+                (void)hipStreamSynchronize(st);
+                return rc;
+            }
+        }
+        // the triplets (target qt[e*nq + t], node qt[e*nq + r]) and their
+        // block-diagonal CSR pattern, on the host while the device sums
+        {
+            std::vector<int32_t> tri_row(ntri), tri_col(ntri);
+            for (int64_t e = 0; e < ne_tot; ++e)
+                for (int t = 0; t < nq; ++t)
+                    for (int r = 0; r < nq; ++r) {
+                        const int64_t q = ((e * nq) + t) * nq + r;
+                        tri_row[q] = qt[e * nq + t];
+                        tri_col[q] = qt[e * nq + r];
+                    }
+            build_pattern(ntot, tri_row, tri_col, pat);
+        }
+        if (hipStreamSynchronize(st) != hipSuccess) return set_error(ctx, INTI_ERR_HIP, "batch_bdim: Theta apply failed");
+        if (!cplx) {
+            hipLaunchKernelGGL(k_widen, dim3(g1), dim3(256), 0, st, (int64_t)nb, (const double *)drt.p, (double2 *)dth.p);
+            INTI_HIP(ctx, hipGetLastError());
+        }
+    }
+    // least squares of every element: the element's own nodes are its targets
+    std::vector<int32_t> pptr(ne_tot + 1), elems(ne_tot);
+    for (int64_t e = 0; e <= ne_tot; ++e) pptr[e] = (int32_t)(e * nq);
+    std::iota(elems.begin(), elems.end(), 0);
+    INTI_ALLOC(ctx, dqt, sizeof(int32_t) * qt.size());
+    INTI_ALLOC(ctx, dpp, sizeof(int32_t) * pptr.size());
+    INTI_ALLOC(ctx, del, sizeof(int32_t) * elems.size());
+    INTI_ALLOC(ctx, dtriS, sizeof(double2) * ntri);
+    INTI_ALLOC(ctx, dtriD, sizeof(double2) * ntri);
+    INTI_ALLOC(ctx, ddiag, sizeof(double) * 3 * ne_tot);
+    INTI_HIP(ctx, hipMemcpyAsync(dqt.p, qt.data(), sizeof(int32_t) * qt.size(), hipMemcpyHostToDevice, st));
+    INTI_HIP(ctx, hipMemcpyAsync(dpp.p, pptr.data(), sizeof(int32_t) * pptr.size(), hipMemcpyHostToDevice, st));
+    INTI_HIP(ctx, hipMemcpyAsync(del.p, elems.data(), sizeof(int32_t) * elems.size(), hipMemcpyHostToDevice, st));
+    {
+        LsArgs a;
+        a.n = ntot; a.nloc = ntot; a.nq = nq; a.ns = ns;
+        a.elems = (const int32_t *)del.p; a.qtags = (const int32_t *)dqt.p;
+        a.pair_ptr = (const int32_t *)dpp.p; a.pair_loc = (const int32_t *)dqt.p;  // pair e*nq + t -> node qt[e*nq + t]
+        a.B0 = (const double2 *)dB0.p; a.B1 = (const double2 *)dB1.p; a.ldb = ntot;
+        a.theta = (const double2 *)dth.p;
+        a.triS = (double2 *)dtriS.p; a.triD = (double2 *)dtriD.p;
+        a.diag = (double *)ddiag.p;
+        launch_dim_ls(a, ne_tot, st);
+        INTI_HIP(ctx, hipGetLastError());
+    }
+    const int64_t nnz = (int64_t)pat.colind.size();
+    const int cw = cplx ? 2 : 1;
+    INTI_ALLOC(ctx, dseg, sizeof(int32_t) * pat.seg.size());
+    INTI_ALLOC(ctx, dperm, sizeof(int32_t) * pat.perm.size());
+    INTI_ALLOC(ctx, dvS, sizeof(double) * cw * nnz);
+    INTI_ALLOC(ctx, dvD, sizeof(double) * cw * nnz);
+    INTI_HIP(ctx, hipMemcpyAsync(dseg.p, pat.seg.data(), sizeof(int32_t) * pat.seg.size(), hipMemcpyHostToDevice, st));
+    INTI_HIP(ctx, hipMemcpyAsync(dperm.p, pat.perm.data(), sizeof(int32_t) * pat.perm.size(), hipMemcpyHostToDevice, st));
+    {
+        Gather1Args g;
+        g.nnz = nnz; g.seg = (const int32_t *)dseg.p; g.perm = (const int32_t *)dperm.p; g.cplx = cplx ? 1 : 0;
+        g.tri = (const double2 *)dtriS.p; g.val = (double *)dvS.p; g.sign = -1.0;
+        launch_gather1(g, st);
+        g.tri = (const double2 *)dtriD.p; g.val = (double *)dvD.p; g.sign = 1.0;
+        launch_gather1(g, st);
+        INTI_HIP(ctx, hipGetLastError());
+    }
+    // one CSR per problem: rows [off_p, off_p+1), columns shifted by off_p
+    std::vector<Csr *> made;
+    auto fail = [&](int code) {
+        for (Csr *c : made) csr_free(c);
+        return code;
+    };
+    // the host rows and columns of every problem stay alive until the one
+    // synchronisation after the last asynchronous upload
+    std::vector<std::vector<int32_t>> rps(P), cis(P);
+    for (int p = 0; p < P; ++p) {
+        const int64_t r0 = B->off[p], n = B->off[p + 1] - r0;
+        const int64_t q0 = pat.rowptr[r0], q1 = pat.rowptr[r0 + n], nzp = q1 - q0;
+        std::vector<int32_t> &rp = rps[p], &ci = cis[p];
+        rp.resize(n + 1);
+        ci.resize(nzp);
+        for (int64_t i = 0; i <= n; ++i) rp[i] = (int32_t)(pat.rowptr[r0 + i] - q0);
+        for (int64_t q = 0; q < nzp; ++q) ci[q] = (int32_t)(pat.colind[q0 + q] - r0);
+        Csr *cS = nullptr, *cD = nullptr;
+        int rc = csr_alloc(ctx, n, n, nzp, cplx ? 1 : 0, &cS);
+        if (rc) return fail(rc);
+        made.push_back(cS);
+        rc = csr_alloc(ctx, n, n, nzp, cplx ? 1 : 0, &cD);
+        if (rc) return fail(rc);
+        made.push_back(cD);
+        for (Csr *c : {cS, cD})
+            if (hipMemcpyAsync(c->rowptr, rp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
+                (nzp && hipMemcpyAsync(c->colind, ci.data(), sizeof(int32_t) * nzp, hipMemcpyHostToDevice, st) != hipSuccess))
+                return fail(set_error(ctx, INTI_ERR_HIP, "batch_bdim: CSR upload failed"));
+        if (nzp && (hipMemcpyAsync(cS->val, (const double *)dvS.p + cw * q0, sizeof(double) * cw * nzp,
+                                   hipMemcpyDeviceToDevice, st) != hipSuccess ||
+                    hipMemcpyAsync(cD->val, (const double *)dvD.p + cw * q0, sizeof(double) * cw * nzp,
+                                   hipMemcpyDeviceToDevice, st) != hipSuccess))
+            return fail(set_error(ctx, INTI_ERR_HIP, "batch_bdim: CSR copy failed"));
+        outS[p] = cS;
+        outD[p] = cD;
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(set_error(ctx, INTI_ERR_HIP, "batch_bdim: upload failed"));
+    return INTI_OK;
+}
+
+}  // namespace inti
+
+int inti_batch_bdim_build(inti_batch *B, int nq, const int64_t *ne, const int64_t *qtags, const double *mu,
+                          const double *params, inti_csr **dS, inti_csr **dD) {
+    if (!B) return INTI_ERR_ARG;
+    Ctx *ctx = B->ctx;
+    BREQ(ctx, ne && qtags && mu && dS && dD && nq > 0, "batch_bdim: null argument");
+    BREQ(ctx, params == nullptr || (params[0] > 0 && params[1] > 0), "batch_bdim: bad parameters");
+    BLOCK(ctx);
+    std::vector<Csr *> s(B->nprob, nullptr), d(B->nprob, nullptr);
+    const int rc = batch_bdim_build(B, nq, ne, qtags, mu, params, s.data(), d.data());
+    if (rc) return rc;
+    for (int p = 0; p < B->nprob; ++p) {
+        dS[p] = static_cast<inti_csr *>(s[p]);
+        dD[p] = static_cast<inti_csr *>(d[p]);
+    }
     return INTI_OK;
 }
 

@@ -301,20 +301,6 @@ __global__ __launch_bounds__(256) void k_theta_reduce(ThetaRedArgs a) {
 }
 
 // ------------------------------------------------------- batched QR / LS
-struct LsArgs {
-    int64_t n, nloc;
-    int nq, ns;
-    const int32_t *elems;     // elements with a non-empty near list
-    const int32_t *qtags;     // nq x ne
-    const int32_t *pair_ptr;  // ne+1 -> pairs of the element
-    const int32_t *pair_loc;  // pair -> local target row
-    const double2 *B0, *B1;   // n x nsp
-    int64_t ldb;              // = n
-    const double2 *theta;     // nloc x ns
-    double2 *triS, *triD;     // pair*nq + r
-    double *diag;             // per element: Rdiag ratio, max |W|, max |Theta|
-};
-
 __global__ __launch_bounds__(64) void k_ls(LsArgs a) {
     __shared__ double2 A[kMaxNs * kMaxNc];  // ns x nc column-major
     __shared__ double2 tau[kMaxNc];
@@ -439,12 +425,12 @@ static bool pick_bdim(int pde, int dim, int deriv, int nst, BdimFns &f) {
 }
 
 // monopole centres: bdim.jl:79-92 with utils.jl:142-149 / 158-171
-static void monopoles(const Quad *src, int ns, double radius_mult, std::vector<double> &zs) {
-    const int dim = src->dim;
+void monopoles_box(int dim, const double *lo, const double *hi, int ns, double radius_mult,
+                   std::vector<double> &zs) {
     double xc[3], diag2 = 0.0;
     for (int d = 0; d < dim; ++d) {
-        xc[d] = (src->lo[d] + src->hi[d]) / 2;
-        diag2 += (src->hi[d] - src->lo[d]) * (src->hi[d] - src->lo[d]);
+        xc[d] = (lo[d] + hi[d]) / 2;
+        diag2 += (hi[d] - lo[d]) * (hi[d] - lo[d]);
     }
     const double R = radius_mult * std::sqrt(diag2) / 2;
     zs.assign((size_t)dim * ns, 0.0);
@@ -467,25 +453,9 @@ static void monopoles(const Quad *src, int ns, double radius_mult, std::vector<d
     }
 }
 
-// device buffer owned for the duration of a build; the stream is drained
-// before the memory is released so that no kernel still reads it
-struct DevBuf {
-    void *p = nullptr;
-    hipStream_t s = nullptr;
-    ~DevBuf() {
-        if (p) {
-            if (s) (void)hipStreamSynchronize(s);
-            (void)hipFree(p);
-        }
-    }
-};
-
-#define INTI_ALLOC(ctx, buf, bytes)                                                    \
-    do {                                                                               \
-        (buf).s = (ctx)->stream;                                                       \
-        if (hipMalloc(&(buf).p, (bytes) > 0 ? (bytes) : 1) != hipSuccess)              \
-            return set_error((ctx), INTI_ERR_NOMEM, "device allocation failed");       \
-    } while (0)
+static void monopoles(const Quad *src, int ns, double radius_mult, std::vector<double> &zs) {
+    monopoles_box(src->dim, src->lo, src->hi, ns, radius_mult, zs);
+}
 
 int csr_alloc(Ctx *ctx, int64_t m, int64_t n, int64_t nnz, int cplx, Csr **out);
 void csr_free(Csr *c);
@@ -766,6 +736,46 @@ int bdim_build_types(Op *opS, Op *opD, int deriv, int ntypes, const int *nq, con
     *outS = cS;
     *outD = cD;
     return INTI_OK;
+}
+
+// --------------------------------------------- DIM corrections of a batch
+template <int PDE, int DIM>
+__global__ __launch_bounds__(256) void k_traces_batch(TraceBatchArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= a.ntot * a.ns) return;
+    const int64_t j = e % a.ntot;
+    const int kk = (int)(e / a.ntot);
+    const int p = a.row2p[j];
+    const double *z = a.zs + (int64_t)p * DIM * a.ns;
+    double r[DIM], ny[DIM], dummy[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        r[d] = a.sx[d * a.ntot + j] - z[d * a.ns + kk];
+        ny[d] = a.sn[d * a.ntot + j];
+        dummy[d] = 0.0;
+    }
+    const KParams kp = a.kp[p];
+    double g0r, g0i, g1r, g1i;
+    pair_value<PDE, DIM, INTI_SL>(r, ny, dummy, kp, g0r, g0i);
+    pair_value<PDE, DIM, INTI_ADL>(r, ny, dummy, kp, g1r, g1i);
+    const double mu = a.mu[j];
+    a.B0[e] = make_double2(g0r, g0i);
+    a.B1[e] = make_double2(g1r, g1i);
+    a.theta[e] = make_double2(mu * g0r, mu * g0i);
+}
+
+bool launch_dim_traces_batch(int pde, int dim, const TraceBatchArgs &a, hipStream_t s) {
+    const dim3 g((unsigned)((a.ntot * a.ns + 255) / 256));
+    if (pde == INTI_LAPLACE && dim == 2) hipLaunchKernelGGL((k_traces_batch<INTI_LAPLACE, 2>), g, dim3(256), 0, s, a);
+    else if (pde == INTI_LAPLACE && dim == 3) hipLaunchKernelGGL((k_traces_batch<INTI_LAPLACE, 3>), g, dim3(256), 0, s, a);
+    else if (pde == INTI_HELMHOLTZ && dim == 2) hipLaunchKernelGGL((k_traces_batch<INTI_HELMHOLTZ, 2>), g, dim3(256), 0, s, a);
+    else if (pde == INTI_HELMHOLTZ && dim == 3) hipLaunchKernelGGL((k_traces_batch<INTI_HELMHOLTZ, 3>), g, dim3(256), 0, s, a);
+    else return false;
+    return true;
+}
+
+void launch_dim_ls(const LsArgs &a, int64_t nel, hipStream_t s) {
+    if (nel > 0) hipLaunchKernelGGL(k_ls, dim3((unsigned)nel), dim3(64), 0, s, a);
 }
 
 int bdim_build(Op *opS, Op *opD, int deriv, int nq, int64_t ne, const int64_t *qtags,

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "pair.hpp"
 
 namespace inti {
 
@@ -166,5 +167,65 @@ struct Gather1Args {
     double sign;
 };
 void launch_gather1(const Gather1Args &g, hipStream_t s);
+
+// device buffer owned for the duration of a build; the stream is drained
+// before the memory is released so that no kernel still reads it
+struct DevBuf {
+    void *p = nullptr;
+    hipStream_t s = nullptr;
+    ~DevBuf() {
+        if (p) {
+            if (s) (void)hipStreamSynchronize(s);
+            (void)hipFree(p);
+        }
+    }
+};
+
+#define INTI_ALLOC(ctx, buf, bytes)                                                    \
+    do {                                                                               \
+        (buf).s = (ctx)->stream;                                                       \
+        if (hipMalloc(&(buf).p, (bytes) > 0 ? (bytes) : 1) != hipSuccess)              \
+            return set_error((ctx), INTI_ERR_NOMEM, "device allocation failed");       \
+    } while (0)
+
+// ----------------------------------------------- DIM pieces (bdim.hip)
+// Monopole centres of bdim.jl:79-92 around the box [lo, hi] (dim x ns,
+// row-major by coordinate).
+void monopoles_box(int dim, const double *lo, const double *hi, int ns, double radius_mult,
+                   std::vector<double> &zs);
+
+// Per-element Householder QR of M^T and the least-squares solve of each of
+// the element's targets (k_ls, one wavefront per element).
+struct LsArgs {
+    int64_t n, nloc;
+    int nq, ns;
+    const int32_t *elems;     // elements with a non-empty near list
+    const int32_t *qtags;     // nq x ne
+    const int32_t *pair_ptr;  // ne+1 -> pairs of the element
+    const int32_t *pair_loc;  // pair -> local target row
+    const double2 *B0, *B1;   // n x nsp
+    int64_t ldb;              // = n
+    const double2 *theta;     // nloc x ns
+    double2 *triS, *triD;     // pair*nq + r
+    double *diag;             // per element: Rdiag ratio, max |W|, max |Theta|
+};
+void launch_dim_ls(const LsArgs &a, int64_t nel, hipStream_t s);
+
+// Traces of every problem of a batch in one launch: node j of problem
+// row2p[j] against that problem's monopoles zs + p * dim * ns with its
+// kernel parameters kp[p]: B0 = G(y_j, z_k), B1 = dG/dn_j (y_j, z_k), and
+// Theta's first term mu_j B0 (the targets are the sources). ntot x ns,
+// column-major, complex.
+struct TraceBatchArgs {
+    int64_t ntot;
+    int ns;
+    const double *sx, *sn, *zs;
+    const int32_t *row2p;
+    const KParams *kp;
+    const double *mu;
+    double2 *B0, *B1, *theta;
+};
+// false when there is no kernel for (pde, dim)
+bool launch_dim_traces_batch(int pde, int dim, const TraceBatchArgs &a, hipStream_t s);
 
 }  // namespace inti
