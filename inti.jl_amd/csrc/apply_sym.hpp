@@ -146,36 +146,46 @@ struct SymRec {
 };
 
 // LDS record of a source node for the fused mixed 3D Helmholtz DL / DL +
-// ch*SL pairs (helm3_sym_fused32): coordinates and normal in FP64 (three
-// double2 slots) and the staged density terms in FP32: (q, k Im q, -k Re q)
-// in one 16-byte slot and ch q in another. CHIK: ch = -i k, so ch q equals
-// (k Im q, -k Re q) and the record has four slots (16 KB of LDS per tile).
+// ch*SL pairs (helm3_sym_fused32), four 16-byte slots (16 KB of LDS per
+// tile): the coordinates in FP64, the normal in FP32 (it enters only the
+// FP32 amplitudes), and the staged density terms in FP32, (q, k Im q,
+// -k Re q) and ch q (CHIK: ch = -i k, so ch q equals (k Im q, -k Re q)).
 template <bool CHIK>
 struct SymRecF {
-    static constexpr int NS = CHIK ? 4 : 5;
+    static constexpr int NS = 4;
+    // y (FP64), the normal in FP32 (it only enters the FP32 amplitudes) and
+    // the FP32 source terms: y0 y1 | y2 n0 n1 | n2 cr ci - | qr qi kqi mkqr
     __device__ static __forceinline__ void put(double2 (*s)[kSymBS], int j, const double *y, const double *nv,
                                                const QExt32 &q) {
         s[0][j] = make_double2(y[0], y[1]);
-        s[1][j] = make_double2(y[2], nv[0]);
-        s[2][j] = make_double2(nv[1], nv[2]);
+        s[1][j].x = y[2];
+        *reinterpret_cast<float2 *>(&s[1][j].y) = make_float2((float)nv[0], (float)nv[1]);
+        *reinterpret_cast<float4 *>(&s[2][j]) =
+            make_float4((float)nv[2], CHIK ? 0.0f : q.cr, CHIK ? 0.0f : q.ci, 0.0f);
         *reinterpret_cast<float4 *>(&s[3][j]) = make_float4(q.qr, q.qi, q.kqi, q.mkqr);
-        if constexpr (!CHIK) *reinterpret_cast<float4 *>(&s[4][j]) = make_float4(q.cr, q.ci, 0.0f, 0.0f);
     }
-    __device__ static __forceinline__ void get(const double2 (*s)[kSymBS], int j, double *y, double *nv,
+    __device__ static __forceinline__ void get(const double2 (*s)[kSymBS], int j, double *y, float *nv,
                                                QExt32 &q) {
-        const double2 v0 = s[0][j], v1 = s[1][j], v2 = s[2][j];
+        const double2 v0 = s[0][j], v1 = s[1][j];
+        const float2 n01 = *reinterpret_cast<const float2 *>(&v1.y);
+        const float4 g = *reinterpret_cast<const float4 *>(&s[2][j]);
         const float4 f = *reinterpret_cast<const float4 *>(&s[3][j]);
         y[0] = v0.x; y[1] = v0.y; y[2] = v1.x;
-        nv[0] = v1.y; nv[1] = v2.x; nv[2] = v2.y;
+        nv[0] = n01.x; nv[1] = n01.y; nv[2] = g.x;
         q.qr = f.x; q.qi = f.y; q.kqi = f.z; q.mkqr = f.w;
         if constexpr (CHIK) {
             q.cr = f.z;
             q.ci = f.w;
         } else {
-            const float4 g = *reinterpret_cast<const float4 *>(&s[4][j]);
-            q.cr = g.x;
-            q.ci = g.y;
+            q.cr = g.y;
+            q.ci = g.z;
         }
+    }
+    __device__ static __forceinline__ void get(const double2 (*s)[kSymBS], int j, double *y, double *nv,
+                                               QExt32 &q) {
+        float n32[3];
+        get(s, j, y, n32, q);
+        nv[0] = n32[0]; nv[1] = n32[1]; nv[2] = n32[2];
     }
 };
 
@@ -302,15 +312,18 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                             // FP32 sums of this pass: s1 for the lane's
                             // targets, the chains c for the rotating sources
                             QExt32 qit[TPL];
-                            float s1r[TPL], s1i[TPL];
+                            float s1r[TPL], s1i[TPL], nif[TPL][3];
 #pragma unroll
                             for (int t = 0; t < TPL; ++t) {
                                 qit[t] = qext32(qir[t], qii[t], p);
                                 s1r[t] = s1i[t] = 0.0f;
+#pragma unroll
+                                for (int d = 0; d < 3; ++d) nif[t][d] = (float)ni[t][d];
                             }
                             float car = 0.0f, cai = 0.0f, cbr = 0.0f, cbi = 0.0f;
                             auto one = [&](int jl, float &ajr, float &aji) {
-                                double yj[3], nj[3];
+                                double yj[3];
+                                float nj[3];
                                 QExt32 qj;
                                 R::get(s_src, jl, yj, nj, qj);
 #pragma unroll
@@ -318,7 +331,7 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                                     double r[3];
 #pragma unroll
                                     for (int d = 0; d < 3; ++d) r[d] = xi[t][d] - yj[d];
-                                    helm3_sym_fused32<MASK, TRANS>(r, ni[t], nj, p, qj, qit[t], s1r[t], s1i[t],
+                                    helm3_sym_fused32<MASK, TRANS>(r, nif[t], nj, p, qj, qit[t], s1r[t], s1i[t],
                                                                    ajr, aji);
                                 }
                             };

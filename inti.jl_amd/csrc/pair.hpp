@@ -433,7 +433,7 @@ __device__ __forceinline__ QExt32 qext32(double qr, double qi, const KParams &p)
     return q;
 }
 template <bool MASK, bool TRANS>
-__device__ __forceinline__ void helm3_sym_fused32(const double *r, const double *ni, const double *nj,
+__device__ __forceinline__ void helm3_sym_fused32(const double *r, const float *ni, const float *nj,
                                                   const KParams &p, const QExt32 &qj, const QExt32 &qi,
                                                   float &a1r, float &a1i, float &a2r, float &a2i) {
     const double d2 = dotv<3>(r, r);
@@ -446,8 +446,10 @@ __device__ __forceinline__ void helm3_sym_fused32(const double *r, const double 
     const float ym = ok ? y : 0.0f;
     const float cf = __builtin_amdgcn_cosf(u) * ym;
     const float sf = __builtin_amdgcn_sinf(u) * ym;
-    const float aj = (float)dotv<3>(r, nj) * y;    // K(x_i, x_j): source normal nj
-    const float ai = -(float)dotv<3>(r, ni) * y;   // K(x_j, x_i): source normal ni, r -> -r
+    // the amplitudes (r . n) / d in FP32 (r rounded once; 1e-7 of the matvec)
+    const float r0 = (float)r[0], r1 = (float)r[1], r2 = (float)r[2];
+    const float aj = __builtin_fmaf(r0, nj[0], __builtin_fmaf(r1, nj[1], r2 * nj[2])) * y;    // K(x_i, x_j)
+    const float ai = -__builtin_fmaf(r0, ni[0], __builtin_fmaf(r1, ni[1], r2 * ni[2])) * y;   // K(x_j, x_i), r -> -r
     const float b1 = TRANS ? ai : aj, b2 = TRANS ? aj : ai;
     // (i <- j): q_j
     const float w1r = __builtin_fmaf(y, qj.qr, qj.kqi), w1i = __builtin_fmaf(y, qj.qi, qj.mkqr);
