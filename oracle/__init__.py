@@ -16,20 +16,28 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 LIB_FMA = os.path.join(HERE, "liboracle_fma.so")
+LIB_REV = os.path.join(HERE, "liboracle_rev.so")
 
 _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
 _lib = None
 _lib_fma = None
+_lib_rev = None
 
 
 def _load(variant=None):
-    """The oracle library; variant="fma" gives the FMA-contracted build."""
-    global _lib, _lib_fma
+    """The oracle library; variant="fma" gives the FMA-contracted build,
+    variant="rev" the build whose Theta sums run over the sources in
+    reverse order."""
+    global _lib, _lib_fma, _lib_rev
     if variant == "fma":
         if _lib_fma is None:
             _lib_fma = _bind(LIB_FMA)
         return _lib_fma
+    if variant == "rev":
+        if _lib_rev is None:
+            _lib_rev = _bind(LIB_REV)
+        return _lib_rev
     if _lib is None:
         _lib = _bind(LIB)
     return _lib

@@ -395,7 +395,12 @@ int64_t ork_bdim_types(int pde, int dim, double k, int derivative,
         }
         cplx sacc[64], dacc[64];
         for (int kk = 0; kk < ns; ++kk) { sacc[kk] = 0; dacc[kk] = 0; }
-        for (int64_t j = 0; j < n; ++j) {
+        for (int64_t jj = 0; jj < n; ++jj) {
+#ifdef ORK_REVERSE_SUMS
+            const int64_t j = n - 1 - jj; /* the same sums, sources in reverse order */
+#else
+            const int64_t j = jj;
+#endif
             gather(dim, sx, n, j, y);
             gather(dim, sn, n, j, ny);
             cplx Sij = kernel_pt(pde, dim, kS, k, x, nx, y, ny) * sw[j];
