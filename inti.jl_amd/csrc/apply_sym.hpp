@@ -203,6 +203,10 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
     // the plain pair
     constexpr bool FUSED =
         MIXED && PDE == INTI_HELMHOLTZ && DIM == 3 && (KIND == INTI_DL || KIND == INTI_COMB_SD);
+    // mixed 3D Laplace SL: 1/d from the FP32 unit, its products and 64-term
+    // running sums in FP32 added into FP64 after every pass, as above (the
+    // distance itself stays FP64)
+    constexpr bool FUSEDL = MIXED && PDE == INTI_LAPLACE && DIM == 3 && KIND == INTI_SL;
     using R = std::conditional_t<FUSED, SymRecF<CHIK>, SymRec<PDE, DIM, KIND>>;
     constexpr bool CPLX = PDE == INTI_HELMHOLTZ;
     constexpr int BS = kSymBS / TPL;  // threads
@@ -357,6 +361,41 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                             aai += (double)lane_next(cai, far);
                             abr += (double)lane_next(cbr, far);
                             abi += (double)lane_next(cbi, far);
+                        } else if constexpr (FUSEDL) {
+                            float s1[TPL], qf[TPL];
+#pragma unroll
+                            for (int t = 0; t < TPL; ++t) {
+                                s1[t] = 0.0f;
+                                qf[t] = (float)qir[t];
+                            }
+                            float ca = 0.0f, cb = 0.0f;
+                            auto one = [&](int jl, float &aj) {
+                                double yj[3], nj[3], qjr, qji;
+                                R::get(s_src, jl, yj, nj, qjr, qji);
+                                const float qj = (float)qjr;
+#pragma unroll
+                                for (int t = 0; t < TPL; ++t) {
+                                    double r[3];
+#pragma unroll
+                                    for (int d = 0; d < 3; ++d) r[d] = xi[t][d] - yj[d];
+                                    const double d2 = dotv<3>(r, r);
+                                    float y = __builtin_amdgcn_rsqf(MASK ? fmaxf((float)d2, 1.0e-30f) : (float)d2);
+                                    if constexpr (MASK) y = d2 > kSamePointTol2 ? y : 0.0f;
+                                    s1[t] = __builtin_fmaf(y, qj, s1[t]);  // K(x_i, x_j) q_j
+                                    aj = __builtin_fmaf(y, qf[t], aj);     // K(x_j, x_i) q_i
+                                }
+                            };
+                            for (int s = 0; s < 32; ++s) {
+                                const int jl = u * 64 + ((lane + s) & 63);
+                                one(jl, ca);
+                                one(jl ^ 32, cb);
+                                ca = lane_next(ca, nxt);
+                                cb = lane_next(cb, nxt);
+                            }
+#pragma unroll
+                            for (int t = 0; t < TPL; ++t) adr[gi][t] += (double)s1[t];
+                            aar += (double)lane_next(ca, far);
+                            abr += (double)lane_next(cb, far);
                         } else {
                             auto one = [&](int jl, double &ajr, double &aji) {
                                 double yj[3], nj[3], qjr, qji;
@@ -401,7 +440,7 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                     };
                     if (A.distinct) pairs(std::false_type{});
                     else pairs(std::true_type{});
-                    if constexpr (!FUSED) {
+                    if constexpr (!FUSED && !FUSEDL) {
                         // exchange the chains' roles for the next pass
                         double t0 = aar; aar = abr; abr = t0;
                         t0 = aai; aai = abi; abi = t0;
