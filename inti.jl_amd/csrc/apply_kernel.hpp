@@ -38,8 +38,9 @@ struct ApplyTile {
 };
 
 // One right-hand side of the mixed 3D Helmholtz DL / DL + ch*SL kinds: the
-// pair value and its accumulation are fused (helm3_fused_one), with q, ch q
-// and (k Im q, -k Re q) staged per source (three double2 slots).
+// pair value and its accumulation are fused (helm3_fused_one32), with q,
+// (k Im q, -k Re q) and ch q staged per source in FP32 (three double2 slots
+// reserved, two used).
 template <int PDE, int DIM, int KIND, bool MIXED, int NR, bool TRANS>
 constexpr bool apply_fused() {
     return NR == 1 && !TRANS && MIXED && PDE == INTI_HELMHOLTZ && DIM == 3 &&
@@ -113,10 +114,9 @@ __device__ __forceinline__ void apply_block(const ApplyArgs &a, int64_t chunk, i
                 }
             }
             if constexpr (FUSED) {
-                const QExt qx = qext(qr, qi, a.p);
-                s_q[3 * jj] = make_double2(qx.qr, qx.qi);
-                s_q[3 * jj + 1] = make_double2(qx.cr, qx.ci);
-                s_q[3 * jj + 2] = make_double2(qx.kqi, qx.mkqr);
+                const QExt32 qx = qext32(qr, qi, a.p);
+                *reinterpret_cast<float4 *>(&s_q[3 * jj]) = make_float4(qx.qr, qx.qi, qx.kqi, qx.mkqr);
+                *reinterpret_cast<float2 *>(&s_q[3 * jj + 1]) = make_float2(qx.cr, qx.ci);
             } else {
                 s_q[jj * NR + r] = make_double2(qr, qi);
             }
@@ -126,6 +126,30 @@ __device__ __forceinline__ void apply_block(const ApplyArgs &a, int64_t chunk, i
         // accumulators already fill the register file) and for the 2D
         // Helmholtz kernels (the Hankel evaluation does)
         constexpr int UNR = (NR == 1 && !(CPLX && DIM == 2)) ? 2 : 1;
+        if constexpr (FUSED) {
+            // FP32 products and 64-term FP32 running sums added into FP64
+            for (int jb = 0; jb < TT; jb += 64) {
+                float sr = 0.0f, si = 0.0f;
+#pragma unroll UNR
+                for (int jj = jb; jj < jb + 64; ++jj) {
+                    const double2 *rc = s_geo + jj * NG;
+                    const double2 v0 = rc[0], v1 = rc[1], v2 = rc[2];
+                    const double ys[3] = {v0.x, v0.y, v1.x}, ns[3] = {v1.y, v2.x, v2.y};
+                    const float4 f = *reinterpret_cast<const float4 *>(&s_q[3 * jj]);
+                    const float2 g = *reinterpret_cast<const float2 *>(&s_q[3 * jj + 1]);
+                    QExt32 qx;
+                    qx.qr = f.x; qx.qi = f.y; qx.kqi = f.z; qx.mkqr = f.w; qx.cr = g.x; qx.ci = g.y;
+                    double r3[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) r3[d] = xt[d] - ys[d];
+                    helm3_fused_one32(r3, ns, a.p, qx, sr, si);
+                }
+                accr[0] += (double)sr;
+                acci[0] += (double)si;
+            }
+            __syncthreads();
+            continue;
+        }
 #pragma unroll UNR
         for (int jj = 0; jj < TT; ++jj) {
             const double2 *rc = s_geo + jj * NG;
@@ -138,16 +162,6 @@ __device__ __forceinline__ void apply_block(const ApplyArgs &a, int64_t chunk, i
                 const double2 v0 = rc[0], v1 = rc[1];
                 ys[0] = v0.x; ys[1] = v0.y;
                 ns[0] = v1.x; ns[1] = v1.y;
-            }
-            if constexpr (FUSED) {
-                const double2 v0 = s_q[3 * jj], v1 = s_q[3 * jj + 1], v2 = s_q[3 * jj + 2];
-                QExt qx;
-                qx.qr = v0.x; qx.qi = v0.y; qx.cr = v1.x; qx.ci = v1.y; qx.kqi = v2.x; qx.mkqr = v2.y;
-                double r3[3];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) r3[d] = xt[d] - ys[d];
-                helm3_fused_one(r3, ns, a.p, qx, accr[0], acci[0]);
-                continue;
             }
             double kr, ki;
             if constexpr (TRANS) pair_raw_xy<PDE, DIM, KIND, MIXED>(ys, ns, xt, nt, a.p, kr, ki);

@@ -465,24 +465,27 @@ __device__ __forceinline__ void helm3_sym_fused32(const double *r, const float *
 
 // One direction of the fused 3D Helmholtz DL / DL + ch*SL pair (target x,
 // source y with normal ny, r = x - y), mixed precision, coincident points
-// masked: acc += raw q = (E/d) [ch q + a (1/d - i k) q], a = (r . ny) / d.
-__device__ __forceinline__ void helm3_fused_one(const double *r, const double *ny, const KParams &p,
-                                                const QExt &q, double &accr, double &acci) {
+// masked: acc += raw q = (E/d) [ch q + a (1/d - i k) q], a = (r . ny) / d,
+// with FP32 products and an FP32 running sum (the caller adds it into FP64
+// every 64 sources); the distance, the phase and the amplitude's dot
+// product stay FP64.
+__device__ __forceinline__ void helm3_fused_one32(const double *r, const double *ny, const KParams &p,
+                                                  const QExt32 &q, float &ar, float &ai) {
     const double d2 = dotv<3>(r, r);
     const bool ok = d2 > kSamePointTol2;
-    const float y32 = __builtin_amdgcn_rsqf(fmaxf((float)d2, 1.0e-30f));
-    const double id = (double)y32;
+    const float y = __builtin_amdgcn_rsqf(fmaxf((float)d2, 1.0e-30f));
+    const double id = (double)y;
     const double e = d2 * id;
     const double t = (p.kappa_half * e) * __builtin_fma(-e, id, 3.0);  // kappa d, one Newton step
     const float u = (float)__builtin_amdgcn_fract(t);
-    const float ym = ok ? y32 : 0.0f;
-    const double cf = (double)(__builtin_amdgcn_cosf(u) * ym);
-    const double sf = (double)(__builtin_amdgcn_sinf(u) * ym);
-    const double b = dotv<3>(r, ny) * id;
-    const double wr = __builtin_fma(id, q.qr, q.kqi), wi = __builtin_fma(id, q.qi, q.mkqr);
-    const double zr = __builtin_fma(b, wr, q.cr), zi = __builtin_fma(b, wi, q.ci);
-    accr = __builtin_fma(cf, zr, __builtin_fma(-sf, zi, accr));
-    acci = __builtin_fma(cf, zi, __builtin_fma(sf, zr, acci));
+    const float ym = ok ? y : 0.0f;
+    const float cf = __builtin_amdgcn_cosf(u) * ym;
+    const float sf = __builtin_amdgcn_sinf(u) * ym;
+    const float b = (float)dotv<3>(r, ny) * y;
+    const float wr = __builtin_fmaf(y, q.qr, q.kqi), wi = __builtin_fmaf(y, q.qi, q.mkqr);
+    const float zr = __builtin_fmaf(b, wr, q.cr), zi = __builtin_fmaf(b, wi, q.ci);
+    ar = __builtin_fmaf(cf, zr, __builtin_fmaf(-sf, zi, ar));
+    ai = __builtin_fmaf(cf, zi, __builtin_fmaf(sf, zr, ai));
 }
 
 inline KParams basic_params(double k) {
