@@ -400,19 +400,6 @@ __device__ __forceinline__ void pair_raw_sym(const double *r, const double *ni, 
 // direction (i <- j) adds to (a1r, a1i) with the source normal nj and q_j,
 // the direction (j <- i) to (a2r, a2i) with ni and q_i. TRANS accumulates
 // the transposed sums (K(x_j, x_i) q_j at i, K(x_i, x_j) q_i at j).
-// QExt / qext serve the one-direction form below.
-struct QExt {
-    double qr, qi, cr, ci, kqi, mkqr;  // q, ch q, k Im q, -k Re q
-};
-__device__ __forceinline__ QExt qext(double qr, double qi, const KParams &p) {
-    QExt q;
-    q.qr = qr; q.qi = qi;
-    q.cr = p.chr * qr - p.chi * qi;
-    q.ci = p.chr * qi + p.chi * qr;
-    q.kqi = p.k * qi;
-    q.mkqr = -p.k * qr;
-    return q;
-}
 // The mixed symmetric form: the distance, 1/d and the phase as in
 // helm3_raw_mixed (FP64 geometry and phase, FP32 transcendentals); the
 // staged density terms, both directions' products and their running sums in
