@@ -205,7 +205,13 @@ __device__ __forceinline__ void apply_block(const ApplyArgs &a, int64_t chunk, i
 // wavefronts, so a block still covers 256 targets and the partial-sum layout
 // and grid are those of apply_block.
 // Accumulator layout: element v of lane l is Y[4 v + l / 16][l % 16].
+// Mixed precision: the kernel values and the weighted densities are rounded
+// to FP32 and multiplied on the FP32 matrix cores (v_mfma_f32_16x16x4f32,
+// twice the FP64 rate), summed in FP32 over one LDS tile of 64 sources and
+// added into the FP64 accumulators; there element v of lane l holds
+// Y[4 (l / 16) + v][l % 16].
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 constexpr int kMfmaTT = 64;  // sources per LDS tile
 constexpr int kMfmaQLd = 17; // LDS row of one source's 16 densities, padded
 constexpr int kMfmaNF = 4;   // 16-target fragments per wavefront
@@ -213,6 +219,20 @@ constexpr int kMfmaNF = 4;   // 16-target fragments per wavefront
 // threads of a block of the direct sum: 256 targets per block either way
 template <int NR>
 constexpr int apply_threads() { return NR == 16 ? 64 * (kApplyBS / 16) / kMfmaNF : kApplyBS; }
+
+// one source's coordinates and normal from its LDS record
+template <int DIM>
+__device__ __forceinline__ void load_geo(const double2 *rc, double *ys, double *ns) {
+    if constexpr (DIM == 3) {
+        const double2 v0 = rc[0], v1 = rc[1], v2 = rc[2];
+        ys[0] = v0.x; ys[1] = v0.y; ys[2] = v1.x;
+        ns[0] = v1.y; ns[1] = v2.x; ns[2] = v2.y;
+    } else {
+        const double2 v0 = rc[0], v1 = rc[1];
+        ys[0] = v0.x; ys[1] = v0.y;
+        ns[0] = v1.x; ns[1] = v1.y;
+    }
+}
 
 template <int PDE, int DIM, int KIND, bool MIXED>
 __device__ __forceinline__ void apply_block_mfma(const ApplyArgs &a, int64_t chunk, int64_t tile,
@@ -238,6 +258,7 @@ __device__ __forceinline__ void apply_block_mfma(const ApplyArgs &a, int64_t chu
             nt[f][d] = a.tn ? a.tn[d * ts + ic] : 0.0;
         }
     }
+    float2 *s_q32 = reinterpret_cast<float2 *>(s_q);
     v4d accr[NF], acci[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
@@ -282,22 +303,60 @@ __device__ __forceinline__ void apply_block_mfma(const ApplyArgs &a, int64_t chu
                     qr = w * a.x[j + r * a.ldx];
                 }
             }
-            s_q[jj * kMfmaQLd + r] = make_double2(qr, qi);
+            if constexpr (MIXED) s_q32[jj * kMfmaQLd + r] = make_float2((float)qr, (float)qi);
+            else s_q[jj * kMfmaQLd + r] = make_double2(qr, qi);
         }
         __syncthreads();
+        if constexpr (MIXED) {
+            // complex products by Gauss's three real products:
+            // P1 = Re K Re Q, P2 = Im K Im Q, P3 = (Re K + Im K)(Re Q + Im Q),
+            // Re Y = P1 - P2, Im Y = P3 - P1 - P2
+            v4f t1[NF], t2[NF], t3[NF];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                t1[f] = v4f{0, 0, 0, 0};
+                t2[f] = v4f{0, 0, 0, 0};
+                t3[f] = v4f{0, 0, 0, 0};
+            }
+            for (int st = 0; st < TT / 4; ++st) {
+                const int jj = 4 * st + lk;
+                double ys[DIM], ns[DIM];
+                load_geo<DIM>(s_geo + jj * NG, ys, ns);
+                const float2 q = s_q32[jj * kMfmaQLd + li];
+                const float qs = q.x + q.y;
+#pragma unroll
+                for (int f = 0; f < NF; ++f) {
+                    double kr, ki;
+                    pair_raw_xy<PDE, DIM, KIND, MIXED>(xt[f], nt[f], ys, ns, a.p, kr, ki);
+                    const float fr = (float)kr;
+                    t1[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(fr, q.x, t1[f], 0, 0, 0);
+                    if constexpr (CPLX) {
+                        const float fi = (float)ki;
+                        t2[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(fi, q.y, t2[f], 0, 0, 0);
+                        t3[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(fr + fi, qs, t3[f], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    if constexpr (CPLX) {
+                        const double p1 = t1[f][v], p2 = t2[f][v];
+                        accr[f][v] += p1 - p2;
+                        acci[f][v] += (double)t3[f][v] - p1 - p2;
+                    } else {
+                        accr[f][v] += (double)t1[f][v];
+                    }
+                }
+            }
+            __syncthreads();
+            continue;
+        }
         for (int st = 0; st < TT / 4; ++st) {
             const int jj = 4 * st + lk;
-            const double2 *rc = s_geo + jj * NG;
             double ys[DIM], ns[DIM];
-            if constexpr (DIM == 3) {
-                const double2 v0 = rc[0], v1 = rc[1], v2 = rc[2];
-                ys[0] = v0.x; ys[1] = v0.y; ys[2] = v1.x;
-                ns[0] = v1.y; ns[1] = v2.x; ns[2] = v2.y;
-            } else {
-                const double2 v0 = rc[0], v1 = rc[1];
-                ys[0] = v0.x; ys[1] = v0.y;
-                ns[0] = v1.x; ns[1] = v1.y;
-            }
+            load_geo<DIM>(s_geo + jj * NG, ys, ns);
             const double2 q = s_q[jj * kMfmaQLd + li];
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
@@ -317,7 +376,7 @@ __device__ __forceinline__ void apply_block_mfma(const ApplyArgs &a, int64_t chu
     for (int f = 0; f < NF; ++f) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int64_t i = t0 + 16 * f + 4 * v + lk;
+            const int64_t i = t0 + 16 * f + (MIXED ? 4 * lk + v : 4 * v + lk);
             if (i < m) {
                 const int64_t off = ((int64_t)li * a.split + chunk) * m + i;
                 if constexpr (CPLX) reinterpret_cast<double2 *>(a.part)[off] = make_double2(accr[f][v], acci[f][v]);
