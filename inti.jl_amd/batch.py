@@ -80,8 +80,8 @@ class BatchOperator:
         """The DIM corrections of every problem's quadrature with itself
         (bdim_correction, bdim.jl:50-196, per problem: self pattern, one
         element type shared by the problems) in one pass over the batch:
-        traces in one launch, Theta by two batched applies, the least
-        squares of all elements in one launch. green_multiplier: one value
+        traces in one launch, Theta in one launch over (problem, target
+        tile) work items, the least squares of all elements in one launch. green_multiplier: one value
         for all nodes (default -1/2) or one per node (ntot). Returns [(dS,
         dD)] per problem, SparseCorrection objects."""
         from .bdim import DimParameters

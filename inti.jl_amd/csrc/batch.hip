@@ -958,24 +958,13 @@ int inti_batch_set_identity(inti_batch *B, const double *cid) {
 
 namespace inti {
 
-// real <-> complex layouts of a column block (Laplace batches apply real
-// vectors; the least squares read complex ones)
-__global__ void k_real_part(int64_t n, const double2 *z, double *x) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) x[i] = z[i].x;
-}
-__global__ void k_widen(int64_t n, const double *x, double2 *z) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) z[i] = make_double2(x[i], 0.0);
-}
-
 // The DIM corrections (dS, dD) of every problem of B (bdim.jl:50-196 per
 // problem: quadrature with itself, the self pattern, one element type) in
 // one pass over the batch: the traces of all problems in one launch (each
-// against its own monopoles and wavenumber), Theta = mu B0 + S B1 - D B0 by
-// two batched applies of the problems' uncorrected SL and DL operators
-// (ns right-hand sides, the persistent launches of k_apply_batch), the
-// least squares of all elements in one launch, and the triplets summed into
+// against its own monopoles and wavenumber), Theta = mu B0 + S B1 - D B0 in
+// one launch over (problem, target tile) work items (the S and D values of a
+// pair share its radial part, all ns columns at once), the least squares of
+// all elements in one launch, and the triplets summed into
 // one block-diagonal CSR that is then cut into one CSR per problem.
 static int batch_bdim_build(Batch *B, int nq, const int64_t *ne, const int64_t *qtags, const double *mu,
                             const double *params, Csr **outS, Csr **outD) {
@@ -1027,7 +1016,8 @@ static int batch_bdim_build(Batch *B, int nq, const int64_t *ne, const int64_t *
     // device buffers
     const bool cplx = B->pde == INTI_HELMHOLTZ;
     const size_t nb = (size_t)ntot * ns;
-    DevBuf dzs, dkp, dmu, dB0, dB1, dth, dqt, dpp, del, dtriS, dtriD, ddiag, dseg, dperm, dvS, dvD, dr0, dr1, drt;
+    DevBuf dzs, dkp, dmu, dB0, dB1, dth, dqt, dpp, del, dtriS, dtriD, ddiag, dseg, dperm, dvS, dvD, drec, ditems,
+        doff, dtkp, dpost;
     INTI_ALLOC(ctx, dzs, sizeof(double) * zs_all.size());
     INTI_ALLOC(ctx, dkp, sizeof(KParams) * P);
     INTI_ALLOC(ctx, dmu, sizeof(double) * ntot);
@@ -1047,68 +1037,53 @@ static int batch_bdim_build(Batch *B, int nq, const int64_t *ne, const int64_t *
             return set_error(ctx, INTI_ERR_UNSUPPORTED, "batch_bdim: no trace kernel for this operator");
         INTI_HIP(ctx, hipGetLastError());
     }
-    // Theta += S B1 - D B0 by the batched applies of the uncorrected SL and DL
-    // operators of the same problems (views sharing B's geometry and row map)
-    const unsigned g1 = (unsigned)((nb + 255) / 256);
+    // Theta += S B1 - D B0: one block per (problem, 64-target tile), largest
+    // problems first, each summing S and D over its own problem's nodes with
+    // one evaluation of the pair's radial part for both
     const int64_t npairs = (int64_t)nq * ne_tot, ntri = npairs * nq;
     if (ntri >= (int64_t)INT32_MAX) return set_error(ctx, INTI_ERR_SIZE, "batch_bdim: too many triplets");
     TripletPattern pat;
     {
-        const double *x1 = (const double *)dB1.p, *x0 = (const double *)dB0.p;
-        double *yt = (double *)dth.p;
-        if (!cplx) {  // real column blocks for the Laplace applies
-            INTI_ALLOC(ctx, dr0, sizeof(double) * nb);
-            INTI_ALLOC(ctx, dr1, sizeof(double) * nb);
-            INTI_ALLOC(ctx, drt, sizeof(double) * nb);
-            hipLaunchKernelGGL(k_real_part, dim3(g1), dim3(256), 0, st, (int64_t)nb, (const double2 *)dB0.p, (double *)dr0.p);
-            hipLaunchKernelGGL(k_real_part, dim3(g1), dim3(256), 0, st, (int64_t)nb, (const double2 *)dB1.p, (double *)dr1.p);
-            hipLaunchKernelGGL(k_real_part, dim3(g1), dim3(256), 0, st, (int64_t)nb, (const double2 *)dth.p, (double *)drt.p);
-            INTI_HIP(ctx, hipGetLastError());
-            x1 = (const double *)dr1.p; x0 = (const double *)dr0.p; yt = (double *)drt.p;
-        }
-        Batch V[2];
-        // declared after V, so destroyed first: V must not free B's arrays
-        struct Unshare {
-            Batch *v;
-            ~Unshare() {
-                for (int q = 0; q < 2; ++q) {
-                    v[q].coords = v[q].normals = v[q].weights = nullptr;
-                    v[q].row2p = nullptr;
-                }
+        const int rl = dim_theta_batch_record(B->pde, dim, ns);
+        if (!rl) return set_error(ctx, INTI_ERR_UNSUPPORTED, "batch_bdim: no Theta kernel for this operator");
+        std::vector<int> order(P);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return B->off[a + 1] - B->off[a] > B->off[b + 1] - B->off[b];
+        });
+        std::vector<int32_t> items;
+        for (int p : order)
+            for (int64_t t = 0; t < (B->off[p + 1] - B->off[p] + 63) / 64; ++t) {
+                items.push_back(p);
+                items.push_back((int32_t)t);
             }
-        } unshare{V};
-        for (int pass = 0; pass < 2; ++pass) {
-            Batch &W = V[pass];
-            const int kind = pass == 0 ? INTI_SL : INTI_DL;
-            W.ctx = ctx; W.pde = B->pde; W.kind = kind; W.dim = dim; W.precision = INTI_FP64;
-            W.nprob = P; W.ntot = ntot; W.off = B->off;
-            W.coefs.assign(4 * (size_t)P, 0.0);
-            W.cid.assign(2 * (size_t)P, 0.0);
-            W.kp.resize(P);
-            for (int p = 0; p < P; ++p) {
-                Op tmp;
-                tmp.ctx = ctx; tmp.pde = B->pde; tmp.kind = kind; tmp.dim = dim; tmp.precision = INTI_FP64;
-                tmp.k = B->kp[p].k;
-                int ek = kind;
-                W.kp[p] = op_params(&tmp, &ek);
-            }
-            W.coords = B->coords; W.normals = B->normals; W.weights = B->weights; W.row2p = B->row2p;
-            W.chunk_len = B->chunk_len; W.grid = B->grid;
-            W.c_rowptr.resize(P); W.c_colind.resize(P); W.c_val.resize(P);
-            if (hipMalloc((void **)&W.cid_d, sizeof(double) * 2 * P) != hipSuccess)
-                return set_error(ctx, INTI_ERR_NOMEM, "batch_bdim: allocation failed");
-            INTI_HIP(ctx, hipMemsetAsync(W.cid_d, 0, sizeof(double) * 2 * P, st));
-            const int rc = batch_reserve(&W);  // the launch plan (synchronises once)
-            if (rc) return rc;
+        const int64_t nitems = (int64_t)items.size() / 2;
+        std::vector<KParams> tkp(P);
+        std::vector<double> post(4 * (size_t)P);
+        for (int p = 0; p < P; ++p) {
+            tkp[p] = basic_params(B->kp[p].k);
+            basic_post(B->pde, dim, INTI_SL, B->kp[p].k, post[4 * p], post[4 * p + 1]);
+            basic_post(B->pde, dim, INTI_DL, B->kp[p].k, post[4 * p + 2], post[4 * p + 3]);
         }
-        for (int pass = 0; pass < 2; ++pass) {
-            const double alpha[2] = {pass == 0 ? 1.0 : -1.0, 0.0}, beta[2] = {1.0, 0.0};
-            const int rc = batch_apply_dev(&V[pass], ns, pass == 0 ? x1 : x0, ntot, yt, ntot, alpha, beta);
-            if (rc) {
-                (void)hipStreamSynchronize(st);
-                return rc;
-            }
-        }
+        INTI_ALLOC(ctx, drec, sizeof(double) * (size_t)ntot * rl);
+        INTI_ALLOC(ctx, ditems, sizeof(int32_t) * items.size());
+        INTI_ALLOC(ctx, doff, sizeof(int64_t) * (P + 1));
+        INTI_ALLOC(ctx, dtkp, sizeof(KParams) * P);
+        INTI_ALLOC(ctx, dpost, sizeof(double) * post.size());
+        INTI_HIP(ctx, hipMemcpyAsync(ditems.p, items.data(), sizeof(int32_t) * items.size(), hipMemcpyHostToDevice, st));
+        INTI_HIP(ctx, hipMemcpyAsync(doff.p, B->off.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
+        INTI_HIP(ctx, hipMemcpyAsync(dtkp.p, tkp.data(), sizeof(KParams) * P, hipMemcpyHostToDevice, st));
+        INTI_HIP(ctx, hipMemcpyAsync(dpost.p, post.data(), sizeof(double) * post.size(), hipMemcpyHostToDevice, st));
+        ThetaBatchArgs a;
+        a.ntot = ntot; a.ns = ns;
+        a.items = (const int32_t *)ditems.p; a.off = (const int64_t *)doff.p; a.row2p = B->row2p;
+        a.kp = (const KParams *)dtkp.p; a.post = (const double *)dpost.p;
+        a.coords = B->coords; a.normals = B->normals; a.weights = B->weights;
+        a.B0 = (const double2 *)dB0.p; a.B1 = (const double2 *)dB1.p;
+        a.rec = (double *)drec.p; a.theta = (double2 *)dth.p;
+        if (!launch_dim_theta_batch(B->pde, dim, a, nitems, st))
+            return set_error(ctx, INTI_ERR_UNSUPPORTED, "batch_bdim: no Theta kernel for this operator");
+        INTI_HIP(ctx, hipGetLastError());
         // the triplets (target qt[e*nq + t], node qt[e*nq + r]) and their
         // block-diagonal CSR pattern, on the host while the device sums
         {
@@ -1121,11 +1096,6 @@ static int batch_bdim_build(Batch *B, int nq, const int64_t *ne, const int64_t *
                         tri_col[q] = qt[e * nq + r];
                     }
             build_pattern(ntot, tri_row, tri_col, pat);
-        }
-        if (hipStreamSynchronize(st) != hipSuccess) return set_error(ctx, INTI_ERR_HIP, "batch_bdim: Theta apply failed");
-        if (!cplx) {
-            hipLaunchKernelGGL(k_widen, dim3(g1), dim3(256), 0, st, (int64_t)nb, (const double *)drt.p, (double2 *)dth.p);
-            INTI_HIP(ctx, hipGetLastError());
         }
     }
     // least squares of every element: the element's own nodes are its targets

@@ -87,38 +87,48 @@ struct ThetaPackArgs {
     double psr, psi, pdr, pdi;  // post factors of the S and D kernels
 };
 
+// column kk of node j's record (kk == nst: the node's coordinates and
+// normal); node arrays and trace columns have leading dimension ld
+template <int PDE, int DIM>
+__device__ __forceinline__ void theta_pack_one(int64_t j, int kk, int64_t ld, int ns, int nst, double *R,
+                                               const double *sx, const double *sn, const double *sw,
+                                               const double2 *B0, const double2 *B1, double psr,
+                                               double psi, double pdr, double pdi) {
+    if (kk == nst) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            R[d] = sx[d * ld + j];
+            R[DIM + d] = sn[d * ld + j];
+        }
+        return;
+    }
+    double2 b1 = make_double2(0.0, 0.0), b0 = b1;
+    if (kk < ns) {
+        b1 = B1[j + ld * kk];
+        b0 = B0[j + ld * kk];
+    }
+    const double w = sw[j];
+    double *q = R + 2 * DIM + theta_per_k<PDE>() * kk;
+    if constexpr (PDE == INTI_LAPLACE) {
+        q[0] = psr * (w * b1.x);
+        q[1] = -pdr * (w * b0.x);
+    } else {
+        const double s1r = w * b1.x, s1i = w * b1.y, s0r = w * b0.x, s0i = w * b0.y;
+        q[0] = psr * s1r - psi * s1i;
+        q[1] = psr * s1i + psi * s1r;
+        q[2] = -(pdr * s0r - pdi * s0i);
+        q[3] = -(pdr * s0i + pdi * s0r);
+    }
+}
+
 template <int PDE, int DIM>
 __global__ __launch_bounds__(256) void k_theta_pack(ThetaPackArgs a) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= a.n * (a.nst + 1)) return;
     const int64_t j = e % a.n;
     const int kk = (int)(e / a.n);
-    double *R = a.rec + j * a.rl;
-    if (kk == a.nst) {
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            R[d] = a.sx[d * a.n + j];
-            R[DIM + d] = a.sn[d * a.n + j];
-        }
-        return;
-    }
-    double2 b1 = make_double2(0.0, 0.0), b0 = b1;
-    if (kk < a.ns) {
-        b1 = a.B1[j + a.n * kk];
-        b0 = a.B0[j + a.n * kk];
-    }
-    const double w = a.sw[j];
-    double *q = R + 2 * DIM + theta_per_k<PDE>() * kk;
-    if constexpr (PDE == INTI_LAPLACE) {
-        q[0] = a.psr * (w * b1.x);
-        q[1] = -a.pdr * (w * b0.x);
-    } else {
-        const double s1r = w * b1.x, s1i = w * b1.y, s0r = w * b0.x, s0i = w * b0.y;
-        q[0] = a.psr * s1r - a.psi * s1i;
-        q[1] = a.psr * s1i + a.psi * s1r;
-        q[2] = -(a.pdr * s0r - a.pdi * s0i);
-        q[3] = -(a.pdr * s0i + a.pdi * s0r);
-    }
+    theta_pack_one<PDE, DIM>(j, kk, a.n, a.ns, a.nst, a.rec + j * a.rl, a.sx, a.sn, a.sw, a.B0, a.B1, a.psr,
+                             a.psi, a.pdr, a.pdi);
 }
 
 struct ThetaArgs {
@@ -151,38 +161,24 @@ __device__ __forceinline__ void theta_pair(const double *r, const double *nx, co
     }
 }
 
+template <int PDE>
+constexpr int theta_accs(int ns) { return PDE != INTI_LAPLACE ? 2 * ns : ns; }  // accumulators per lane
+
+// acc += the (S, D) sums of one target over the wavefront's share of the
+// sources [b0, b1) of a record array (kThetaWaves shares, in wave order)
 template <int PDE, int DIM, int DERIV, int NS>
-__global__ __launch_bounds__(64 * kThetaWaves) void k_theta(ThetaArgs a) {
+__device__ __forceinline__ void theta_sources(const double *xt, const double *nt, const double *recp, int rl,
+                                              int64_t b0, int64_t b1, int wave, const KParams &p,
+                                              double *acc) {
     constexpr int KS = DERIV ? INTI_ADL : INTI_SL;
     constexpr int KD = DERIV ? INTI_HS : INTI_DL;
     constexpr int PK = theta_per_k<PDE>();
-    constexpr int NA = PDE != INTI_LAPLACE ? 2 * NS : NS;  // accumulators per lane
-    constexpr int NR = (NA + kThetaRed - 1) / kThetaRed;
-    __shared__ double s_red[kThetaWaves - 1][kThetaRed][64];
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t chunk = blockIdx.x % a.split, tile = blockIdx.x / a.split;
-    const int64_t l = tile * 64 + lane;
-    const int64_t lc = l < a.nloc ? l : a.nloc - 1;
-    const int64_t i = a.glob[lc];
-    double xt[DIM], nt[DIM];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        xt[d] = a.tx[d * a.m + i];
-        nt[d] = a.tn ? a.tn[d * a.m + i] : 0.0;
-    }
-    double acc[NA];
-#pragma unroll
-    for (int q = 0; q < NA; ++q) acc[q] = 0.0;
-    // this wavefront's sources: a quarter of the block's chunk
-    const int64_t b0 = chunk * a.chunk < a.n ? chunk * a.chunk : a.n;
-    const int64_t b1 = b0 + a.chunk < a.n ? b0 + a.chunk : a.n;
     const int64_t share = (b1 - b0 + kThetaWaves - 1) / kThetaWaves;
     const int64_t j0 = b0 + wave * share;
     const int64_t j1 = j0 + share < b1 ? j0 + share : b1;
-    const uniform_double *rec = (const uniform_double *)a.rec;
+    const uniform_double *rec = (const uniform_double *)recp;
     for (int64_t j = j0; j < j1; ++j) {
-        const uniform_double *R = rec + j * a.rl;
+        const uniform_double *R = rec + j * rl;
         double r[DIM], ny[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
@@ -190,7 +186,7 @@ __global__ __launch_bounds__(64 * kThetaWaves) void k_theta(ThetaArgs a) {
             ny[d] = R[DIM + d];
         }
         double sr, si, dr, di;
-        theta_pair<PDE, DIM, KS, KD>(r, nt, ny, a.p, sr, si, dr, di);
+        theta_pair<PDE, DIM, KS, KD>(r, nt, ny, p, sr, si, dr, di);
         const uniform_double *Q = R + 2 * DIM;
         if constexpr (PDE == INTI_LAPLACE) {
 #pragma unroll
@@ -214,7 +210,13 @@ __global__ __launch_bounds__(64 * kThetaWaves) void k_theta(ThetaArgs a) {
             }
         }
     }
-    // add the waves' sums in wave order, kThetaRed accumulators per round
+}
+
+// adds the wavefronts' sums into wave 0's, in wave order, kThetaRed
+// accumulators per round
+template <int NA>
+__device__ __forceinline__ void theta_wave_sum(double *acc, double (*s_red)[kThetaRed][64], int wave, int lane) {
+    constexpr int NR = (NA + kThetaRed - 1) / kThetaRed;
 #pragma unroll
     for (int rd = 0; rd < NR; ++rd) {
         if (wave > 0) {
@@ -235,6 +237,32 @@ __global__ __launch_bounds__(64 * kThetaWaves) void k_theta(ThetaArgs a) {
         }
         __syncthreads();
     }
+}
+
+template <int PDE, int DIM, int DERIV, int NS>
+__global__ __launch_bounds__(64 * kThetaWaves) void k_theta(ThetaArgs a) {
+    constexpr int NA = theta_accs<PDE>(NS);
+    __shared__ double s_red[kThetaWaves - 1][kThetaRed][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t chunk = blockIdx.x % a.split, tile = blockIdx.x / a.split;
+    const int64_t l = tile * 64 + lane;
+    const int64_t lc = l < a.nloc ? l : a.nloc - 1;
+    const int64_t i = a.glob[lc];
+    double xt[DIM], nt[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        xt[d] = a.tx[d * a.m + i];
+        nt[d] = a.tn ? a.tn[d * a.m + i] : 0.0;
+    }
+    double acc[NA];
+#pragma unroll
+    for (int q = 0; q < NA; ++q) acc[q] = 0.0;
+    // this wavefront's sources: a quarter of the block's chunk
+    const int64_t b0 = chunk * a.chunk < a.n ? chunk * a.chunk : a.n;
+    const int64_t b1 = b0 + a.chunk < a.n ? b0 + a.chunk : a.n;
+    theta_sources<PDE, DIM, DERIV, NS>(xt, nt, a.rec, a.rl, b0, b1, wave, a.p, acc);
+    theta_wave_sum<NA>(acc, s_red, wave, lane);
     if (wave != 0 || l >= a.nloc) return;
     if (a.split == 1) {
         const double mu = a.mu[i];
@@ -772,6 +800,91 @@ bool launch_dim_traces_batch(int pde, int dim, const TraceBatchArgs &a, hipStrea
     else if (pde == INTI_HELMHOLTZ && dim == 3) hipLaunchKernelGGL((k_traces_batch<INTI_HELMHOLTZ, 3>), g, dim3(256), 0, s, a);
     else return false;
     return true;
+}
+
+template <int PDE, int DIM>
+__global__ __launch_bounds__(256) void k_theta_pack_batch(ThetaBatchArgs a, int nst, int rl) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= a.ntot * (nst + 1)) return;
+    const int64_t j = e % a.ntot;
+    const int kk = (int)(e / a.ntot);
+    const double *pf = a.post + 4 * (int64_t)a.row2p[j];
+    theta_pack_one<PDE, DIM>(j, kk, a.ntot, a.ns, nst, a.rec + j * rl, a.coords, a.normals, a.weights, a.B0,
+                             a.B1, pf[0], pf[1], pf[2], pf[3]);
+}
+
+template <int PDE, int DIM, int NS>
+__global__ __launch_bounds__(64 * kThetaWaves) void k_theta_batch(ThetaBatchArgs a, int rl) {
+    constexpr int NA = theta_accs<PDE>(NS);
+    __shared__ double s_red[kThetaWaves - 1][kThetaRed][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int p = a.items[2 * (int64_t)blockIdx.x], tile = a.items[2 * (int64_t)blockIdx.x + 1];
+    const int64_t off = a.off[p], n = a.off[p + 1] - off;
+    const int64_t l = (int64_t)tile * 64 + lane;
+    const int64_t i = off + (l < n ? l : n - 1);
+    double xt[DIM], nt[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        xt[d] = a.coords[d * a.ntot + i];
+        nt[d] = a.normals[d * a.ntot + i];
+    }
+    double acc[NA];
+#pragma unroll
+    for (int q = 0; q < NA; ++q) acc[q] = 0.0;
+    theta_sources<PDE, DIM, 0, NS>(xt, nt, a.rec + off * rl, rl, 0, n, wave, a.kp[p], acc);
+    theta_wave_sum<NA>(acc, s_red, wave, lane);
+    if (wave != 0 || l >= n) return;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        if (k >= a.ns) break;
+        double2 *t = a.theta + (off + l) + a.ntot * k;
+        const double2 v = *t;
+        *t = PDE != INTI_LAPLACE ? make_double2(v.x + acc[2 * k], v.y + acc[2 * k + 1])
+                                 : make_double2(v.x + acc[k], v.y);
+    }
+}
+
+template <int PDE, int DIM, int NS>
+static void theta_batch_launch(const ThetaBatchArgs &a, int64_t nitems, hipStream_t s) {
+    const int rl = 2 * DIM + theta_per_k<PDE>() * NS;
+    hipLaunchKernelGGL((k_theta_pack_batch<PDE, DIM>), dim3((unsigned)((a.ntot * (NS + 1) + 255) / 256)),
+                       dim3(256), 0, s, a, NS, rl);
+    hipLaunchKernelGGL((k_theta_batch<PDE, DIM, NS>), dim3((unsigned)nitems), dim3(64 * kThetaWaves), 0, s, a,
+                       rl);
+}
+
+static int theta_nst(int ns) {
+    for (int c : kThetaNs)
+        if (c >= ns) return c;
+    return 0;
+}
+
+int dim_theta_batch_record(int pde, int dim, int ns) {
+    const int nst = theta_nst(ns);
+    if (!nst || (pde != INTI_LAPLACE && pde != INTI_HELMHOLTZ) || (dim != 2 && dim != 3)) return 0;
+    return 2 * dim + (pde != INTI_LAPLACE ? 4 : 2) * nst;
+}
+
+template <int PDE, int DIM>
+static bool theta_batch_ns(const ThetaBatchArgs &a, int64_t nitems, hipStream_t s) {
+#define INTI_T(N) \
+    case N: theta_batch_launch<PDE, DIM, N>(a, nitems, s); return true;
+    switch (theta_nst(a.ns)) {
+        INTI_T(6) INTI_T(9) INTI_T(12) INTI_T(15) INTI_T(18) INTI_T(21) INTI_T(24) INTI_T(27) INTI_T(30)
+        INTI_T(32)
+    }
+#undef INTI_T
+    return false;
+}
+
+bool launch_dim_theta_batch(int pde, int dim, const ThetaBatchArgs &a, int64_t nitems, hipStream_t s) {
+    if (nitems <= 0) return true;
+    if (pde == INTI_LAPLACE && dim == 2) return theta_batch_ns<INTI_LAPLACE, 2>(a, nitems, s);
+    if (pde == INTI_LAPLACE && dim == 3) return theta_batch_ns<INTI_LAPLACE, 3>(a, nitems, s);
+    if (pde == INTI_HELMHOLTZ && dim == 2) return theta_batch_ns<INTI_HELMHOLTZ, 2>(a, nitems, s);
+    if (pde == INTI_HELMHOLTZ && dim == 3) return theta_batch_ns<INTI_HELMHOLTZ, 3>(a, nitems, s);
+    return false;
 }
 
 void launch_dim_ls(const LsArgs &a, int64_t nel, hipStream_t s) {

@@ -228,4 +228,26 @@ struct TraceBatchArgs {
 // false when there is no kernel for (pde, dim)
 bool launch_dim_traces_batch(int pde, int dim, const TraceBatchArgs &a, hipStream_t s);
 
+// Theta += S B1 - D B0 for every problem of a batch (the S and D sums of
+// each problem over its own nodes): the per-node records are packed in one
+// launch, then one block per work item (problem, tile of 64 targets) sums
+// over all the problem's sources, its wavefronts taking shares of them.
+struct ThetaBatchArgs {
+    int64_t ntot;
+    int ns;
+    const int32_t *items;  // nitems x (problem, tile), by decreasing problem size
+    const int64_t *off;    // problem p's nodes are rows [off[p], off[p + 1])
+    const int32_t *row2p;
+    const KParams *kp;     // per problem: the kernel parameters of S and D
+    const double *post;    // per problem: post factors (re, im) of S, then of D
+    const double *coords, *normals, *weights;
+    const double2 *B0, *B1;  // ntot x ns traces, column-major
+    double *rec;             // ntot x record length, written by the pack
+    double2 *theta;          // ntot x ns, column-major
+};
+// record length in doubles per node (to size `rec`), or 0 when there is no
+// kernel for (pde, dim, ns)
+int dim_theta_batch_record(int pde, int dim, int ns);
+bool launch_dim_theta_batch(int pde, int dim, const ThetaBatchArgs &a, int64_t nitems, hipStream_t s);
+
 }  // namespace inti

@@ -214,8 +214,8 @@ def test_batch_error_paths(inti):
 @pytest.mark.parametrize("case", ["helm2", "lap2", "helm3"])
 def test_batch_bdim_matches_per_problem_builds(inti, oracle, case):
     """The DIM corrections of all problems at once (traces in one launch,
-    Theta by two batched applies, the least squares of all elements in one
-    launch) against the oracle's build of each problem: pattern bit-exact;
+    Theta in one launch over (problem, target tile) work items, the least
+    squares of all elements in one launch) against the oracle's build of each problem: pattern bit-exact;
     the corrected operator on a smooth density within 1e-12 of the oracle's
     (or 10x the oracle's own rounding spread, with and without FMA), and the
     weights within the spread the single-problem build is held to."""
