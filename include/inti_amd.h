@@ -241,7 +241,6 @@ int inti_batch_destroy(inti_batch *batch);
 int inti_batch_info(const inti_batch *batch, int64_t *ntot, int64_t *offsets);
 /* identity coefficients, 2 doubles (re, im) per problem */
 int inti_batch_set_identity(inti_batch *batch, const double *cid);
-/* add coef * csr (problem-local indices, n_p x n_p) to problem p */
 /* The DIM corrections (dS, dD) of every problem of the batch, as
  * bdim_correction (src/bdim.jl:50-196) gives them for each problem's
  * quadrature with itself (the self pattern, one element type of nq nodes):
@@ -254,6 +253,7 @@ int inti_batch_set_identity(inti_batch *batch, const double *cid);
  * loops over problems). */
 int inti_batch_bdim_build(inti_batch *batch, int nq, const int64_t *ne, const int64_t *qtags,
                           const double *mu, const double *params, inti_csr **dS, inti_csr **dD);
+/* add coef * csr (problem-local indices, n_p x n_p) to problem p */
 int inti_batch_attach_correction(inti_batch *batch, int p, const inti_csr *csr,
                                  double coef_re, double coef_im);
 /* source chunk length of a work item (0 = 2048) and persistent grid size

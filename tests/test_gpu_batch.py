@@ -211,7 +211,7 @@ def test_batch_error_paths(inti):
         inti.BatchOperator([inti.DoubleLayerKernel(op), inti.SingleLayerKernel(op)], [Q, Q])
 
 
-@pytest.mark.parametrize("case", ["helm2", "lap2", "helm3"])
+@pytest.mark.parametrize("case", ["helm2", "lap2", "helm3", "lap3"])
 def test_batch_bdim_matches_per_problem_builds(inti, oracle, case):
     """The DIM corrections of all problems at once (traces in one launch,
     Theta in one launch over (problem, target tile) work items, the least
@@ -227,6 +227,9 @@ def test_batch_bdim_matches_per_problem_builds(inti, oracle, case):
     elif case == "helm2":
         qs, ks = _ellipses(inti, [9, 60, 23, 41], seed=7)
         ops = [inti.Helmholtz(k=k, dim=2) for k in ks]
+    elif case == "lap3":
+        qs = [inti.sphere_quadrature(f) for f in (3, 1, 2)]
+        ops = [inti.Laplace(dim=3) for _ in qs]
     else:
         qs = [inti.sphere_quadrature(f) for f in (2, 3, 1)]
         ops = [inti.Helmholtz(k=k, dim=3) for k in (1.5, 2.5, 0.7)]
