@@ -13,6 +13,8 @@
 // few steps. No memset is recorded in the graph: clearing is done by kernels.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <complex>
 #include <vector>
 
@@ -393,6 +395,9 @@ static int gmres_core(Op *op, Op *prec, GmresWork &W, const double *b, double *x
     if (rc) return rc;
     const int saved_prof = ctx->profiling;
     ctx->profiling = 0;  // no event records inside the solver (or its graph)
+    // INTI_GMRES_LOG=N: a progress line on stderr every N Arnoldi steps
+    const char *log_env = std::getenv("INTI_GMRES_LOG");
+    const int log_every = log_env ? std::max(0, std::atoi(log_env)) : 0;
     while (rn > tol_abs && it < maxiter) {
         // V[:,0] = r / rn; g = (rn, 0, ...); H = 0; k = 0
         hipLaunchKernelGGL(k_gaxpby<CPLX>, gv, dim3(kGB), 0, s, n, 1.0 / rn, (const T *)W.r, 0.0, (T *)W.V);
@@ -434,6 +439,9 @@ static int gmres_core(Op *op, Op *prec, GmresWork &W, const double *b, double *x
             INTI_HIP(ctx, hipMemcpyAsync(res.data(), W.res, sizeof(double) * k, hipMemcpyDeviceToHost, s));
             INTI_HIP(ctx, hipStreamSynchronize(s));
             kk = k;
+            if (log_every > 0 && (it + k) / log_every != (it + k - steps) / log_every)
+                std::fprintf(stderr, "gmres: step %d, residual estimate %.3e (target %.3e)\n", it + k, res[k - 1],
+                             tol_abs);
             for (int q = k - steps; q < k; ++q)
                 if (res[q] <= tol_abs) { kk = q + 1; break; }
             if (kk < k || res[k - 1] <= tol_abs) break;
