@@ -237,10 +237,54 @@ __device__ __forceinline__ Helm2Radial helm2_radial(double d2s, const KParams &p
     return R;
 }
 
+// MIXED: the Hankel values are FP32 (widened into R; narrowing them back is
+// exact), so the geometric factors are rounded to FP32 once and the
+// combination is done in FP32
+template <int KIND>
+__device__ __forceinline__ void helm2_combine32(const double *r, bool ok, const double *nx,
+                                                const double *ny, const KParams &p,
+                                                const Helm2Radial &R, double &kr, double &ki) {
+    const double id = R.id;
+    const float h0r = (float)R.h0r, h0i = (float)R.h0i, h1r = (float)R.h1r, h1i = (float)R.h1i;
+    float zr, zi;
+    if constexpr (KIND == INTI_SL) {
+        zr = h0r; zi = h0i;
+    } else if constexpr (KIND == INTI_DL || KIND == INTI_ADL) {
+        const float f = (float)(id * dotv<2>(r, KIND == INTI_DL ? ny : nx));
+        zr = f * h1r; zi = f * h1i;
+    } else if constexpr (KIND == INTI_COMB_SD) {  // rdny id H1 + ch H0
+        const float f = (float)(id * dotv<2>(r, ny));
+        const float chr = (float)p.chr, chi = (float)p.chi;
+        zr = f * h1r + chr * h0r - chi * h0i;
+        zi = f * h1i + chr * h0i + chi * h0r;
+    } else {
+        const float ix = (float)rcpv<true>(R.x);
+        const float h2r = 2.0f * ix * h1r - h0r, h2i = 2.0f * ix * h1i - h0i;
+        const double rdnx = dotv<2>(r, nx);
+        const float fa = (float)(-p.k * id * id * rdnx * dotv<2>(r, ny));
+        const float fb = (float)(id * dotv<2>(nx, ny));
+        if constexpr (KIND == INTI_HS) {
+            zr = fa * h2r + fb * h1r; zi = fa * h2i + fb * h1i;
+        } else {  // HS raw - ch * rdnx id H1
+            const float fc = (float)(-rdnx * id);
+            const float chr = (float)p.chr, chi = (float)p.chi;
+            const float ar = fc * h1r, ai = fc * h1i;
+            zr = fa * h2r + fb * h1r + chr * ar - chi * ai;
+            zi = fa * h2i + fb * h1i + chr * ai + chi * ar;
+        }
+    }
+    kr = (double)(ok ? zr : 0.0f);
+    ki = (double)(ok ? zi : 0.0f);
+}
+
 template <int KIND, bool MIXED>
 __device__ __forceinline__ void helm2_combine(const double *r, bool ok, const double *nx,
                                               const double *ny, const KParams &p,
                                               const Helm2Radial &R, double &kr, double &ki) {
+    if constexpr (MIXED) {
+        helm2_combine32<KIND>(r, ok, nx, ny, p, R, kr, ki);
+        return;
+    }
     const double k = p.k, id = R.id;
     const double h0r = R.h0r, h0i = R.h0i, h1r = R.h1r, h1i = R.h1i;
     if constexpr (KIND == INTI_SL) {
