@@ -203,10 +203,12 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
     // the plain pair
     constexpr bool FUSED =
         MIXED && PDE == INTI_HELMHOLTZ && DIM == 3 && (KIND == INTI_DL || KIND == INTI_COMB_SD);
-    // mixed 3D Laplace SL: 1/d from the FP32 unit, its products and 64-term
-    // running sums in FP32 added into FP64 after every pass, as above (the
-    // distance itself stays FP64)
-    constexpr bool FUSEDL = MIXED && PDE == INTI_LAPLACE && DIM == 3 && KIND == INTI_SL;
+    // mixed 3D Laplace SL, DL, ADL and DL + ch SL: 1/d from the FP32 unit,
+    // the values, their products and the 64-term running sums in FP32, added
+    // into FP64 after every pass, as above (the distance and the normal dot
+    // products stay FP64)
+    constexpr bool FUSEDL = MIXED && PDE == INTI_LAPLACE && DIM == 3 &&
+                            (KIND == INTI_SL || KIND == INTI_DL || KIND == INTI_ADL || KIND == INTI_COMB_SD);
     using R = std::conditional_t<FUSED, SymRecF<CHIK>, SymRec<PDE, DIM, KIND>>;
     constexpr bool CPLX = PDE == INTI_HELMHOLTZ;
     constexpr int BS = kSymBS / TPL;  // threads
@@ -368,6 +370,7 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                                 s1[t] = 0.0f;
                                 qf[t] = (float)qir[t];
                             }
+                            const float ch = (float)p.chr;
                             float ca = 0.0f, cb = 0.0f;
                             auto one = [&](int jl, float &aj) {
                                 double yj[3], nj[3], qjr, qji;
@@ -381,8 +384,31 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                                     const double d2 = dotv<3>(r, r);
                                     float y = __builtin_amdgcn_rsqf(MASK ? fmaxf((float)d2, 1.0e-30f) : (float)d2);
                                     if constexpr (MASK) y = d2 > kSamePointTol2 ? y : 0.0f;
-                                    s1[t] = __builtin_fmaf(y, qj, s1[t]);  // K(x_i, x_j) q_j
-                                    aj = __builtin_fmaf(y, qf[t], aj);     // K(x_j, x_i) q_i
+                                    // k1 = K(x_i, x_j), k2 = K(x_j, x_i); r = x_i - x_j
+                                    float k1 = y, k2 = y;
+                                    if constexpr (KIND != INTI_SL) {
+                                        // r . n in FP64: on a smooth surface it is O(|r|^2) for
+                                        // close nodes, and an FP32 dot of O(|r|) terms would lose
+                                        // digits in proportion to 1/|r|
+                                        const float y3 = y * y * y;
+                                        const float rj = (float)dotv<3>(r, nj);
+                                        const float ri = (float)dotv<3>(r, ni[t]);
+                                        if constexpr (KIND == INTI_DL) {  // (r . n_source) / d^3
+                                            k1 = rj * y3;
+                                            k2 = -ri * y3;
+                                        } else if constexpr (KIND == INTI_ADL) {  // (r . n_target) / d^3
+                                            k1 = ri * y3;
+                                            k2 = -rj * y3;
+                                        } else {  // DL + ch SL
+                                            k1 = __builtin_fmaf(rj, y3, ch * y);
+                                            k2 = __builtin_fmaf(-ri, y3, ch * y);
+                                        }
+                                        if constexpr (TRANS) {
+                                            const float tk = k1; k1 = k2; k2 = tk;
+                                        }
+                                    }
+                                    s1[t] = __builtin_fmaf(k1, qj, s1[t]);  // K(x_i, x_j) q_j
+                                    aj = __builtin_fmaf(k2, qf[t], aj);     // K(x_j, x_i) q_i
                                 }
                             };
                             for (int s = 0; s < 32; ++s) {
