@@ -51,6 +51,28 @@ def test_gmres_fp64_matches_dense_solve(inti, cfie, use_graph):
     assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) <= 2e-13
 
 
+def test_gmres_without_restarts_and_progress_log(inti, cfie, capfd, monkeypatch):
+    """restart = maxiter (one Krylov basis for the whole solve) reaches the
+    dense solution, and INTI_GMRES_LOG=N prints a progress line on stderr
+    every N Arnoldi steps (read at each solve)."""
+    say("[gpu] gmres without restarts, progress log")
+    Q, L64, Lmx, A, b, x_ref = cfie
+    monkeypatch.setenv("INTI_GMRES_LOG", "8")
+    x, info = inti.gmres(L64, b, restart=200, maxiter=200, rtol=1e-13)
+    err = capfd.readouterr().err
+    assert info["relres"] <= 1e-13
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= 1e-10
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) <= 2e-13
+    steps = [int(line.split()[2].rstrip(",")) for line in err.splitlines() if line.startswith("gmres: step")]
+    assert steps and all(s % 8 == 0 for s in steps) and steps == sorted(steps)
+    # a line per 8-step check of the residual estimate; the solve stops
+    # inside the last checked block
+    assert steps[-1] - 8 < info["iterations"] <= steps[-1]
+    monkeypatch.delenv("INTI_GMRES_LOG")
+    inti.gmres(L64, b, restart=200, maxiter=16, rtol=1e-13)
+    assert "gmres: step" not in capfd.readouterr().err
+
+
 def test_gmres_graph_replay_is_deterministic(inti, cfie):
     say("[gpu] gmres graph vs eager")
     Q, L64, Lmx, A, b, x_ref = cfie
