@@ -203,12 +203,11 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
     // the plain pair
     constexpr bool FUSED =
         MIXED && PDE == INTI_HELMHOLTZ && DIM == 3 && (KIND == INTI_DL || KIND == INTI_COMB_SD);
-    // mixed 3D Laplace SL, DL, ADL and DL + ch SL: 1/d from the FP32 unit,
+    // mixed 3D Laplace (every kind): 1/d from the FP32 unit,
     // the values, their products and the 64-term running sums in FP32, added
     // into FP64 after every pass, as above (the distance and the normal dot
     // products stay FP64)
-    constexpr bool FUSEDL = MIXED && PDE == INTI_LAPLACE && DIM == 3 &&
-                            (KIND == INTI_SL || KIND == INTI_DL || KIND == INTI_ADL || KIND == INTI_COMB_SD);
+    constexpr bool FUSEDL = MIXED && PDE == INTI_LAPLACE && DIM == 3;
     using R = std::conditional_t<FUSED, SymRecF<CHIK>, SymRec<PDE, DIM, KIND>>;
     constexpr bool CPLX = PDE == INTI_HELMHOLTZ;
     constexpr int BS = kSymBS / TPL;  // threads
@@ -399,9 +398,21 @@ __global__ __launch_bounds__(kSymBS / TPL) void k_apply_sym(SymArgs A) {
                                         } else if constexpr (KIND == INTI_ADL) {  // (r . n_target) / d^3
                                             k1 = ri * y3;
                                             k2 = -rj * y3;
-                                        } else {  // DL + ch SL
+                                        } else if constexpr (KIND == INTI_COMB_SD) {  // DL + ch SL
                                             k1 = __builtin_fmaf(rj, y3, ch * y);
                                             k2 = __builtin_fmaf(-ri, y3, ch * y);
+                                        } else {
+                                            // HS: (n_i . n_j - 3 (r . n_i)(r . n_j) / d^2) / d^3, the
+                                            // same both ways; HS + ch ADL adds ch (r . n_target) / d^3
+                                            const float nn = (float)dotv<3>(ni[t], nj);
+                                            const float hs = y3 * __builtin_fmaf(-3.0f * ri * rj, y * y, nn);
+                                            if constexpr (KIND == INTI_HS) {
+                                                k1 = hs;
+                                                k2 = hs;
+                                            } else {
+                                                k1 = __builtin_fmaf(ch * ri, y3, hs);
+                                                k2 = __builtin_fmaf(-ch * rj, y3, hs);
+                                            }
                                         }
                                         if constexpr (TRANS) {
                                             const float tk = k1; k1 = k2; k2 = tk;
